@@ -1,0 +1,604 @@
+// adg_capi.hip -- implementation of the C-ABI declared in include/adg.h.
+//
+// The context owns the element tables (one read-only device buffer), the
+// flattened hierarchy lists and every field buffer.  All launches go to the
+// context stream; the only host synchronisations are the explicit
+// readbacks.  Time t and step size Dt live on the device (TimeState) so a
+// whole step can be replayed without host round trips.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "adg.h"
+#include "adg_hier.hpp"
+#include "adg_kernels.hpp"
+#include "adg_tables.hpp"
+
+namespace adg {
+void project_ic(const ElementTables& t, const Hierarchy& H, int lev, int ic_case, const double* prm, double gamma,
+                std::vector<double>& out);
+}
+
+struct adg_ctx {
+    int D = 2, P = 1, Q = 4, precision = 0;
+    double gamma = 1.4;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    adg::ElementTables tab;
+    adg::TabOff to{};
+    double* d_tab = nullptr;
+
+    bool geo_set = false, levels_set = false;
+    int dom0[3] = {1, 1, 1};
+    double plo[3] = {0, 0, 0}, phi[3] = {1, 1, 1};
+    int per[3] = {1, 1, 1};
+    adg::Hierarchy H;
+    adg::LevelGeo geo{};
+
+    double *Uw = nullptr, *Vol = nullptr, *Tr = nullptr, *Hw_dbg = nullptr, *Mom = nullptr, *partial = nullptr;
+    double* Fnum[3] = {nullptr, nullptr, nullptr};
+    int* valid_list = nullptr;
+    int* cell_face[3][2] = {};
+    int* face_lr[3] = {};
+    int *g_slot = nullptr, *g_src = nullptr;
+    uint8_t *g_kind = nullptr, *g_child = nullptr, *slot_lev = nullptr;
+    int* ad_coarse[adg::kMaxLev] = {};
+    int* ad_fine[adg::kMaxLev] = {};
+    int *mom_face = nullptr, *rc_slot = nullptr, *rc_start = nullptr, *rc_count = nullptr, *re_mom = nullptr;
+    uint8_t *mom_d = nullptr, *mom_kind = nullptr, *mom_sel = nullptr, *mom_lev = nullptr, *rc_lev = nullptr;
+    double* re_coef = nullptr;
+    int* uncovered = nullptr;
+    adg::TimeState* ts = nullptr;
+    bool keep_hw = false;
+    std::string err;
+    std::vector<void*> level_allocs;
+};
+
+namespace {
+
+int fail(adg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, ADG_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dev_upload(adg_ctx* c, T** dst, const std::vector<T>& src) {
+    *dst = nullptr;
+    if (src.empty()) return ADG_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, src.size() * sizeof(T));
+    if (e != hipSuccess) return fail(c, ADG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->level_allocs.push_back(p);
+    HIPCHK(c, hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = static_cast<T*>(p);
+    return ADG_OK;
+}
+
+int dev_alloc(adg_ctx* c, double** dst, size_t n) {
+    *dst = nullptr;
+    if (n == 0) return ADG_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(double));
+    if (e != hipSuccess) return fail(c, ADG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->level_allocs.push_back(p);
+    HIPCHK(c, hipMemsetAsync(p, 0, n * sizeof(double), c->stream));
+    *dst = static_cast<double*>(p);
+    return ADG_OK;
+}
+
+void free_levels(adg_ctx* c) {
+    for (void* p : c->level_allocs) (void)hipFree(p);
+    c->level_allocs.clear();
+    c->levels_set = false;
+    c->Hw_dbg = nullptr;
+}
+
+bool supported(int P, int D) { return (D == 2 || D == 3) && P >= 1 && P <= 3; }
+
+#define ADG_DISPATCH(c, MACRO)                \
+    switch ((c)->P * 10 + (c)->D) {           \
+        case 12: MACRO(1, 2); break;          \
+        case 22: MACRO(2, 2); break;          \
+        case 32: MACRO(3, 2); break;          \
+        case 13: MACRO(1, 3); break;          \
+        case 23: MACRO(2, 3); break;          \
+        case 33: MACRO(3, 3); break;          \
+        default: return fail(c, ADG_E_UNSUP, "unsupported (p, dim)"); \
+    }
+
+inline unsigned nblocks(long threads, int bs = 256) { return (unsigned)((threads + bs - 1) / bs); }
+
+int pack_tables(adg_ctx* c) {
+    const adg::ElementTables& t = c->tab;
+    std::vector<double> buf;
+    auto add = [&](const std::vector<double>& v) {
+        int off = (int)buf.size();
+        buf.insert(buf.end(), v.begin(), v.end());
+        while (buf.size() % 8) buf.push_back(0.0);  // 64-byte alignment of every table
+        return off;
+    };
+    std::vector<double> mkc(t.Np_s), minv(t.Np_s);
+    for (int k = 0; k < t.Np_s; ++k) {
+        mkc[k] = t.Mk_corr[(size_t)k * t.Np_s + k];
+        minv[k] = t.Minv[(size_t)k * t.Np_s + k];
+    }
+    c->to.A0 = add(t.A0);
+    c->to.B = add(t.B);
+    c->to.V = add(t.V);
+    c->to.Skc = add(t.Sk_corr);
+    c->to.Vbd = add(t.Vbd);
+    c->to.Mkbdm = add(t.Mkbdm);
+    c->to.Mkbdp = add(t.Mkbdp);
+    c->to.Mkc = add(mkc);
+    c->to.Minv = add(minv);
+    c->to.Pcf = add(t.P_cf);
+    c->to.Pfc = add(t.P_fc);
+    c->to.Pfl = add(t.P_flux_low);
+    c->to.Pfh = add(t.P_flux_high);
+    c->to.phi0 = add(t.phi0);
+    c->to.total = (int)buf.size();
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, buf.size() * sizeof(double)));
+    c->d_tab = static_cast<double*>(p);
+    HIPCHK(c, hipMemcpy(c->d_tab, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+    return ADG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void* hip_stream) {
+    if (!out) return ADG_E_ARG;
+    *out = nullptr;
+    if (!supported(p, dim)) return ADG_E_UNSUP;
+    if (precision != ADG_FP64 && precision != ADG_FP32_MFMA) return ADG_E_ARG;
+    if (!(gamma > 1.0)) return ADG_E_ARG;
+    adg_ctx* c = new adg_ctx();
+    c->D = dim;
+    c->P = p;
+    c->Q = dim + 2;
+    c->gamma = gamma;
+    c->precision = precision;
+    try {
+        c->tab = adg::build_element_tables(p, dim);
+    } catch (const std::exception& e) {
+        delete c;
+        return ADG_E_ARG;
+    }
+    if (hip_stream) {
+        c->stream = static_cast<hipStream_t>(hip_stream);
+    } else {
+        if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return ADG_E_HIP; }
+        c->own_stream = true;
+    }
+    int rc = pack_tables(c);
+    if (rc != ADG_OK) { adg_destroy(c); return rc; }
+    void* tsp = nullptr;
+    if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
+    c->ts = static_cast<adg::TimeState*>(tsp);
+    adg::TimeState t0{0.0, 0.0, 0x7ff0000000000000ull, 0.0};
+    if (hipMemcpy(c->ts, &t0, sizeof(t0), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
+    *out = c;
+    return ADG_OK;
+}
+
+int adg_set_geometry(adg_ctx* c, const int* dom_n, const double* prob_lo, const double* prob_hi,
+                     const int* periodic) {
+    if (!c || !dom_n || !prob_lo || !prob_hi || !periodic) return fail(c, ADG_E_ARG, "null argument");
+    for (int d = 0; d < c->D; ++d) {
+        if (dom_n[d] <= 0 || !(prob_hi[d] > prob_lo[d])) return fail(c, ADG_E_ARG, "bad domain");
+        c->dom0[d] = dom_n[d];
+        c->plo[d] = prob_lo[d];
+        c->phi[d] = prob_hi[d];
+        c->per[d] = periodic[d] ? 1 : 0;
+    }
+    c->geo_set = true;
+    return ADG_OK;
+}
+
+int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
+    if (!c) return ADG_E_ARG;
+    if (!c->geo_set) return fail(c, ADG_E_STATE, "adg_set_geometry must come first");
+    if (nlev < 1 || nlev > adg::kMaxLev || !npatch || !boxes) return fail(c, ADG_E_ARG, "bad level arguments");
+    free_levels(c);
+    try {
+        c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes);
+    } catch (const std::exception& e) {
+        return fail(c, ADG_E_ARG, e.what());
+    }
+    adg::Hierarchy& H = c->H;
+    const int D = c->D, Q = c->Q;
+    const adg::ElementTables& t = c->tab;
+    std::memset(&c->geo, 0, sizeof(c->geo));
+    for (int l = 0; l < nlev; ++l) {
+        double vol = 1.0, avg = 0.0;
+        for (int d = 0; d < D; ++d) {
+            c->geo.dx[l][d] = H.lev[l].dx[d];
+            vol *= H.lev[l].dx[d];
+            avg += H.lev[l].dx[d];
+        }
+        for (int d = 0; d < D; ++d) {
+            double a = 1.0;
+            for (int e = 0; e < D; ++e)
+                if (e != d) a *= H.lev[l].dx[e];
+            c->geo.area[l][d] = a;
+        }
+        c->geo.vol[l] = vol;
+        c->geo.dx_avg[l] = avg / (double)D;
+    }
+    int rc;
+#define UP(dst, src) if ((rc = dev_upload(c, &(dst), (src))) != ADG_OK) return rc
+    UP(c->slot_lev, H.slot_lev);
+    UP(c->valid_list, H.valid_list);
+    for (int d = 0; d < D; ++d) {
+        UP(c->cell_face[d][0], H.cell_face[d][0]);
+        UP(c->cell_face[d][1], H.cell_face[d][1]);
+        UP(c->face_lr[d], H.face_lr[d]);
+    }
+    UP(c->g_slot, H.g_slot);
+    UP(c->g_src, H.g_src);
+    UP(c->g_kind, H.g_kind);
+    UP(c->g_child, H.g_child);
+    for (int l = 1; l < nlev; ++l) {
+        UP(c->ad_coarse[l], H.ad_coarse[l]);
+        UP(c->ad_fine[l], H.ad_fine[l]);
+    }
+    UP(c->mom_face, H.mom_face);
+    UP(c->mom_d, H.mom_d);
+    UP(c->mom_kind, H.mom_kind);
+    UP(c->mom_sel, H.mom_sel);
+    UP(c->mom_lev, H.mom_lev);
+    UP(c->rc_slot, H.rc_slot);
+    UP(c->rc_start, H.rc_start);
+    UP(c->rc_count, H.rc_count);
+    UP(c->rc_lev, H.rc_lev);
+    UP(c->re_mom, H.re_mom);
+    UP(c->re_coef, H.re_coef);
+    UP(c->uncovered, H.uncovered);
+#undef UP
+    const size_t NS = (size_t)H.NS;
+    if ((rc = dev_alloc(c, &c->Uw, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
+    if ((rc = dev_alloc(c, &c->Vol, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
+    if ((rc = dev_alloc(c, &c->Tr, (size_t)2 * D * Q * t.qMp_st_bd * NS)) != ADG_OK) return rc;
+    for (int d = 0; d < D; ++d)
+        if ((rc = dev_alloc(c, &c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d])) != ADG_OK) return rc;
+    if ((rc = dev_alloc(c, &c->Mom, (size_t)Q * t.Np_s * H.mom_face.size())) != ADG_OK) return rc;
+    if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
+    if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->levels_set = true;
+    return ADG_OK;
+}
+
+int adg_level_nslot(adg_ctx* c, int lev) {
+    if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev) return ADG_E_ARG;
+    return (int)c->H.lev[lev].nslot;
+}
+
+int adg_level_nface(adg_ctx* c, int lev, int d) {
+    if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev || d < 0 || d >= c->D) return ADG_E_ARG;
+    return (int)c->H.lev[lev].nface[d];
+}
+
+static int copy_level(adg_ctx* c, int lev, double* dev, int ncomp, double* host, bool up) {
+    if (!c) return ADG_E_ARG;
+    if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (lev < 0 || lev >= c->H.nlev || !host || !dev) return fail(c, ADG_E_ARG, "bad level or buffer");
+    const size_t ns = (size_t)c->H.lev[lev].nslot, off = (size_t)c->H.lev[lev].slot_off, NS = (size_t)c->H.NS;
+    const size_t rows = (size_t)c->Q * ncomp;
+    if (up)
+        HIPCHK(c, hipMemcpy2DAsync(dev + off, NS * sizeof(double), host, ns * sizeof(double), ns * sizeof(double),
+                                   rows, hipMemcpyHostToDevice, c->stream));
+    else
+        HIPCHK(c, hipMemcpy2DAsync(host, ns * sizeof(double), dev + off, NS * sizeof(double), ns * sizeof(double),
+                                   rows, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+int adg_upload_Uw(adg_ctx* c, int lev, const double* host) {
+    return copy_level(c, lev, c ? c->Uw : nullptr, c ? c->tab.Np_s : 0, const_cast<double*>(host), true);
+}
+int adg_download_Uw(adg_ctx* c, int lev, double* host) {
+    return copy_level(c, lev, c ? c->Uw : nullptr, c ? c->tab.Np_s : 0, host, false);
+}
+
+int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
+    if (!c) return ADG_E_ARG;
+    if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 3)
+        return fail(c, ADG_E_ARG, "bad level, case or params");
+    if (ic_case <= 1 && c->D != 2) return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
+    std::vector<double> host;
+    adg::project_ic(c->tab, c->H, lev, ic_case, params, c->gamma, host);
+    return copy_level(c, lev, c->Uw, c->tab.Np_s, host.data(), true);
+}
+
+int adg_keep_Hw(adg_ctx* c, int on) {
+    if (!c) return ADG_E_ARG;
+    c->keep_hw = on != 0;
+    if (c->keep_hw && c->levels_set && !c->Hw_dbg) {
+        int rc = dev_alloc(c, &c->Hw_dbg, (size_t)c->Q * c->tab.Np_st * c->H.NS);
+        if (rc != ADG_OK) return rc;
+    }
+    return ADG_OK;
+}
+int adg_download_Hw(adg_ctx* c, int lev, double* host) {
+    if (!c) return ADG_E_ARG;
+    if (!c->Hw_dbg) return fail(c, ADG_E_STATE, "adg_keep_Hw(ctx,1) was not enabled");
+    return copy_level(c, lev, c->Hw_dbg, c->tab.Np_st, host, false);
+}
+
+int adg_download_Fnum(adg_ctx* c, int lev, int d, double* host) {
+    if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev || d < 0 || d >= c->D || !host)
+        return fail(c, ADG_E_ARG, "bad arguments");
+    const size_t nf = (size_t)c->H.lev[lev].nface[d], off = (size_t)c->H.lev[lev].face_off[d];
+    const size_t NF = (size_t)c->H.NF[d], rows = (size_t)c->Q * c->tab.qMp_st_bd;
+    HIPCHK(c, hipMemcpy2DAsync(host, nf * sizeof(double), c->Fnum[d] + off, NF * sizeof(double), nf * sizeof(double),
+                               rows, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+int adg_fill_ghosts(adg_ctx* c) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    const long ng = (long)c->H.g_slot.size();
+    if (ng == 0) return ADG_OK;
+    adg::GhostArgs a{c->d_tab, c->to, c->Uw, c->g_slot, c->g_src, c->g_kind, c->g_child, c->H.NS, ng};
+#define L_GH(PP, DD) hipLaunchKernelGGL((adg::k_ghosts<PP, DD>), dim3(nblocks(ng * c->Q)), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_GH);
+#undef L_GH
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+int adg_average_down(adg_ctx* c) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    for (int l = c->H.nlev - 1; l > 0; --l) {
+        const long ng = (long)c->H.ad_coarse[l].size();
+        if (ng == 0) continue;
+        adg::AvgArgs a{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(ng * c->Q)), dim3(256), 0, c->stream, a)
+        ADG_DISPATCH(c, L_AD);
+#undef L_AD
+        HIPCHK(c, hipGetLastError());
+    }
+    return ADG_OK;
+}
+
+static int launch_cfl(adg_ctx* c, double c_dt, double T, int advance) {
+    const long nv = (long)c->H.valid_list.size();
+    const double cfl = (1.0 / (2.0 * (double)c->P + 1.0)) * (1.0 / (double)c->D);
+    adg::CflArgs a{c->d_tab, c->to, c->Uw, c->valid_list, c->slot_lev, c->ts, c->geo, c->H.NS, nv, c->gamma, cfl};
+#define L_CFL(PP, DD) hipLaunchKernelGGL((adg::k_cfl<PP, DD>), dim3(nblocks(nv)), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_CFL);
+#undef L_CFL
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(adg::k_dt_finalize, dim3(1), dim3(64), 0, c->stream, c->ts, c_dt, T, advance);
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+int adg_set_time(adg_ctx* c, double t) {
+    if (!c) return ADG_E_ARG;
+    HIPCHK(c, hipMemcpyAsync(&c->ts->t, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+int adg_set_dt(adg_ctx* c, double c_dt, double T, double t, double* dt_out) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    int rc = adg_set_time(c, t);
+    if (rc != ADG_OK) return rc;
+    if ((rc = launch_cfl(c, c_dt, T, 0)) != ADG_OK) return rc;
+    if (dt_out) {
+        HIPCHK(c, hipMemcpyAsync(dt_out, &c->ts->Dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ADG_OK;
+}
+
+int adg_set_dt_value(adg_ctx* c, double Dt) {
+    if (!c) return ADG_E_ARG;
+    if (!(Dt > 0.0)) return fail(c, ADG_E_ARG, "Dt must be positive");
+    HIPCHK(c, hipMemcpyAsync(&c->ts->Dt, &Dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (mode != 0) return fail(c, ADG_E_UNSUP, "only mode 0 (P Picard iterations) is available");
+    if (lev < -1 || lev >= c->H.nlev) return fail(c, ADG_E_ARG, "bad level");
+    if (n_fallback) *n_fallback = 0;
+    long begin = 0, n = c->H.NS;
+    if (lev >= 0) { begin = c->H.lev[lev].slot_off; n = c->H.lev[lev].nslot; }
+    if (n == 0) return ADG_OK;
+    adg::PredArgs a{c->d_tab, c->to, c->Uw, c->Vol, c->Tr, c->keep_hw ? c->Hw_dbg : nullptr, c->slot_lev, c->ts,
+                    c->geo, c->H.NS, begin, n, c->gamma};
+#define L_PR(PP, DD)                                                                              \
+    {                                                                                             \
+        constexpr int epw = adg::Dims<PP, DD>::EPW;                                               \
+        long waves = (n + epw - 1) / epw;                                                         \
+        hipLaunchKernelGGL((adg::k_predictor<PP, DD>), dim3(nblocks(waves * 64)), dim3(256), 0, c->stream, a); \
+    }
+    ADG_DISPATCH(c, L_PR);
+#undef L_PR
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+int adg_face_flux(adg_ctx* c, int lev) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    (void)lev;  // faces of every level are processed in one launch
+    adg::FaceArgs a{};
+    a.Tr = c->Tr;
+    long tot = 0;
+    for (int d = 0; d < c->D; ++d) {
+        a.Fnum[d] = c->Fnum[d];
+        a.face_lr[d] = c->face_lr[d];
+        a.NF[d] = c->H.NF[d];
+        tot += c->H.NF[d] * c->tab.qMp_st_bd;
+    }
+    a.NS = c->H.NS;
+    a.gamma = c->gamma;
+#define L_FF(PP, DD) hipLaunchKernelGGL((adg::k_faceflux<PP, DD>), dim3(nblocks(tot)), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_FF);
+#undef L_FF
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+int adg_corrector(adg_ctx* c, int lev) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    long begin = 0, n = (long)c->H.valid_list.size();
+    if (lev >= 0) {
+        if (lev >= c->H.nlev) return fail(c, ADG_E_ARG, "bad level");
+        begin = c->H.lev[lev].valid_begin;
+        n = c->H.lev[lev].valid_end - begin;
+    }
+    if (n == 0) return ADG_OK;
+    adg::CorrArgs a{};
+    a.tab = c->d_tab;
+    a.to = c->to;
+    a.Uw = c->Uw;
+    a.Vol = c->Vol;
+    for (int d = 0; d < c->D; ++d) {
+        a.Fnum[d] = c->Fnum[d];
+        a.cell_face[d][0] = c->cell_face[d][0] + begin;
+        a.cell_face[d][1] = c->cell_face[d][1] + begin;
+        a.NF[d] = c->H.NF[d];
+    }
+    a.valid_list = c->valid_list + begin;
+    a.slot_lev = c->slot_lev;
+    a.ts = c->ts;
+    a.geo = c->geo;
+    a.NS = c->H.NS;
+    a.nvalid = n;
+#define L_CO(PP, DD) hipLaunchKernelGGL((adg::k_corrector<PP, DD>), dim3(nblocks(n * c->Q)), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_CO);
+#undef L_CO
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+int adg_reflux(adg_ctx* c, int crse_lev) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    (void)crse_lev;  // every coarse level in one launch (disjoint cells)
+    const long NM = (long)c->H.mom_face.size(), nrc = (long)c->H.rc_slot.size();
+    if (NM == 0 || nrc == 0) return ADG_OK;
+    adg::MomArgs m{};
+    m.tab = c->d_tab;
+    m.to = c->to;
+    for (int d = 0; d < c->D; ++d) { m.Fnum[d] = c->Fnum[d]; m.NF[d] = c->H.NF[d]; }
+    m.Mom = c->Mom;
+    m.mom_face = c->mom_face;
+    m.mom_d = c->mom_d;
+    m.mom_kind = c->mom_kind;
+    m.mom_sel = c->mom_sel;
+    m.mom_lev = c->mom_lev;
+    m.ts = c->ts;
+    m.geo = c->geo;
+    m.NM = NM;
+#define L_MO(PP, DD) hipLaunchKernelGGL((adg::k_moments<PP, DD>), dim3(nblocks(NM * c->Q)), dim3(256), 0, c->stream, m)
+    ADG_DISPATCH(c, L_MO);
+#undef L_MO
+    HIPCHK(c, hipGetLastError());
+    adg::RefluxArgs r{c->d_tab, c->to, c->Uw, c->Mom, c->rc_slot, c->rc_start, c->rc_count, c->rc_lev,
+                      c->re_mom, c->re_coef, c->geo, c->H.NS, NM, nrc};
+#define L_RF(PP, DD) hipLaunchKernelGGL((adg::k_reflux<PP, DD>), dim3(nblocks(nrc * c->Q)), dim3(256), 0, c->stream, r)
+    ADG_DISPATCH(c, L_RF);
+#undef L_RF
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+int adg_ader(adg_ctx* c) {
+    int rc;
+    if ((rc = adg_predictor(c, -1, 0, nullptr)) != ADG_OK) return rc;
+    if ((rc = adg_face_flux(c, -1)) != ADG_OK) return rc;
+    if ((rc = adg_corrector(c, -1)) != ADG_OK) return rc;
+    if (c->H.nlev > 1 && (rc = adg_reflux(c, -1)) != ADG_OK) return rc;
+    return ADG_OK;
+}
+
+int adg_step(adg_ctx* c, double c_dt, double T, double* t_out) {
+    int rc;
+    if ((rc = adg_ader(c)) != ADG_OK) return rc;
+    if ((rc = adg_average_down(c)) != ADG_OK) return rc;
+    if ((rc = adg_fill_ghosts(c)) != ADG_OK) return rc;
+    if ((rc = launch_cfl(c, c_dt, T, 1)) != ADG_OK) return rc;
+    if (t_out) {
+        HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ADG_OK;
+}
+
+int adg_conserved_totals(adg_ctx* c, double* totals) {
+    if (!c || !c->levels_set || !totals) return fail(c, ADG_E_STATE, "levels not set");
+    const long n = (long)c->H.uncovered.size();
+    const unsigned nb = nblocks(n);
+    adg::TotArgs a{c->Uw, c->uncovered, c->slot_lev, c->partial, c->geo, c->H.NS, n, c->tab.Np_s, c->Q};
+    hipLaunchKernelGGL(adg::k_totals, dim3(nb), dim3(256), 0, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)c->Q * nb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < c->Q; ++q) {
+        double s = 0.0;
+        for (unsigned b = 0; b < nb; ++b) s += h[(size_t)q * nb + b];
+        totals[q] = s;
+    }
+    return ADG_OK;
+}
+
+int adg_sync(adg_ctx* c) {
+    if (!c) return ADG_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+void* adg_stream(adg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int adg_sizes(adg_ctx* c, int* o) {
+    if (!c || !o) return ADG_E_ARG;
+    o[0] = c->tab.Np_s; o[1] = c->tab.Np_st; o[2] = c->tab.qMp_s; o[3] = c->tab.qMp_st;
+    o[4] = c->tab.qMp_st_bd; o[5] = c->tab.N; o[6] = c->Q;
+    return ADG_OK;
+}
+
+int adg_get_table(adg_ctx* c, const char* name, double* out, int cap) {
+    if (!c || !name) return ADG_E_ARG;
+    const std::vector<double>* v = adg::table_by_name(c->tab, name);
+    if (!v) return fail(c, ADG_E_ARG, std::string("unknown table ") + name);
+    if (out)
+        for (size_t i = 0; i < v->size() && (int)i < cap; ++i) out[i] = (*v)[i];
+    return (int)v->size();
+}
+
+const char* adg_last_error(adg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void adg_destroy(adg_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_levels(c);
+    if (c->d_tab) (void)hipFree(c->d_tab);
+    if (c->ts) (void)hipFree(c->ts);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

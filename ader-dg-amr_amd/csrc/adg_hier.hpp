@@ -1,0 +1,87 @@
+// adg_hier.hpp -- host-side AMR substrate: the device layout of a patch
+// hierarchy and every index list the kernels walk.
+//
+// The reference keeps one AMReX MultiFab per (level, variable) with one ghost
+// cell and lets AMReX FillPatch / FluxRegister / average_down move data
+// between boxes and levels.  Here the same semantics are flattened once, on
+// the host, into plain index lists so that each stage is a single kernel
+// launch over all patches of all levels:
+//   - slots: every cell (valid + ghost) of every patch of every level, levels
+//     concatenated, patches in order, x fastest inside a patch;
+//   - faces: every valid face of direction d of every patch;
+//   - ghost sources (same-level / periodic copy or coarse L2 projection);
+//   - average-down groups (coarse cell <- 2^D fine cells);
+//   - flux-register algebra reduced to a sparse list of signed face moments
+//     per coarse cell (FluxRegister CrseAdd/FineAdd/Reflux + the reference's
+//     cf_face_b_coarse ownership rule, AmrDG_Level.tpp:240-446).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace adg {
+
+struct LevelLayout {
+    int dom[3] = {1, 1, 1};
+    double dx[3] = {1, 1, 1};
+    long slot_off = 0, nslot = 0;
+    long face_off[3] = {0, 0, 0}, nface[3] = {0, 0, 0};
+    int p_begin = 0, p_end = 0;
+    long valid_begin = 0, valid_end = 0;  // range in valid_list
+};
+
+struct PatchLayout {
+    int lev = 0;
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int n[3] = {1, 1, 1}, g[3] = {1, 1, 1};
+    long off = 0;        // global slot offset
+    long foff[3] = {0, 0, 0};  // global face offset per direction
+};
+
+struct Hierarchy {
+    int D = 2, nlev = 0;
+    int dom0[3] = {1, 1, 1};
+    double plo[3] = {0, 0, 0}, phi[3] = {1, 1, 1};
+    int per[3] = {1, 1, 1};
+    std::vector<LevelLayout> lev;
+    std::vector<PatchLayout> patch;
+    long NS = 0;               // total slots
+    long NF[3] = {0, 0, 0};    // total faces per direction
+
+    std::vector<uint8_t> slot_lev;  // level of every slot
+    std::vector<int> valid_list;    // global slots of valid cells, level-major
+    std::vector<int> cell_face[3][2];  // per valid cell: lo/hi face id per d
+    std::vector<int> face_lr[3];    // per face: left slot, right slot (2 ints)
+    std::vector<uint8_t> face_lev[3];
+
+    // ghost sources: kind 0 copy, kind 1 L2 projection from coarse parent
+    std::vector<int> g_slot, g_src;
+    std::vector<uint8_t> g_kind, g_child;
+
+    // average down, per fine level lf (index lf): coarse slot + 2^D fine slots
+    std::vector<std::vector<int>> ad_coarse, ad_fine;
+
+    // flux-register algebra
+    // moments: kind 0 coarse face (Mkbd{p|m}), kind 1 fine face (P_flux_{high|low})
+    std::vector<int> mom_face;     // face id (global, within its direction)
+    std::vector<uint8_t> mom_d, mom_kind, mom_sel, mom_lev;  // sel: side (0 m/low,1 p/high) | child<<1
+    std::vector<int> rc_slot, rc_start, rc_count;  // corrected coarse cells
+    std::vector<uint8_t> rc_lev;
+    std::vector<int> re_mom;       // entries: moment index
+    std::vector<double> re_coef;   //          coefficient (+-1, summed)
+
+    // uncovered valid cells (for conserved totals)
+    std::vector<int> uncovered;
+
+    // face masks for tests (global face ids per direction)
+    std::vector<int8_t> bcoarse[3], bfine[3], cidx[3];
+
+    long slot_of(int p, const int* iv) const;
+    long face_of(int p, int d, const int* iv) const;
+};
+
+// Throws std::runtime_error on malformed input.
+Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
+                          const int* per, int nlev, const int* npatch, const int* boxes);
+
+}  // namespace adg

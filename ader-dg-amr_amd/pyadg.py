@@ -1,0 +1,230 @@
+"""ctypes binding of libadg.so (the C-ABI in include/adg.h).
+
+Loads the in-tree library built by ader-dg-amr_amd/Makefile and fails loudly
+when it is missing: there is no CPU fallback for the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libadg.so")
+
+ADG_FP64 = 0
+ADG_FP32_MFMA = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_LIB = None
+
+# every entry point declared in include/adg.h
+EXPORTS = [
+    "adg_create", "adg_set_geometry", "adg_set_levels", "adg_level_nslot", "adg_level_nface",
+    "adg_upload_Uw", "adg_download_Uw", "adg_fill_ghosts", "adg_average_down", "adg_set_dt",
+    "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
+    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_step", "adg_set_time",
+    "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
+    "adg_project_ic", "adg_last_error", "adg_destroy",
+]
+
+
+class AdgError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise AdgError(f"{LIBPATH} is not built; run __graft_entry__.build() or make -C ader-dg-amr_amd")
+    L = C.CDLL(LIBPATH)
+    L.adg_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p]
+    L.adg_set_geometry.argtypes = [C.c_void_p, _ip, _dp, _dp, _ip]
+    L.adg_set_levels.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+    L.adg_level_nslot.argtypes = [C.c_void_p, C.c_int]
+    L.adg_level_nface.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.adg_upload_Uw.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_download_Uw.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_fill_ghosts.argtypes = [C.c_void_p]
+    L.adg_average_down.argtypes = [C.c_void_p]
+    L.adg_set_dt.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp]
+    L.adg_set_dt_value.argtypes = [C.c_void_p, C.c_double]
+    L.adg_predictor.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
+    L.adg_keep_Hw.argtypes = [C.c_void_p, C.c_int]
+    L.adg_download_Hw.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_face_flux.argtypes = [C.c_void_p, C.c_int]
+    L.adg_download_Fnum.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.adg_corrector.argtypes = [C.c_void_p, C.c_int]
+    L.adg_reflux.argtypes = [C.c_void_p, C.c_int]
+    L.adg_ader.argtypes = [C.c_void_p]
+    L.adg_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    L.adg_set_time.argtypes = [C.c_void_p, C.c_double]
+    L.adg_conserved_totals.argtypes = [C.c_void_p, _dp]
+    L.adg_sync.argtypes = [C.c_void_p]
+    L.adg_stream.argtypes = [C.c_void_p]
+    L.adg_stream.restype = C.c_void_p
+    L.adg_sizes.argtypes = [C.c_void_p, _ip]
+    L.adg_get_table.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_int]
+    L.adg_project_ic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.adg_last_error.argtypes = [C.c_void_p]
+    L.adg_last_error.restype = C.c_char_p
+    L.adg_destroy.argtypes = [C.c_void_p]
+    for nm in EXPORTS:
+        f = getattr(L, nm)
+        if nm not in ("adg_stream", "adg_last_error", "adg_destroy"):
+            f.restype = C.c_int
+    _LIB = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Context:
+    """One adg_ctx: element tables, device layout of a hierarchy, fields."""
+
+    def __init__(self, dim, p, gamma=1.4, precision=ADG_FP64):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.adg_create(C.byref(h), dim, p, gamma, precision, None)
+        if rc != 0:
+            raise AdgError(f"adg_create failed rc={rc}")
+        self.h = h
+        self.dim, self.p = dim, p
+        s = np.zeros(7, dtype=np.int32)
+        self.L.adg_sizes(self.h, _i(s))
+        self.Np_s, self.Np_st, self.qMp_s, self.qMp_st, self.qMp_st_bd, self.N, self.Q = map(int, s)
+        self.nlev = 0
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise AdgError(f"{what} failed rc={rc}: {self.L.adg_last_error(self.h).decode()}")
+
+    def set_geometry(self, dom_n, prob_lo, prob_hi, periodic=None):
+        per = np.array(periodic if periodic is not None else [1] * self.dim, dtype=np.int32)
+        self._chk(self.L.adg_set_geometry(self.h, _i(np.array(dom_n, dtype=np.int32)),
+                                          _d(np.array(prob_lo, dtype=np.float64)),
+                                          _d(np.array(prob_hi, dtype=np.float64)), _i(per)),
+                  "adg_set_geometry")
+
+    def set_levels(self, levels):
+        npatch = np.array([len(l) for l in levels], dtype=np.int32)
+        flat = []
+        for lev in levels:
+            for lo, hi in lev:
+                flat += list(lo) + list(hi)
+        self._chk(self.L.adg_set_levels(self.h, len(levels), _i(npatch), _i(np.array(flat, dtype=np.int32))),
+                  "adg_set_levels")
+        self.nlev = len(levels)
+
+    def nslot(self, lev):
+        return self.L.adg_level_nslot(self.h, lev)
+
+    def nface(self, lev, d):
+        return self.L.adg_level_nface(self.h, lev, d)
+
+    def upload_Uw(self, lev, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (self.Q, self.Np_s, self.nslot(lev))
+        self._chk(self.L.adg_upload_Uw(self.h, lev, _d(U)), "adg_upload_Uw")
+
+    def download_Uw(self, lev):
+        out = np.zeros((self.Q, self.Np_s, self.nslot(lev)))
+        self._chk(self.L.adg_download_Uw(self.h, lev, _d(out)), "adg_download_Uw")
+        return out
+
+    def keep_Hw(self, on=True):
+        self._chk(self.L.adg_keep_Hw(self.h, 1 if on else 0), "adg_keep_Hw")
+
+    def download_Hw(self, lev):
+        out = np.zeros((self.Q, self.Np_st, self.nslot(lev)))
+        self._chk(self.L.adg_download_Hw(self.h, lev, _d(out)), "adg_download_Hw")
+        return out
+
+    def download_Fnum(self, lev, d):
+        out = np.zeros((self.Q, self.qMp_st_bd, self.nface(lev, d)))
+        self._chk(self.L.adg_download_Fnum(self.h, lev, d, _d(out)), "adg_download_Fnum")
+        return out
+
+    def fill_ghosts(self):
+        self._chk(self.L.adg_fill_ghosts(self.h), "adg_fill_ghosts")
+
+    def average_down(self):
+        self._chk(self.L.adg_average_down(self.h), "adg_average_down")
+
+    def set_dt(self, c_dt, T, t):
+        out = np.zeros(1)
+        self._chk(self.L.adg_set_dt(self.h, c_dt, T, t, _d(out)), "adg_set_dt")
+        return float(out[0])
+
+    def set_dt_value(self, Dt):
+        self._chk(self.L.adg_set_dt_value(self.h, Dt), "adg_set_dt_value")
+
+    def set_time(self, t):
+        self._chk(self.L.adg_set_time(self.h, t), "adg_set_time")
+
+    def predictor(self, lev=-1, mode=0):
+        nf = C.c_int(0)
+        self._chk(self.L.adg_predictor(self.h, lev, mode, C.byref(nf)), "adg_predictor")
+        return nf.value
+
+    def face_flux(self, lev=-1):
+        self._chk(self.L.adg_face_flux(self.h, lev), "adg_face_flux")
+
+    def corrector(self, lev=-1):
+        self._chk(self.L.adg_corrector(self.h, lev), "adg_corrector")
+
+    def reflux(self, lev=-1):
+        self._chk(self.L.adg_reflux(self.h, lev), "adg_reflux")
+
+    def ader(self):
+        self._chk(self.L.adg_ader(self.h), "adg_ader")
+
+    def step(self, c_dt, T, want_t=False):
+        if want_t:
+            out = np.zeros(1)
+            self._chk(self.L.adg_step(self.h, c_dt, T, _d(out)), "adg_step")
+            return float(out[0])
+        self._chk(self.L.adg_step(self.h, c_dt, T, None), "adg_step")
+        return None
+
+    def totals(self):
+        out = np.zeros(self.Q)
+        self._chk(self.L.adg_conserved_totals(self.h, _d(out)), "adg_conserved_totals")
+        return out
+
+    def project_ic(self, lev, ic_case, params):
+        prm = np.array(params, dtype=np.float64)
+        self._chk(self.L.adg_project_ic(self.h, lev, ic_case, _d(prm)), "adg_project_ic")
+
+    def sync(self):
+        self._chk(self.L.adg_sync(self.h), "adg_sync")
+
+    def stream(self):
+        return self.L.adg_stream(self.h)
+
+    def table(self, name):
+        n = self.L.adg_get_table(self.h, name.encode(), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        self.L.adg_get_table(self.h, name.encode(), _d(out), n)
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.adg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

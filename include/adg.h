@@ -1,0 +1,164 @@
+/*
+ * adg.h -- C-ABI of the MI355X ADER-DG hot path (libadg.so).
+ *
+ * The reference (TrustiestMass28/ADER-DG-AMR) has no runtime ABI: its
+ * boundary is the compile-time CRTP of Solver<AmrDG<P>> and
+ * ModelEquation<Compressible_Euler>.  These entry points sit under that
+ * interface: the CRTP methods stay on the host and their MFIter loop bodies
+ * become calls below.  Each entry point names the reference member it
+ * replaces (paths relative to the reference root, prefix Source/).
+ *
+ * Conventions
+ *  - Every call returns ADG_OK (0) or a negative ADG_E_* code; nothing
+ *    throws across the boundary.  adg_last_error() describes the failure.
+ *  - The context owns all device memory; the caller owns box metadata.
+ *  - One host thread per context.  Calls are ordered on the context's HIP
+ *    stream and asynchronous, except the download / *_out readbacks and
+ *    adg_sync(), which block.
+ *  - Modal data layout per level: [q][n][slot] FP64, slots enumerate the
+ *    grown (one ghost cell) box of every patch of the level in the order
+ *    given to adg_set_levels, x fastest.
+ */
+#ifndef ADG_H
+#define ADG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct adg_ctx adg_ctx;
+
+enum {
+    ADG_OK = 0,
+    ADG_E_ARG = -1,     /* bad argument / shape */
+    ADG_E_HIP = -2,     /* HIP runtime error */
+    ADG_E_STATE = -3,   /* call order violated (e.g. no levels set) */
+    ADG_E_NOMEM = -4,   /* device allocation failed */
+    ADG_E_UNSUP = -5    /* (dim, p) combination not compiled */
+};
+
+enum { ADG_FP64 = 0, ADG_FP32_MFMA = 1 };
+
+/* Replaces AmrDG<P>::init (NumericalMethod/AmrDG/AmrDG.tpp:125-246: basis,
+ * quadrature, element and projection matrices) together with
+ * Compressible_Euler::settings (Model/CompressibleEuler/Compressible_Euler.cpp:5-48).
+ * dim in {2,3}; p in {1,2,3}; precision ADG_FP64 or ADG_FP32_MFMA;
+ * hip_stream may be NULL (the context creates its own stream). */
+int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void* hip_stream);
+
+/* Domain of level 0 (AmrCore geometry, Core/Mesh.h:128-147): cells per
+ * dimension, physical extent and periodicity flags. */
+int adg_set_geometry(adg_ctx* ctx, const int* dom_n, const double* prob_lo,
+                     const double* prob_hi, const int* periodic);
+
+/* Replaces AmrDG::set_init_data_system / set_init_data_component
+ * (AmrDG.tpp:279-334) and AmrDG::AMR_set_flux_registers
+ * (AmrDG_Level.tpp:240-414): allocates the device layout of every level,
+ * builds ghost-source lists, face lists, coarse-fine face masks and the
+ * flux-register algebra.  boxes holds lo[dim], hi[dim] per patch, level
+ * index space, ratio 2 between levels, fine boxes aligned to the ratio. */
+int adg_set_levels(adg_ctx* ctx, int nlev, const int* npatch_per_lev, const int* boxes);
+
+/* Number of slots of a level (valid + ghost cells of all its patches). */
+int adg_level_nslot(adg_ctx* ctx, int lev);
+/* Number of faces of direction d of a level (valid faces of all patches). */
+int adg_level_nface(adg_ctx* ctx, int lev, int d);
+
+/* Host <-> device copies of U_w for one level, layout [q][n][slot]. */
+int adg_upload_Uw(adg_ctx* ctx, int lev, const double* host);
+int adg_download_Uw(adg_ctx* ctx, int lev, double* host);
+
+/* Replaces AmrDG::set_initial_condition (AmrDG.h:782-836) with the
+ * pointwise Compressible_Euler::pde_IC (Compressible_Euler.h:141-368): L2
+ * projection of an initial state onto the modes of every slot of a level
+ * (ghost cells included), evaluated on the host and uploaded.
+ *   ic_case 0: isentropic vortex, params {beta, xc, yc, u_inf, v_inf}
+ *   ic_case 1: Kelvin-Helmholtz shear layer, params {amp, noise_amp, seed}
+ *              (v = amp sin(4 pi x) + noise_amp * U[-1,1] per cell)
+ *   ic_case 2: uniform state, params {rho, u, v, [w,] p}
+ *   ic_case 3: spherical/radial shock tube, params {r0, rho_in, p_in,
+ *              rho_out, p_out} centred in the domain */
+int adg_project_ic(adg_ctx* ctx, int lev, int ic_case, const double* params);
+
+/* Replaces AmrDG::AMR_FillPatch on every level (AmrDG_Level.tpp:172-224)
+ * and Solver::FillBoundaryCells for periodic domains: same-level and
+ * periodic copies plus L2ProjInterp::interp / amr_scatter into coarse-fine
+ * ghosts (AmrDG_Interpolater.tpp:251-281,344-364). */
+int adg_fill_ghosts(adg_ctx* ctx);
+
+/* Replaces AmrDG::AMR_average_fine_coarse (AmrDG_Level.tpp:227-238) ->
+ * L2ProjInterp::average_down / amr_gather (AmrDG_Interpolater.tpp:283-398). */
+int adg_average_down(adg_ctx* ctx);
+
+/* Replaces AmrDG::set_Dt (AmrDG.h:1253-1357) and the clamp Dt=min(Dt,T-t)
+ * of AmrDG::evolve (AmrDG.h:1028-1029).  The result stays on the device for
+ * the next step; dt_out (nullable) receives a blocking copy. */
+int adg_set_dt(adg_ctx* ctx, double c_dt, double T, double t, double* dt_out);
+/* Sets the device-resident step size directly (tests, restarts). */
+int adg_set_dt_value(adg_ctx* ctx, double Dt);
+
+/* Replaces AmrDG::set_predictor + the Picard loop of AmrDG::ADER
+ * (AmrDG.h:1100-1122, AmrDG.tpp:387-435,495-537,744-827) on the grown boxes
+ * of every level, plus the final modal->nodal evaluation, volume flux
+ * (AmrDG::flux, AmrDG.h:1167-1205) and face traces (get_H_from_H_w_bd,
+ * AmrDG.tpp:437-493).  lev = -1 runs every level in one launch.
+ * mode 0: exactly P Picard iterations (the reference); other modes are
+ * reserved.  n_fallback (nullable) receives 0 in mode 0. */
+int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
+/* Debug readback of the last predictor H_w of a level, [q][Np_st][slot]
+ * (only after adg_keep_Hw(ctx,1)). */
+int adg_keep_Hw(adg_ctx* ctx, int on);
+int adg_download_Hw(adg_ctx* ctx, int lev, double* host);
+
+/* Replaces AmrDG::flux_bd + AmrDG::numflux / LLF_numflux for every
+ * direction (AmrDG.h:1126-1137, AmrDG.tpp:539-650,
+ * AmrDG_NumericalFlux.tpp:4-41): face-centred, once per face.
+ * lev = -1: every level. */
+int adg_face_flux(adg_ctx* ctx, int lev);
+/* Readback of Fnum for (lev, d): [q][qMp_st_bd][face]. */
+int adg_download_Fnum(adg_ctx* ctx, int lev, int d, double* host);
+
+/* Replaces AmrDG::update_U_w (AmrDG.tpp:652-742) on valid cells.
+ * lev = -1: every level. */
+int adg_corrector(adg_ctx* ctx, int lev);
+
+/* Replaces the FluxRegister CrseAdd/FineAdd of AmrDG::ADER
+ * (AmrDG.h:1138-1153) and AmrDG::AMR_flux_correction
+ * (AmrDG_Level.tpp:416-446) -> L2ProjInterp::reflux
+ * (AmrDG_Interpolater.tpp:7-72). crse_lev = -1: every coarse level. */
+int adg_reflux(adg_ctx* ctx, int crse_lev);
+
+/* Replaces AmrDG::ADER (AmrDG.h:1071-1165): predictor, face flux,
+ * corrector on all levels, then flux correction. Uses the device Dt. */
+int adg_ader(adg_ctx* ctx);
+
+/* One iteration of the AmrDG::evolve loop without regrid, limiter and
+ * plots (AmrDG.h:988-1029): ADER, average_down, FillPatch, t+=Dt, set_Dt,
+ * clamp to T.  t stays on the device; t_out (nullable) gets a blocking copy. */
+int adg_step(adg_ctx* ctx, double c_dt, double T, double* t_out);
+/* Sets the device-resident time. */
+int adg_set_time(adg_ctx* ctx, double t);
+
+/* Conserved totals sum(vol * U_w[q][0]) over uncovered valid cells, FP64
+ * (the quantity LpNorm_DG_AMR integrates over, AmrDG.h:1399-1578). */
+int adg_conserved_totals(adg_ctx* ctx, double* totals);
+
+/* Blocks until all work on the context stream has finished. */
+int adg_sync(adg_ctx* ctx);
+/* The hipStream_t the context launches on. */
+void* adg_stream(adg_ctx* ctx);
+/* Basis sizes: out[0..6] = Np_s, Np_st, qMp_s, qMp_st, qMp_st_bd, N, Q. */
+int adg_sizes(adg_ctx* ctx, int* out7);
+/* Copies a named FP64 table of the context (same names as the reference
+ * members: nodes weights V Vinv Mk_h_w_inv Mk_pred Sk_predVinv Sk_corr
+ * Mkbdm Mkbdp quadmat Minv P_cf P_fc P_flux_low P_flux_high).  Returns the
+ * number of doubles or a negative code. */
+int adg_get_table(adg_ctx* ctx, const char* name, double* out, int cap);
+
+const char* adg_last_error(adg_ctx* ctx);
+void adg_destroy(adg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,118 @@
+/*
+ * adg_oracle.h -- CPU restatement of the reference AmrDG ADER-DG hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it may be loaded
+ * by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of
+ * bench.py, never by the product library or its host code.
+ *
+ * It restates, in FP64 and in the reference's loop order, the arithmetic of
+ *   Source/NumericalMethod/AmrDG/AmrDG.h:53-353        (basis, GL quadrature)
+ *   Source/NumericalMethod/AmrDG/AmrDG_ElementMatrix.tpp (element matrices)
+ *   Source/NumericalMethod/AmrDG/AmrDG.tpp:336-827       (predictor, numflux, corrector)
+ *   Source/NumericalMethod/AmrDG/AmrDG_NumericalFlux.tpp:4-41 (LLF)
+ *   Source/NumericalMethod/AmrDG/AmrDG.h:1071-1357       (ADER order, set_Dt)
+ *   Source/NumericalMethod/AmrDG/AmrDG_Interpolater.tpp  (scatter/gather/reflux)
+ *   Source/NumericalMethod/AmrDG/AmrDG_Level.tpp:240-446 (C-F masks, flux registers)
+ *   Source/Model/CompressibleEuler/Compressible_Euler.h:370-604, .cpp:5-70
+ * plus the AMReX FillPatch / FluxRegister semantics those call sites rely on.
+ *
+ * Layout convention shared with the product's C-ABI (an interface, not an
+ * algorithm): per level, modal data is [q][comp][slot]; slots enumerate the
+ * grown (1 ghost cell) box of every patch of the level, patches in the order
+ * given, x fastest:  slot = off[b] + ((kk*g1)+jj)*g0 + ii,  ii = i-lo0+1, ...
+ * Face data of direction d is [q][m][fslot] over the valid faces of every
+ * patch (n_d+1 faces along d).
+ */
+#ifndef ADG_ORACLE_H
+#define ADG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_tables orc_tables;
+typedef struct orc_hier orc_hier;
+
+/* ---- tables (basis, quadrature, element and projection matrices) ---- */
+orc_tables* orc_tables_create(int P, int D);
+void        orc_tables_destroy(orc_tables*);
+/* sizes: [Np_s, Np_st, qMp_s, qMp_st, qMp_st_bd, N, Q] */
+void        orc_tables_sizes(const orc_tables*, int* out7);
+/* Copy a named table into out (row-major as in the reference's Eigen (r,c)).
+ * Returns number of doubles, or -1 for an unknown name.  Names:
+ *  nodes weights val dval bd_val shifted_lo shifted_hi mi_s mi_st
+ *  V Vinv Mk_h_w Mk_h_w_inv Mk_pred Sk_pred Sk_predVinv Mk_corr Sk_corr
+ *  Mkbdm Mkbdp quadmat Minv P_cf P_fc P_flux_low P_flux_high
+ * Per-direction tables are stacked [d][r][c]; P_cf/P_fc [child][r][c];
+ * P_flux_* [d][child][r][c]. */
+int         orc_tables_get(const orc_tables*, const char* name, double* out, int cap);
+
+/* ---- hierarchy (boxes per level, ratio 2, 1 ghost cell) ---- */
+/* boxes: for each level, npatch[l] entries of lo[D], hi[D] (ints, level index
+ * space).  dom_n: level-0 cells per dim.  periodic: per dim flags. */
+orc_hier*   orc_hier_create(int D, int nlev, const int* npatch, const int* boxes,
+                            const int* dom_n, const double* prob_lo,
+                            const double* prob_hi, const int* periodic);
+void        orc_hier_destroy(orc_hier*);
+int         orc_hier_nslot(const orc_hier*, int lev);
+int         orc_hier_nfslot(const orc_hier*, int lev, int d);
+/* 1 if slot is a valid cell, 0 if ghost; out has nslot ints */
+void        orc_hier_valid_mask(const orc_hier*, int lev, int* out);
+/* flux-register face masks, restated from AMR_set_flux_registers:
+ * kind 0 = b_coarse, 1 = b_fine, 2 = child idx; out has nfslot ints */
+void        orc_hier_face_mask(const orc_hier*, int lev, int d, int kind, int* out);
+/* cell-centre coordinate of a slot (ghosts included, not wrapped) */
+void        orc_hier_slot_centre(const orc_hier*, int lev, int slot, double* x);
+
+/* ---- initial condition: L2 projection of a pointwise IC ----
+ * case 0: isentropic vortex (params: beta, xc, yc, uinf, vinf)
+ * case 1: Kelvin-Helmholtz (params: amp, noise_amp, seed)  [no reference IC]
+ * case 2: uniform state (params: rho, u, v, [w,] p)
+ * case 3: radial shock tube (params: r0, rho_in, p_in, rho_out, p_out, centre[D])
+ * Fills every slot (ghosts too, as AmrDG::set_initial_condition on growntilebox). */
+void        orc_project_ic(const orc_tables*, const orc_hier*, int lev, int ic_case,
+                           const double* params, double gamma, double* Uw);
+
+/* ---- pointwise model (Compressible_Euler) ---- */
+double      orc_pressure(int D, const double* u, double gamma);
+double      orc_pde_flux(int D, int d, int q, const double* u, double gamma);
+double      orc_pde_dflux(int D, int d, const double* u, double gamma);
+double      orc_cfl_lambda(int D, int d, const double* u, double gamma);
+double      orc_llf(double uL, double uR, double fL, double fR, double dfL, double dfR);
+
+/* ---- hot-path stages (Uw: array of nlev pointers to per-level data) ---- */
+/* Predictor on the grown boxes of one level: P Picard iterations from the
+ * set_predictor guess.  Hw_out: [q][Np_st][slot]. */
+void        orc_predictor(const orc_tables*, const orc_hier*, int lev,
+                          const double* Uw, double Dt, double gamma, double* Hw_out);
+/* One AmrDG::ADER call (all levels, finest first, then flux correction).
+ * Fnum_out (nullable): per level per d face arrays [q][qMp_st_bd][fslot],
+ * packed level-major then d. */
+void        orc_ader(const orc_tables*, const orc_hier*, double** Uw, double Dt,
+                     double gamma, double* Fnum_out);
+void        orc_average_down(const orc_tables*, const orc_hier*, double** Uw);
+void        orc_fill_patch(const orc_tables*, const orc_hier*, double** Uw);
+/* set_Dt: returns c_dt * min over levels and valid cells */
+double      orc_set_dt(const orc_tables*, const orc_hier*, double** Uw,
+                       double c_dt, double gamma);
+/* Full time steps as AmrDG::evolve without regrid/limiter/plot:
+ * ADER, average_down, FillPatch, t += Dt, set_Dt, clamp to T.
+ * Returns the number of steps taken; *t_io is advanced; Dt_io holds the
+ * step size for the next step. nsteps_max <= 0 means run to T. */
+int         orc_evolve(const orc_tables*, const orc_hier*, double** Uw,
+                       double* t_io, double* Dt_io, double T, double c_dt,
+                       double gamma, int nsteps_max);
+
+/* Composite Lp error of component q vs the IC case evaluated at time t
+ * (periodic shift by background velocity for the vortex), over uncovered
+ * cells, normalised by uncovered volume (LpNorm_DG_AMR). */
+double      orc_lp_error(const orc_tables*, const orc_hier*, double** Uw, int q,
+                         int p, int ic_case, const double* params, double gamma,
+                         double t);
+/* Conserved totals: sum over uncovered valid cells of vol * Uw[q][0] */
+void        orc_totals(const orc_tables*, const orc_hier*, double** Uw, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

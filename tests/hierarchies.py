@@ -1,0 +1,45 @@
+"""Box layouts used by the tests (level index space, ratio 2)."""
+
+
+def tiles(n, mg, D=2, lo0=None, hi0=None):
+    """Boxes of edge mg covering [lo0, hi0] (default the whole n^D domain), x fastest."""
+    import itertools
+    lo0 = lo0 or [0] * D
+    hi0 = hi0 or [n - 1] * D
+    axes = [list(range(lo0[d], hi0[d] + 1, mg)) for d in range(D)]
+    out = []
+    for rev in itertools.product(*reversed(axes)):
+        lo = tuple(reversed(rev))
+        hi = tuple(min(lo[d] + mg - 1, hi0[d]) for d in range(D))
+        out.append((lo, hi))
+    return out
+
+
+def single_level(n, mg=None, D=2):
+    return [tiles(n, mg or n, D)]
+
+
+def two_level_centre(n, mg=None, D=2):
+    """Level 0 n^D; level 1 covers the central half (ratio-aligned)."""
+    l0 = tiles(n, mg or n, D)
+    lo = [n // 2] * D
+    hi = [n + n // 2 - 1] * D
+    l1 = tiles(2 * n, mg or n, D, lo, hi)
+    return [l0, l1]
+
+
+def kh_bands(n0, nlev=3, mg=32, D=2):
+    """Kelvin-Helmholtz layout: level l refines full-width bands around the
+    two shear layers y=0.25, 0.75, narrowing with level."""
+    levels = [tiles(n0, mg, D)]
+    for l in range(1, nlev):
+        n = n0 << l
+        w = max(4, n // (4 << l))  # half width in level cells, multiple of 2
+        w += w % 2
+        boxes = []
+        for yc in (n // 4, 3 * n // 4):
+            lo = [0, yc - w]
+            hi = [n - 1, yc + w - 1]
+            boxes += tiles(n, mg, D, lo, hi)
+        levels.append(boxes)
+    return levels

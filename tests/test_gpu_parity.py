@@ -1,0 +1,152 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the
+same seeded inputs.  Bar: FP64 form within 1e-12 normwise-relative per
+conserved variable (the north-star tolerance for U_w modes)."""
+import numpy as np
+import pytest
+
+from hierarchies import kh_bands, single_level, two_level_centre
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+VORTEX = [5.0, 5.0, 5.0, 1.0, 0.0]
+
+
+def rel_err(a, b):
+    """max over variables q of max|a-b| / max|b| (arrays [q][...])."""
+    out = 0.0
+    for q in range(b.shape[0]):
+        den = max(np.abs(b[q]).max(), 1e-300)
+        out = max(out, np.abs(a[q] - b[q]).max() / den)
+    return out
+
+
+def setup(adg, orc, D, P, levels, dom, plo, phi, ic_case, params, gamma=1.4):
+    T = orc.Tables(P, D)
+    H = orc.Hier(D, levels, dom, plo, phi)
+    Uw = orc.project_ic(T, H, ic_case, params, gamma)
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    ctx = adg.Context(D, P, gamma)
+    ctx.set_geometry(dom, plo, phi)
+    ctx.set_levels(levels)
+    for l in range(len(levels)):
+        assert ctx.nslot(l) == H.nslot(l)
+        ctx.upload_Uw(l, Uw[l])
+    return T, H, Uw, ctx
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_predictor_single_patch(adg, orc, P):
+    print(f"case predictor vortex P={P} 16^2", flush=True)
+    levels = single_level(16)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [10, 10], 0, VORTEX)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.keep_Hw(True)
+    ctx.set_dt_value(Dt)
+    ctx.predictor()
+    Hw = ctx.download_Hw(0)
+    ref = orc.predictor(T, H, 0, Uw[0], Dt)
+    assert rel_err(Hw, ref) < TOL
+
+
+def test_set_dt_matches(adg, orc):
+    print("case set_dt vortex P=2 32^2 4 patches", flush=True)
+    levels = single_level(32, 16)
+    T, H, Uw, ctx = setup(adg, orc, 2, 2, levels, [32, 32], [0, 0], [10, 10], 0, VORTEX)
+    dt_ref = orc.set_dt(T, H, Uw, 0.5)
+    dt = ctx.set_dt(0.5, 10.0, 0.0)
+    assert abs(dt - dt_ref) <= 1e-14 * dt_ref
+
+
+@pytest.mark.parametrize("P,mg", [(2, 32), (2, 16), (3, 8)])
+def test_ader_step_single_level(adg, orc, P, mg):
+    print(f"case ader vortex P={P} 32^2 patches {mg}", flush=True)
+    levels = single_level(32, mg)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [32, 32], [0, 0], [10, 10], 0, VORTEX)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt_value(Dt)
+    ctx.ader()
+    orc.ader(T, H, Uw, Dt)
+    U = ctx.download_Uw(0)
+    valid = H.valid_mask(0)
+    assert rel_err(U[:, :, valid], Uw[0][:, :, valid]) < TOL
+
+
+def test_face_flux_matches(adg, orc):
+    print("case Fnum vortex P=2 16^2 2x2 patches", flush=True)
+    levels = single_level(16, 8)
+    T, H, Uw, ctx = setup(adg, orc, 2, 2, levels, [16, 16], [0, 0], [10, 10], 0, VORTEX)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt_value(Dt)
+    ctx.predictor()
+    ctx.face_flux()
+    fn = orc.ader(T, H, [u.copy() for u in Uw], Dt, want_fnum=True)
+    at = 0
+    for d in range(2):
+        nf = H.nfslot(0, d)
+        ref = fn[at:at + T.Q * T.qMp_st_bd * nf].reshape(T.Q, T.qMp_st_bd, nf)
+        at += ref.size
+        assert rel_err(ctx.download_Fnum(0, d), ref) < TOL
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_evolve_steps_single_level(adg, orc, P):
+    print(f"case evolve 10 steps vortex P={P} 32^2", flush=True)
+    levels = single_level(32, 16)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [32, 32], [0, 0], [10, 10], 0, VORTEX)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    for _ in range(10):
+        ctx.step(0.5, 1.0)
+    n, t, _ = orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=10)
+    assert n == 10
+    valid = H.valid_mask(0)
+    assert rel_err(ctx.download_Uw(0)[:, :, valid], Uw[0][:, :, valid]) < TOL
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_two_level_step(adg, orc, P):
+    print(f"case two-level vortex P={P} 16^2+centre", flush=True)
+    levels = two_level_centre(16, 8)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [10, 10], 0, VORTEX)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    for _ in range(3):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=3)
+    for l in range(2):
+        U = ctx.download_Uw(l)
+        assert rel_err(U, Uw[l]) < TOL, f"level {l}"
+
+
+def test_kh_three_levels(adg, orc):
+    print("case KH P=3 16^2 3 levels", flush=True)
+    levels = kh_bands(16, 3, 16)
+    T, H, Uw, ctx = setup(adg, orc, 2, 3, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7])
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    m0 = ctx.totals()
+    for _ in range(3):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=3)
+    for l in range(3):
+        assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL, f"level {l}"
+    m1 = ctx.totals()
+    assert abs(m1[0] - m0[0]) / abs(m0[0]) < 1e-13
+
+
+def test_gpu_ic_matches_oracle(adg, orc):
+    print("case IC projection vortex + KH", flush=True)
+    levels = single_level(16, 8)
+    T = orc.Tables(3, 2)
+    H = orc.Hier(2, levels, [16, 16], [0, 0], [10, 10])
+    ctx = adg.Context(2, 3)
+    ctx.set_geometry([16, 16], [0, 0], [10, 10])
+    ctx.set_levels(levels)
+    ctx.project_ic(0, 0, VORTEX)
+    ref = orc.project_ic(T, H, 0, VORTEX)[0]
+    assert rel_err(ctx.download_Uw(0), ref) < 1e-13
+    ctx.project_ic(0, 1, [0.01, 1e-3, 7])
+    ref = orc.project_ic(T, H, 1, [0.01, 1e-3, 7])[0]
+    assert rel_err(ctx.download_Uw(0), ref) < 1e-13
