@@ -10,7 +10,9 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "adg.h"
@@ -39,8 +41,17 @@ struct adg_ctx {
     adg::Hierarchy H;
     adg::LevelGeo geo{};
 
-    double *Uw = nullptr, *Vol = nullptr, *Tr = nullptr, *Hw_dbg = nullptr, *Mom = nullptr, *partial = nullptr;
-    double* Fnum[3] = {nullptr, nullptr, nullptr};
+    double *Uw = nullptr, *Hw_dbg = nullptr, *Mom = nullptr, *partial = nullptr;
+    void *Vol = nullptr, *Tr = nullptr;     // element type of the form (double / float)
+    void* Fnum[3] = {nullptr, nullptr, nullptr};
+    void* Smom[3] = {nullptr, nullptr, nullptr};
+    size_t esz = sizeof(double);
+    adg::Fragments frags;
+    adg::FragOff fo{};
+    double* d_frag64 = nullptr;
+    float* d_frag32 = nullptr;
+    int pred_kind = 0;       // 0 auto, 1 VALU, 2 MFMA
+    int ncu = 256;
     int* valid_list = nullptr;
     int* cell_face[3][2] = {};
     int* face_lr[3] = {};
@@ -85,16 +96,33 @@ int dev_upload(adg_ctx* c, T** dst, const std::vector<T>& src) {
     return ADG_OK;
 }
 
-int dev_alloc(adg_ctx* c, double** dst, size_t n) {
+int dev_alloc_bytes(adg_ctx* c, void** dst, size_t bytes) {
     *dst = nullptr;
-    if (n == 0) return ADG_OK;
+    if (bytes == 0) return ADG_OK;
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, n * sizeof(double));
+    hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return fail(c, ADG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->level_allocs.push_back(p);
-    HIPCHK(c, hipMemsetAsync(p, 0, n * sizeof(double), c->stream));
-    *dst = static_cast<double*>(p);
+    HIPCHK(c, hipMemsetAsync(p, 0, bytes, c->stream));
+    *dst = p;
     return ADG_OK;
+}
+int dev_alloc(adg_ctx* c, double** dst, size_t n) {
+    void* p = nullptr;
+    int rc = dev_alloc_bytes(c, &p, n * sizeof(double));
+    *dst = static_cast<double*>(p);
+    return rc;
+}
+
+// LDS bytes of the MFMA predictor for a 512-thread workgroup.
+size_t mfma_lds_bytes(const adg_ctx* c) {
+    return ((size_t)c->frags.total * 64 + (size_t)8 * c->frags.MB2 * 256) * c->esz;
+}
+bool use_mfma(const adg_ctx* c) {
+    const bool fits = mfma_lds_bytes(c) <= 160 * 1024;
+    if (c->pred_kind == 1) return false;
+    if (c->pred_kind == 2) return fits;
+    return fits && !(c->P == 1);
 }
 
 void free_levels(adg_ctx* c) {
@@ -147,11 +175,28 @@ int pack_tables(adg_ctx* c) {
     c->to.Pfl = add(t.P_flux_low);
     c->to.Pfh = add(t.P_flux_high);
     c->to.phi0 = add(t.phi0);
+    std::vector<double> sgn((size_t)t.D * t.Np_s);
+    for (int d = 0; d < t.D; ++d)
+        for (int k = 0; k < t.Np_s; ++k) sgn[(size_t)d * t.Np_s + k] = (t.mi_s[k][d] % 2) ? -1.0 : 1.0;
+    c->to.sgn = add(sgn);
     c->to.total = (int)buf.size();
     void* p = nullptr;
     HIPCHK(c, hipMalloc(&p, buf.size() * sizeof(double)));
     c->d_tab = static_cast<double*>(p);
     HIPCHK(c, hipMemcpy(c->d_tab, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+    // MFMA A-operand fragments in both element types (K order follows each
+    // type's accumulator row order)
+    c->frags = adg::build_fragments(t, true);
+    c->fo = adg::FragOff{c->frags.A0, c->frags.V, c->frags.B, c->frags.S, c->frags.T, c->frags.total};
+    const std::vector<double>& fd = c->frags.data;
+    adg::Fragments f32 = adg::build_fragments(t, false);
+    std::vector<float> ff(f32.data.begin(), f32.data.end());
+    HIPCHK(c, hipMalloc(&p, fd.size() * sizeof(double)));
+    c->d_frag64 = static_cast<double*>(p);
+    HIPCHK(c, hipMemcpy(c->d_frag64, fd.data(), fd.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&p, ff.size() * sizeof(float)));
+    c->d_frag32 = static_cast<float*>(p);
+    HIPCHK(c, hipMemcpy(c->d_frag32, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
     return ADG_OK;
 }
 
@@ -171,6 +216,7 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
     c->Q = dim + 2;
     c->gamma = gamma;
     c->precision = precision;
+    c->esz = precision == ADG_FP32_MFMA ? sizeof(float) : sizeof(double);
     try {
         c->tab = adg::build_element_tables(p, dim);
     } catch (const std::exception& e) {
@@ -185,6 +231,13 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
     }
     int rc = pack_tables(c);
     if (rc != ADG_OK) { adg_destroy(c); return rc; }
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            c->ncu = ncu;
+    }
+    if (precision == ADG_FP32_MFMA && mfma_lds_bytes(c) > 160 * 1024) { adg_destroy(c); return ADG_E_UNSUP; }
     void* tsp = nullptr;
     if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->ts = static_cast<adg::TimeState*>(tsp);
@@ -270,10 +323,13 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
 #undef UP
     const size_t NS = (size_t)H.NS;
     if ((rc = dev_alloc(c, &c->Uw, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
-    if ((rc = dev_alloc(c, &c->Vol, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
-    if ((rc = dev_alloc(c, &c->Tr, (size_t)2 * D * Q * t.qMp_st_bd * NS)) != ADG_OK) return rc;
+    const size_t ves = (size_t)c->frags.MBS * 16 * Q, tes = (size_t)2 * D * c->frags.MBD * 16 * Q;
+    if ((rc = dev_alloc_bytes(c, &c->Vol, ves * NS * c->esz)) != ADG_OK) return rc;
+    if ((rc = dev_alloc_bytes(c, &c->Tr, tes * NS * c->esz)) != ADG_OK) return rc;
     for (int d = 0; d < D; ++d)
-        if ((rc = dev_alloc(c, &c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d])) != ADG_OK) return rc;
+        if ((rc = dev_alloc_bytes(c, &c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d] * c->esz)) != ADG_OK ||
+            (rc = dev_alloc_bytes(c, &c->Smom[d], (size_t)Q * t.Np_s * H.NF[d] * c->esz)) != ADG_OK)
+            return rc;
     if ((rc = dev_alloc(c, &c->Mom, (size_t)Q * t.Np_s * H.mom_face.size())) != ADG_OK) return rc;
     if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
@@ -345,10 +401,14 @@ int adg_download_Fnum(adg_ctx* c, int lev, int d, double* host) {
     if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev || d < 0 || d >= c->D || !host)
         return fail(c, ADG_E_ARG, "bad arguments");
     const size_t nf = (size_t)c->H.lev[lev].nface[d], off = (size_t)c->H.lev[lev].face_off[d];
-    const size_t NF = (size_t)c->H.NF[d], rows = (size_t)c->Q * c->tab.qMp_st_bd;
-    HIPCHK(c, hipMemcpy2DAsync(host, nf * sizeof(double), c->Fnum[d] + off, NF * sizeof(double), nf * sizeof(double),
-                               rows, hipMemcpyDeviceToHost, c->stream));
+    const size_t NF = (size_t)c->H.NF[d], rows = (size_t)c->Q * c->tab.qMp_st_bd, es = c->esz;
+    std::vector<unsigned char> tmp(rows * nf * es);
+    HIPCHK(c, hipMemcpy2DAsync(tmp.data(), nf * es, (unsigned char*)c->Fnum[d] + off * es, NF * es, nf * es, rows,
+                               hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < rows * nf; ++i)
+        host[i] = es == sizeof(double) ? reinterpret_cast<double*>(tmp.data())[i]
+                                       : (double)reinterpret_cast<float*>(tmp.data())[i];
     return ADG_OK;
 }
 
@@ -418,6 +478,45 @@ int adg_set_dt_value(adg_ctx* c, double Dt) {
     return ADG_OK;
 }
 
+extern "C++" {
+template <typename T, int P, int D>
+static int launch_pred_mfma(adg_ctx* c, long begin, long n) {
+    adg::PredMArgs<T> a{};
+    a.frags = std::is_same<T, double>::value ? (const T*)c->d_frag64 : (const T*)c->d_frag32;
+    a.fo = c->fo;
+    a.Uw = c->Uw;
+    a.Vol = (T*)c->Vol;
+    a.Tr = (T*)c->Tr;
+    a.Hw_dbg = c->keep_hw ? c->Hw_dbg : nullptr;
+    a.slot_lev = c->slot_lev;
+    a.ts = c->ts;
+    a.geo = c->geo;
+    a.NS = c->H.NS;
+    a.slot_begin = begin;
+    a.nelem = n;
+    a.gamma = c->gamma;
+    const size_t lds = mfma_lds_bytes(c);
+    auto kern = adg::k_predictor_mfma<T, P, D>;
+    HIPCHK(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long per_batch = 8L * adg::Dims<P, D>::EPM;
+    const long nbatch = (n + per_batch - 1) / per_batch;
+    const long per_cu = std::max<long>(1, (long)((160 * 1024) / lds));
+    const long grid = std::min<long>(nbatch, per_cu * c->ncu);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(512), lds, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+}  // extern "C++"
+
+int adg_set_predictor_kernel(adg_ctx* c, int kind) {
+    if (!c) return ADG_E_ARG;
+    if (kind < 0 || kind > 2) return fail(c, ADG_E_ARG, "kind must be 0, 1 or 2");
+    if (kind == 1 && c->precision != ADG_FP64) return fail(c, ADG_E_UNSUP, "the VALU predictor is FP64 only");
+    if (kind == 2 && mfma_lds_bytes(c) > 160 * 1024) return fail(c, ADG_E_UNSUP, "MFMA fragments exceed LDS");
+    c->pred_kind = kind;
+    return ADG_OK;
+}
+
 int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     if (mode != 0) return fail(c, ADG_E_UNSUP, "only mode 0 (P Picard iterations) is available");
@@ -426,8 +525,21 @@ int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     long begin = 0, n = c->H.NS;
     if (lev >= 0) { begin = c->H.lev[lev].slot_off; n = c->H.lev[lev].nslot; }
     if (n == 0) return ADG_OK;
-    adg::PredArgs a{c->d_tab, c->to, c->Uw, c->Vol, c->Tr, c->keep_hw ? c->Hw_dbg : nullptr, c->slot_lev, c->ts,
-                    c->geo, c->H.NS, begin, n, c->gamma};
+    if (use_mfma(c) || c->precision == ADG_FP32_MFMA) {
+        int rc = ADG_OK;
+        if (c->precision == ADG_FP64) {
+#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n)
+            ADG_DISPATCH(c, L_PM);
+#undef L_PM
+        } else {
+#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n)
+            ADG_DISPATCH(c, L_PM);
+#undef L_PM
+        }
+        return rc;
+    }
+    adg::PredArgs a{c->d_tab, c->to, c->Uw, (double*)c->Vol, (double*)c->Tr, c->keep_hw ? c->Hw_dbg : nullptr,
+                    c->slot_lev, c->ts, c->geo, c->H.NS, begin, n, c->gamma};
 #define L_PR(PP, DD)                                                                              \
     {                                                                                             \
         constexpr int epw = adg::Dims<PP, DD>::EPW;                                               \
@@ -440,43 +552,46 @@ int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     return ADG_OK;
 }
 
-int adg_face_flux(adg_ctx* c, int lev) {
-    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    (void)lev;  // faces of every level are processed in one launch
-    adg::FaceArgs a{};
-    a.Tr = c->Tr;
+extern "C++" {
+template <typename T>
+static int launch_faceflux(adg_ctx* c) {
+    adg::FaceArgs<T> a{};
+    a.tab = c->d_tab;
+    a.to = c->to;
+    a.Tr = (const T*)c->Tr;
     long tot = 0;
     for (int d = 0; d < c->D; ++d) {
-        a.Fnum[d] = c->Fnum[d];
+        a.Fnum[d] = (T*)c->Fnum[d];
+        a.Smom[d] = (T*)c->Smom[d];
         a.face_lr[d] = c->face_lr[d];
         a.NF[d] = c->H.NF[d];
-        tot += c->H.NF[d] * c->tab.qMp_st_bd;
+        tot += c->H.NF[d];
     }
-    a.NS = c->H.NS;
     a.gamma = c->gamma;
-#define L_FF(PP, DD) hipLaunchKernelGGL((adg::k_faceflux<PP, DD>), dim3(nblocks(tot)), dim3(256), 0, c->stream, a)
+#define L_FF(PP, DD) hipLaunchKernelGGL((adg::k_faceflux<T, PP, DD>), dim3(nblocks(tot, 64)), dim3(64), 0, c->stream, a)
     ADG_DISPATCH(c, L_FF);
 #undef L_FF
     HIPCHK(c, hipGetLastError());
     return ADG_OK;
 }
+}  // extern "C++"
 
-int adg_corrector(adg_ctx* c, int lev) {
+int adg_face_flux(adg_ctx* c, int lev) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    long begin = 0, n = (long)c->H.valid_list.size();
-    if (lev >= 0) {
-        if (lev >= c->H.nlev) return fail(c, ADG_E_ARG, "bad level");
-        begin = c->H.lev[lev].valid_begin;
-        n = c->H.lev[lev].valid_end - begin;
-    }
-    if (n == 0) return ADG_OK;
-    adg::CorrArgs a{};
+    (void)lev;  // faces of every level are processed in one launch
+    return c->precision == ADG_FP64 ? launch_faceflux<double>(c) : launch_faceflux<float>(c);
+}
+
+extern "C++" {
+template <typename T>
+static int launch_corrector(adg_ctx* c, long begin, long n) {
+    adg::CorrArgs<T> a{};
     a.tab = c->d_tab;
     a.to = c->to;
     a.Uw = c->Uw;
-    a.Vol = c->Vol;
+    a.Vol = (const T*)c->Vol;
     for (int d = 0; d < c->D; ++d) {
-        a.Fnum[d] = c->Fnum[d];
+        a.Smom[d] = (const T*)c->Smom[d];
         a.cell_face[d][0] = c->cell_face[d][0] + begin;
         a.cell_face[d][1] = c->cell_face[d][1] + begin;
         a.NF[d] = c->H.NF[d];
@@ -487,22 +602,37 @@ int adg_corrector(adg_ctx* c, int lev) {
     a.geo = c->geo;
     a.NS = c->H.NS;
     a.nvalid = n;
-#define L_CO(PP, DD) hipLaunchKernelGGL((adg::k_corrector<PP, DD>), dim3(nblocks(n * c->Q)), dim3(256), 0, c->stream, a)
+#define L_CO(PP, DD) hipLaunchKernelGGL((adg::k_corrector<T, PP, DD>), dim3(nblocks(n * c->Q)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_CO);
 #undef L_CO
     HIPCHK(c, hipGetLastError());
     return ADG_OK;
 }
+}  // extern "C++"
 
-int adg_reflux(adg_ctx* c, int crse_lev) {
+int adg_corrector(adg_ctx* c, int lev) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    (void)crse_lev;  // every coarse level in one launch (disjoint cells)
-    const long NM = (long)c->H.mom_face.size(), nrc = (long)c->H.rc_slot.size();
-    if (NM == 0 || nrc == 0) return ADG_OK;
-    adg::MomArgs m{};
+    long begin = 0, n = (long)c->H.valid_list.size();
+    if (lev >= 0) {
+        if (lev >= c->H.nlev) return fail(c, ADG_E_ARG, "bad level");
+        begin = c->H.lev[lev].valid_begin;
+        n = c->H.lev[lev].valid_end - begin;
+    }
+    if (n == 0) return ADG_OK;
+    return c->precision == ADG_FP64 ? launch_corrector<double>(c, begin, n) : launch_corrector<float>(c, begin, n);
+}
+
+extern "C++" {
+template <typename T>
+static int launch_moments(adg_ctx* c, long NM) {
+    adg::MomArgs<T> m{};
     m.tab = c->d_tab;
     m.to = c->to;
-    for (int d = 0; d < c->D; ++d) { m.Fnum[d] = c->Fnum[d]; m.NF[d] = c->H.NF[d]; }
+    for (int d = 0; d < c->D; ++d) {
+        m.Fnum[d] = (const T*)c->Fnum[d];
+        m.Smom[d] = (const T*)c->Smom[d];
+        m.NF[d] = c->H.NF[d];
+    }
     m.Mom = c->Mom;
     m.mom_face = c->mom_face;
     m.mom_d = c->mom_d;
@@ -512,10 +642,21 @@ int adg_reflux(adg_ctx* c, int crse_lev) {
     m.ts = c->ts;
     m.geo = c->geo;
     m.NM = NM;
-#define L_MO(PP, DD) hipLaunchKernelGGL((adg::k_moments<PP, DD>), dim3(nblocks(NM * c->Q)), dim3(256), 0, c->stream, m)
+#define L_MO(PP, DD) hipLaunchKernelGGL((adg::k_moments<T, PP, DD>), dim3(nblocks(NM * c->Q)), dim3(256), 0, c->stream, m)
     ADG_DISPATCH(c, L_MO);
 #undef L_MO
     HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+}  // extern "C++"
+
+int adg_reflux(adg_ctx* c, int crse_lev) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    (void)crse_lev;  // every coarse level in one launch (disjoint cells)
+    const long NM = (long)c->H.mom_face.size(), nrc = (long)c->H.rc_slot.size();
+    if (NM == 0 || nrc == 0) return ADG_OK;
+    int rc = c->precision == ADG_FP64 ? launch_moments<double>(c, NM) : launch_moments<float>(c, NM);
+    if (rc != ADG_OK) return rc;
     adg::RefluxArgs r{c->d_tab, c->to, c->Uw, c->Mom, c->rc_slot, c->rc_start, c->rc_count, c->rc_lev,
                       c->re_mom, c->re_coef, c->geo, c->H.NS, NM, nrc};
 #define L_RF(PP, DD) hipLaunchKernelGGL((adg::k_reflux<PP, DD>), dim3(nblocks(nrc * c->Q)), dim3(256), 0, c->stream, r)
@@ -531,6 +672,17 @@ int adg_ader(adg_ctx* c) {
     if ((rc = adg_face_flux(c, -1)) != ADG_OK) return rc;
     if ((rc = adg_corrector(c, -1)) != ADG_OK) return rc;
     if (c->H.nlev > 1 && (rc = adg_reflux(c, -1)) != ADG_OK) return rc;
+    return ADG_OK;
+}
+
+int adg_advance(adg_ctx* c, double c_dt, double T, double* t_out) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    int rc;
+    if ((rc = launch_cfl(c, c_dt, T, 1)) != ADG_OK) return rc;
+    if (t_out) {
+        HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return ADG_OK;
 }
 
@@ -589,6 +741,28 @@ int adg_get_table(adg_ctx* c, const char* name, double* out, int cap) {
     return (int)v->size();
 }
 
+// Lane-layout probe of the MFMA builtins used by the predictor (diagnostic;
+// not part of include/adg.h).  prec 0: f64, 1: f32; out receives 256 doubles.
+int adg_probe_mfma_layout(int prec, int mode, double* out) {
+    if (!out) return ADG_E_ARG;
+    void* d = nullptr;
+    if (hipMalloc(&d, 256 * sizeof(double)) != hipSuccess) return ADG_E_NOMEM;
+    if (prec == 0) hipLaunchKernelGGL(adg::k_probe_mfma<double>, dim3(1), dim3(64), 0, 0, (double*)d, mode);
+    else hipLaunchKernelGGL(adg::k_probe_mfma<float>, dim3(1), dim3(64), 0, 0, (float*)d, mode);
+    std::vector<float> f(256);
+    int rc = ADG_OK;
+    if (hipDeviceSynchronize() != hipSuccess) rc = ADG_E_HIP;
+    if (rc == ADG_OK) {
+        if (prec == 0) rc = hipMemcpy(out, d, 256 * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess ? 0 : ADG_E_HIP;
+        else {
+            rc = hipMemcpy(f.data(), d, 256 * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? 0 : ADG_E_HIP;
+            for (int i = 0; i < 256; ++i) out[i] = f[i];
+        }
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
 const char* adg_last_error(adg_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 void adg_destroy(adg_ctx* c) {
@@ -596,6 +770,8 @@ void adg_destroy(adg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_levels(c);
     if (c->d_tab) (void)hipFree(c->d_tab);
+    if (c->d_frag64) (void)hipFree(c->d_frag64);
+    if (c->d_frag32) (void)hipFree(c->d_frag32);
     if (c->ts) (void)hipFree(c->ts);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -1,13 +1,18 @@
-// adg_kernels.hpp -- gfx950 kernels of the ADER-DG hot path (FP64 form).
+// adg_kernels.hpp -- gfx950 kernels of the ADER-DG hot path.
 //
-// Data layout in HBM (structure of arrays, cells contiguous):
-//   Uw   [Q][Np_s][NS]            modal solution, every slot of every level
-//   Vol  [Q][Np_s][NS]            corrector volume term sum_d (Dt/dx_d) Sk_corr[d] F_d
-//   Tr   [D][2][Q][qMp_bd][NS]    predictor traces on the faces (side 0: xi=-1, 1: +1)
-//   Fnum [d]: [Q][qMp_bd][NF_d]   LLF flux at the face quadrature nodes
-//   Mom  [Q][Np_s][NM]            flux-register face moments
-// Element matrices live in one read-only buffer; every index into it is
-// wave-uniform, so the loads are scalar (SGPR operands of v_fma_f64).
+// Data layout in HBM:
+//   Uw   [Q][Np_s][NS]  FP64        modal solution, every slot of every level
+//   Vol  per slot, [MBS][4][4][Q]   corrector volume term sum_d (Dt/dx_d) Sk_corr[d] F_d
+//   Tr   per slot, [2D][MBD][4][4][Q] predictor traces on the faces
+//                                    (side 0: xi_d=-1, side 1: xi_d=+1)
+//   Fnum [d]: [Q][qMp_st_bd][NF_d]   LLF flux at the face quadrature nodes
+//   Smom [d]: [Q][Np_s][NF_d]        face moments sum_m Mkbdp[d](k,m) Fnum[m]
+//   Mom  [Q][Np_s][NM]  FP64         flux-register face moments
+// Vol and Tr are stored in the order the 16x16 MFMA accumulators hold them
+// (row = 16*block + 4*lane_group + register), so a wavefront writes each
+// element's block as contiguous 16*Q values.  Vol/Tr/Fnum use the form's
+// element type T (double, or float in the FP32-MFMA form); U_w, reflux and
+// all reductions stay FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,6 +28,7 @@ __host__ __device__ constexpr int binom(int n, int k) {
     return r;
 }
 __host__ __device__ constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+__host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 template <int P, int D>
 struct Dims {
@@ -35,12 +41,51 @@ struct Dims {
     static constexpr int Mbd = ipow_c(N, D);
     static constexpr int NCH = 1 << D;
     static constexpr int NFC = 1 << (D - 1);
-    static constexpr int EPW = 64 / Q;  // elements per wavefront (lane = element * Q + q)
+    static constexpr int EPW = 64 / Q;   // VALU predictor: elements per wavefront
+    static constexpr int EPM = 16 / Q;   // MFMA predictor: elements per 16 columns
+    // MFMA blocking
+    static constexpr int MB2 = cdiv(Nst, 16);  // row blocks of the H_w update
+    static constexpr int KSU = cdiv(Ns, 4);    // K steps over U_w modes
+    static constexpr int NB = cdiv(Mst, 16);   // node blocks
+    static constexpr int KSH = cdiv(Nst, 4);   // K steps over H_w modes
+    static constexpr int MBS = cdiv(Ns, 16);   // row blocks of the corrector volume term
+    static constexpr int MBD = cdiv(Mbd, 16);  // row blocks of a face trace
+    static constexpr int TES = 2 * D * MBD * 16 * Q;  // Tr entries per slot
+    static constexpr int VES = MBS * 16 * Q;          // Vol entries per slot
 };
+
+// Row of a 16x16x4 MFMA accumulator held by register r of lane group g
+// (g = lane/16), measured on gfx950: FP32 row = 4g + r, FP64 row = 4r + g.
+template <typename T>
+__host__ __device__ constexpr int acc_row(int g, int r) {
+    return sizeof(T) == 8 ? 4 * r + g : 4 * g + r;
+}
+
+// Index of trace node mb, variable q, direction/side ds inside a slot's Tr
+// block, and of mode k inside its Vol block: the position a wavefront's
+// accumulator store puts it at (register r, lane group g -> ((r*4)+g)*Q+q).
+template <typename T, int P, int D>
+__host__ __device__ __forceinline__ int tr_off(int ds, int mb, int q) {
+    using Dm = Dims<P, D>;
+    const int blk = mb >> 4, row = mb & 15;
+    const int r = sizeof(T) == 8 ? row >> 2 : row & 3, g = sizeof(T) == 8 ? row & 3 : row >> 2;
+    return (((ds * Dm::MBD + blk) * 4 + r) * 4 + g) * Dm::Q + q;
+}
+template <typename T, int P, int D>
+__host__ __device__ __forceinline__ int vol_off(int k, int q) {
+    using Dm = Dims<P, D>;
+    const int blk = k >> 4, row = k & 15;
+    const int r = sizeof(T) == 8 ? row >> 2 : row & 3, g = sizeof(T) == 8 ? row & 3 : row >> 2;
+    return ((blk * 4 + r) * 4 + g) * Dm::Q + q;
+}
 
 // Offsets (in doubles) of every table inside the device table buffer.
 struct TabOff {
-    int A0, B, V, Skc, Vbd, Mkbdm, Mkbdp, Mkc, Minv, Pcf, Pfc, Pfl, Pfh, phi0, total;
+    int A0, B, V, Skc, Vbd, Mkbdm, Mkbdp, Mkc, Minv, Pcf, Pfc, Pfl, Pfh, phi0, sgn, total;
+};
+// Offsets (in 64-entry fragments) of the MFMA A-operand fragment sets.
+struct FragOff {
+    int A0, V, B, S, T, total;
 };
 
 struct LevelGeo {
@@ -61,64 +106,61 @@ static constexpr double kNumLimit = 10.0 * 2.220446049250313e-16;  // 10 * DBL_E
 
 // ---------------------------------------------------------------------------
 // Euler pointwise model (Compressible_Euler.h:370-604, .cpp:50-70).
-template <int D>
-__device__ __forceinline__ double pressure(const double* s, double rinv, double gm1) {
-    double k = 0.0;
+template <int D, typename T>
+__device__ __forceinline__ T pressure(const T* s, T rinv, T gm1) {
+    T k = T(0);
 #pragma unroll
     for (int a = 0; a < D; ++a) k += s[a + 1] * s[a + 1] * rinv;
-    double p = (-0.5 * k + s[D + 1]) * gm1;
-    return fmax(p, kNumLimit);
+    T p = (T(-0.5) * k + s[D + 1]) * gm1;
+    return fmax(p, T(kNumLimit));
 }
 
 // Flux component q of direction d, branch-free in q (q varies across lanes).
-template <int D>
-__device__ __forceinline__ double flux_q(int d, int q, const double* s, double rinv, double prs) {
-    const double md = s[d + 1];
-    const double ud = md * rinv;
-    double sq = s[0];
-#pragma unroll
-    for (int a = 1; a < D + 2; ++a) sq = (q == a) ? s[a] : sq;
-    double mom = ud * sq + ((q == d + 1) ? prs : 0.0);
-    double en = (s[D + 1] + prs) * ud;
+// sq is the conserved variable q itself (the lane's own value), which keeps
+// the compiler from turning a select chain into a private-memory lookup.
+template <int D, typename T>
+__device__ __forceinline__ T flux_q(int d, int q, const T* s, T sq, T rinv, T prs) {
+    const T md = s[d + 1];
+    const T ud = md * rinv;
+    T mom = ud * sq + ((q == d + 1) ? prs : T(0));
+    T en = (s[D + 1] + prs) * ud;
     return (q == 0) ? md : ((q == D + 1) ? en : mom);
 }
 
 // ---------------------------------------------------------------------------
-// Broadcast of the value held by lane (base + src) of each element group.
-template <int Q>
-__device__ __forceinline__ double group_bcast(double v, int src, int lane_base) {
+// DPP quad_perm broadcast of lane `src` of every quad (no LDS traffic).
+template <int SRC>
+__device__ __forceinline__ int quad_bcast_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, SRC * 0x55, 0xf, 0xf, false);
+}
+template <int SRC>
+__device__ __forceinline__ double quad_bcast(double v) {
+    return __hiloint2double(quad_bcast_i<SRC>(__double2hiint(v)), quad_bcast_i<SRC>(__double2loint(v)));
+}
+template <int SRC>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __int_as_float(quad_bcast_i<SRC>(__float_as_int(v)));
+}
+
+// Gather the Q conserved variables of this lane's element: lanes src_base+k.
+template <int Q, typename T>
+__device__ __forceinline__ void gather_state(T v, int src_base, T* s) {
     if constexpr (Q == 4) {
-        // DPP quad_perm broadcast inside each quad of lanes: no LDS traffic.
-        int lo = __double2loint(v), hi = __double2hiint(v);
-        int rlo, rhi;
-        switch (src) {
-            case 0:
-                rlo = __builtin_amdgcn_mov_dpp(lo, 0x00, 0xf, 0xf, false);
-                rhi = __builtin_amdgcn_mov_dpp(hi, 0x00, 0xf, 0xf, false);
-                break;
-            case 1:
-                rlo = __builtin_amdgcn_mov_dpp(lo, 0x55, 0xf, 0xf, false);
-                rhi = __builtin_amdgcn_mov_dpp(hi, 0x55, 0xf, 0xf, false);
-                break;
-            case 2:
-                rlo = __builtin_amdgcn_mov_dpp(lo, 0xaa, 0xf, 0xf, false);
-                rhi = __builtin_amdgcn_mov_dpp(hi, 0xaa, 0xf, 0xf, false);
-                break;
-            default:
-                rlo = __builtin_amdgcn_mov_dpp(lo, 0xff, 0xf, 0xf, false);
-                rhi = __builtin_amdgcn_mov_dpp(hi, 0xff, 0xf, 0xf, false);
-                break;
-        }
-        return __hiloint2double(rhi, rlo);
+        s[0] = quad_bcast<0>(v);
+        s[1] = quad_bcast<1>(v);
+        s[2] = quad_bcast<2>(v);
+        s[3] = quad_bcast<3>(v);
     } else {
-        return __shfl(v, lane_base + src, 64);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) s[k] = __shfl(v, src_base + k, 64);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Predictor: set_predictor guess, P Picard iterations
-//   H_w <- A0 U_w + sum_d (Dt/dx_d) B_d F_d(V H_w),  B_d = -Mk_h_w^-1 Sk_predVinv[d]
-// then the final nodal evaluation: corrector volume term and face traces.
+// VALU predictor (FP64): lane = (element, q); set_predictor guess, P Picard
+// iterations  H_w <- A0 U_w + sum_d (Dt/dx_d) B_d F_d(V H_w),
+// B_d = -Mk_h_w^-1 Sk_predVinv[d]; then the corrector volume term and traces.
+// Element matrices are read through wave-uniform (scalar) loads.
 struct PredArgs {
     const double* __restrict__ tab;
     TabOff to;
@@ -143,12 +185,12 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int el = lane / Q, q = lane - (lane / Q) * Q;
     const bool lane_ok = lane < EPW * Q;
-    if (!lane_ok) { el = EPW - 1; q = Q - 1; }
+    if (!lane_ok) { el = EPW - 1; q = 0; }
     long e = wave * EPW + el;
     const bool active = lane_ok && e < a.nelem;
     if (e >= a.nelem) e = a.nelem - 1;
     const long slot = a.slot_begin + e;
-    const int lane_base = lane - q;
+    const int lane_base = lane_ok ? lane - q : lane;
     const int lev = a.slot_lev[slot];
     const double Dt = a.ts->Dt;
     double cd[D];
@@ -187,13 +229,13 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
 #pragma unroll
             for (int n = 0; n < Nst; ++n) h = fma(V[m * Nst + n], hw[n], h);
             double s[Q];
-#pragma unroll
-            for (int k = 0; k < Q; ++k) s[k] = group_bcast<Q>(h, k, lane_base);
+            if constexpr (Q == 4) gather_state<Q>(h, lane_base, s);
+            else for (int k = 0; k < Q; ++k) s[k] = __shfl(h, lane_base + k, 64);
             const double rinv = 1.0 / s[0];
             const double prs = pressure<D>(s, rinv, gm1);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const double g = cd[d] * flux_q<D>(d, q, s, rinv, prs);
+                const double g = cd[d] * flux_q<D>(d, q, s, h, rinv, prs);
                 const double* __restrict__ Bd = B + (long)d * Nst * Mst + m;
 #pragma unroll
                 for (int n = 0; n < Nst; ++n) acc[n] = fma(Bd[n * Mst], g, acc[n]);
@@ -203,7 +245,6 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
         for (int n = 0; n < Nst; ++n) hw[n] = acc[n];
     }
 
-    // final evaluation: corrector volume term sum_d (Dt/dx_d) Sk_corr[d] F_d
     double vol[Ns];
 #pragma unroll
     for (int k = 0; k < Ns; ++k) vol[k] = 0.0;
@@ -212,13 +253,13 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
 #pragma unroll
         for (int n = 0; n < Nst; ++n) h = fma(V[m * Nst + n], hw[n], h);
         double s[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) s[k] = group_bcast<Q>(h, k, lane_base);
+        if constexpr (Q == 4) gather_state<Q>(h, lane_base, s);
+        else for (int k = 0; k < Q; ++k) s[k] = __shfl(h, lane_base + k, 64);
         const double rinv = 1.0 / s[0];
         const double prs = pressure<D>(s, rinv, gm1);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const double g = cd[d] * flux_q<D>(d, q, s, rinv, prs);
+            const double g = cd[d] * flux_q<D>(d, q, s, h, rinv, prs);
             const double* __restrict__ Sd = Skc + (long)d * Ns * Mst + m;
 #pragma unroll
             for (int k = 0; k < Ns; ++k) vol[k] = fma(Sd[k * Mst], g, vol[k]);
@@ -226,89 +267,315 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
     }
     if (active) {
 #pragma unroll
-        for (int k = 0; k < Ns; ++k) a.Vol[((long)q * Ns + k) * NS + slot] = vol[k];
+        for (int k = 0; k < Ns; ++k) a.Vol[slot * Dm::VES + vol_off<double, P, D>(k, q)] = vol[k];
         if (a.Hw_dbg) {
 #pragma unroll
             for (int n = 0; n < Nst; ++n) a.Hw_dbg[((long)q * Nst + n) * NS + slot] = hw[n];
         }
     }
-    // face traces (get_H_from_H_w_bd) for every direction and side
     for (int ds = 0; ds < 2 * D; ++ds) {
         const double* __restrict__ Vb = Vbd + (long)ds * Mbd * Nst;
         for (int mb = 0; mb < Mbd; ++mb) {
             double tr = 0.0;
 #pragma unroll
             for (int n = 0; n < Nst; ++n) tr = fma(Vb[mb * Nst + n], hw[n], tr);
-            if (active) a.Tr[(((long)ds * Q + q) * Mbd + mb) * NS + slot] = tr;
+            if (active) a.Tr[slot * Dm::TES + tr_off<double, P, D>(ds, mb, q)] = tr;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Face flux: LLF at every face quadrature node (AmrDG_NumericalFlux.tpp:4-41)
-// from the traces of the left (side +) and right (side -) cells.
-struct FaceArgs {
-    const double* __restrict__ Tr;
-    double* __restrict__ Fnum[3];
-    const int* __restrict__ face_lr[3];
-    long NF[3];
+// MFMA predictor.  One wavefront owns 16 matrix columns = EPM elements x Q
+// variables (column c = Q*e + q).  Every contraction is a 16x16x4 MFMA with
+// the element matrix as the A operand (pre-swizzled fragments, staged once
+// per persistent workgroup in LDS) and the per-element data as B:
+//   C0 = A0 * U_w                       (Nst x Ns)
+//   H  = V * H_w                        nodal state, 16 nodes per block
+//   g  = (Dt/dx_d) F_d(H)               pointwise, in the accumulator layout
+//   H_w' = C0 + sum_d B_d * g_d          K ordered so that the accumulator
+//                                        registers of H feed B directly
+// The H_w' accumulators are re-laid as B operands through a per-wave LDS
+// buffer.  Accumulator rows follow acc_row<T>(lane group, register).  Final stage: Vol = sum_d Sk_corr[d] g_d and the face traces.
+template <typename T>
+struct MF;
+template <>
+struct MF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename T>
+struct PredMArgs {
+    const T* __restrict__ frags;   // [nfrag][64]
+    FragOff fo;
+    const double* __restrict__ Uw;
+    T* __restrict__ Vol;
+    T* __restrict__ Tr;
+    double* __restrict__ Hw_dbg;   // nullable
+    const uint8_t* __restrict__ slot_lev;
+    const TimeState* __restrict__ ts;
+    LevelGeo geo;
     long NS;
+    long slot_begin;
+    long nelem;
     double gamma;
 };
 
-template <int P, int D>
-__global__ void __launch_bounds__(256) k_faceflux(FaceArgs a) {
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Mbd = Dm::Mbd;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int d = 0;
-    long tot = 0;
+    using M = MF<T>;
+    using v4 = typename M::v4;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst, Mst = Dm::Mst, EPM = Dm::EPM;
+    constexpr int MB2 = Dm::MB2, KSU = Dm::KSU, NB = Dm::NB, KSH = Dm::KSH, MBS = Dm::MBS, MBD = Dm::MBD;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    T* fr = reinterpret_cast<T*>(smem_raw);
+    const int nfr = a.fo.total * 64;
+    for (int i = threadIdx.x; i < nfr; i += blockDim.x) fr[i] = a.frags[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    T* xch = fr + nfr + wave * (MB2 * 16 * 16);
+    const int g = lane >> 4, c = lane & 15;
+    const bool col_ok = c < EPM * Q;
+    const int e_loc = col_ok ? c / Q : 0;
+    const int q = col_ok ? c - e_loc * Q : 0;
+    const int src_base = col_ok ? (lane - q) : lane;
+    const T gm1 = T(a.gamma - 1.0);
+    const long per_batch = (long)nwave * EPM;
+    const long nbatch = (a.nelem + per_batch - 1) / per_batch;
+
+    for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
+        const long e = batch * per_batch + (long)wave * EPM + e_loc;
+        const bool active = col_ok && e < a.nelem;
+        const long slot = a.slot_begin + (e < a.nelem ? e : a.nelem - 1);
+        const int lev = a.slot_lev[slot];
+        const double Dt = a.ts->Dt;
+        T cd[D];
 #pragma unroll
-    for (int dd = 0; dd < D; ++dd) {
-        long nd = a.NF[dd] * Mbd;
-        if (t >= tot && t < tot + nd) d = dd;
-        tot += nd;
-    }
-    if (t >= tot) return;
-    long base = 0;
-    for (int dd = 0; dd < d; ++dd) base += a.NF[dd] * Mbd;
-    long r = t - base;
-    const long NFd = a.NF[d];
-    const int mb = (int)(r / NFd);
-    const long f = r - (long)mb * NFd;
-    const long sl = a.face_lr[d][2 * f], sr = a.face_lr[d][2 * f + 1];
-    const long NS = a.NS;
-    double uL[Q], uR[Q];
+        for (int d = 0; d < D; ++d) cd[d] = T(Dt / a.geo.dx[lev][d]);
+
+        // U_w as B operand (modes 4ks+g), then C0 = A0 U_w and the initial
+        // guess H_w[mode] = U_w[mode] for mode < Np_s (AmrDG.tpp:522-533).
+        T hb[KSH];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        uL[q] = a.Tr[(((long)(2 * d + 1) * Q + q) * Mbd + mb) * NS + sl];
-        uR[q] = a.Tr[(((long)(2 * d + 0) * Q + q) * Mbd + mb) * NS + sr];
-    }
-    const double gm1 = a.gamma - 1.0;
-    const double riL = 1.0 / uL[0], riR = 1.0 / uR[0];
-    const double pL = pressure<D>(uL, riL, gm1), pR = pressure<D>(uR, riR, gm1);
-    const double cL = sqrt(a.gamma * (pL / fmax(uL[0], kNumLimit)));
-    const double cR = sqrt(a.gamma * (pR / fmax(uR[0], kNumLimit)));
-    const double unL = uL[d + 1] * riL, unR = uR[d + 1] * riR;
-    const double dfL = fmax(fmax(fabs(unL - cL), fabs(unL)), fabs(unL + cL));
-    const double dfR = fmax(fmax(fabs(unR - cR), fabs(unR)), fabs(unR + cR));
-    const double C = fmax(fabs(dfL), fabs(dfR));
+        for (int ks = 0; ks < KSH; ++ks) {
+            const int k = 4 * ks + g;
+            hb[ks] = (k < Ns) ? T(a.Uw[((long)q * Ns + k) * a.NS + slot]) : T(0);
+        }
+        v4 c0[MB2];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const double fL = flux_q<D>(d, q, uL, riL, pL);
-        const double fR = flux_q<D>(d, q, uR, riR, pR);
-        a.Fnum[d][((long)q * Mbd + mb) * NFd + f] = 0.5 * ((fL + fR) - C * (uR[q] - uL[q]));
+        for (int mb = 0; mb < MB2; ++mb) {
+            c0[mb] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KSU; ++ks)
+                c0[mb] = M::mma(fr[(a.fo.A0 + mb * KSU + ks) * 64 + lane], hb[ks], c0[mb]);
+        }
+
+        for (int it = 0; it < P; ++it) {
+            v4 acc[MB2];
+#pragma unroll
+            for (int mb = 0; mb < MB2; ++mb) acc[mb] = c0[mb];
+#pragma unroll 1
+            for (int nb = 0; nb < NB; ++nb) {
+                v4 h = v4{0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) h = M::mma(fr[(a.fo.V + nb * KSH + ks) * 64 + lane], hb[ks], h);
+                T gd[D][4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    T s[Q];
+                    gather_state<Q>(h[r], src_base, s);
+                    const bool ok = col_ok && (16 * nb + acc_row<T>(g, r)) < Mst;
+                    if (!ok) s[0] = T(1);
+                    const T rinv = T(1) / s[0];
+                    const T prs = pressure<D>(s, rinv, gm1);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) gd[d][r] = ok ? cd[d] * flux_q<D>(d, q, s, h[r], rinv, prs) : T(0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int mb = 0; mb < MB2; ++mb)
+                            acc[mb] = M::mma(fr[(a.fo.B + ((mb * D + d) * NB + nb) * 4 + r) * 64 + lane], gd[d][r],
+                                             acc[mb]);
+            }
+            // accumulator layout (mode 16mb+4g+r) -> B operand layout (mode 4ks+g)
+#pragma unroll
+            for (int mb = 0; mb < MB2; ++mb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) xch[(16 * mb + acc_row<T>(g, r)) * 16 + c] = acc[mb][r];
+            wave_lds_sync();
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) hb[ks] = xch[(4 * ks + g) * 16 + c];
+            wave_lds_sync();
+        }
+
+        // final evaluation: corrector volume term
+        v4 vol[MBS];
+#pragma unroll
+        for (int mbs = 0; mbs < MBS; ++mbs) vol[mbs] = v4{0, 0, 0, 0};
+#pragma unroll 1
+        for (int nb = 0; nb < NB; ++nb) {
+            v4 h = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) h = M::mma(fr[(a.fo.V + nb * KSH + ks) * 64 + lane], hb[ks], h);
+            T gd[D][4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                T s[Q];
+                gather_state<Q>(h[r], src_base, s);
+                const bool ok = col_ok && (16 * nb + acc_row<T>(g, r)) < Mst;
+                if (!ok) s[0] = T(1);
+                const T rinv = T(1) / s[0];
+                const T prs = pressure<D>(s, rinv, gm1);
+#pragma unroll
+                for (int d = 0; d < D; ++d) gd[d][r] = ok ? cd[d] * flux_q<D>(d, q, s, h[r], rinv, prs) : T(0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+#pragma unroll
+                    for (int mbs = 0; mbs < MBS; ++mbs)
+                        vol[mbs] = M::mma(fr[(a.fo.S + ((mbs * D + d) * NB + nb) * 4 + r) * 64 + lane], gd[d][r],
+                                          vol[mbs]);
+        }
+        if (active) {
+            T* vo = a.Vol + slot * Dm::VES;
+#pragma unroll
+            for (int mbs = 0; mbs < MBS; ++mbs)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) vo[((mbs * 4 + r) * 4 + g) * Q + q] = vol[mbs][r];
+            if (a.Hw_dbg) {
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) {
+                    const int mode = 4 * ks + g;
+                    if (mode < Nst) a.Hw_dbg[((long)q * Nst + mode) * a.NS + slot] = double(hb[ks]);
+                }
+            }
+        }
+        // face traces: Tr[ds][mb] = Vbd[ds] * H_w
+        T* to = a.Tr + slot * Dm::TES;
+#pragma unroll 1
+        for (int ds = 0; ds < 2 * D; ++ds)
+#pragma unroll
+            for (int mbd = 0; mbd < MBD; ++mbd) {
+                v4 t = v4{0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks)
+                    t = M::mma(fr[(a.fo.T + (ds * MBD + mbd) * KSH + ks) * 64 + lane], hb[ks], t);
+                if (active) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) to[(((ds * MBD + mbd) * 4 + r) * 4 + g) * Q + q] = t[r];
+                }
+            }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Corrector on valid cells (AmrDG.tpp:652-742).
+// Face flux: LLF at every face quadrature node (AmrDG_NumericalFlux.tpp:4-41)
+// from the traces of the left (side +) and right (side -) cells, computed
+// once per face.  The face moments S[q][k] = sum_m Mkbdp[d](k,m) Fnum[q][m]
+// are summed here, once: the left cell (its + face) subtracts S, the right
+// cell (its - face) adds (-1)^{k_d} S since Mkbdm[d](k,m) = (-1)^{k_d}
+// Mkbdp[d](k,m) exactly, and the flux registers reuse the same S.  Both
+// neighbours and the reflux therefore see bitwise equal mode-0 moments.
+template <typename T>
+struct FaceArgs {
+    const double* __restrict__ tab;
+    TabOff to;
+    const T* __restrict__ Tr;
+    T* __restrict__ Fnum[3];
+    T* __restrict__ Smom[3];
+    const int* __restrict__ face_lr[3];
+    long NF[3];
+    double gamma;
+};
+
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(256) k_faceflux(FaceArgs<T> a) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Mbd = Dm::Mbd, Ns = Dm::Ns;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int d = 0;
+    long base = 0, tot = 0;
+#pragma unroll
+    for (int dd = 0; dd < D; ++dd) {
+        if (t >= tot + a.NF[dd]) base = tot + a.NF[dd], d = dd + 1;
+        tot += a.NF[dd];
+    }
+    if (t >= tot) return;
+    const long f = t - base;
+    const long NFd = a.NF[d];
+    const long sl = a.face_lr[d][2 * f], sr = a.face_lr[d][2 * f + 1];
+    const T* __restrict__ Mp = nullptr;
+    const double* __restrict__ Mbp = a.tab + a.to.Mkbdp + (long)d * Ns * Mbd;
+    const T gm = T(a.gamma), gm1 = T(a.gamma - 1.0);
+    T S[Q][Ns];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int k = 0; k < Ns; ++k) S[q][k] = T(0);
+    (void)Mp;
+    for (int mb = 0; mb < Mbd; ++mb) {
+        T uL[Q], uR[Q];
+        const T* pl = a.Tr + sl * Dm::TES + tr_off<T, P, D>(2 * d + 1, mb, 0);
+        const T* pr = a.Tr + sr * Dm::TES + tr_off<T, P, D>(2 * d + 0, mb, 0);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) { uL[q] = pl[q]; uR[q] = pr[q]; }
+        const T riL = T(1) / uL[0], riR = T(1) / uR[0];
+        const T pL = pressure<D>(uL, riL, gm1), pR = pressure<D>(uR, riR, gm1);
+        const T cL = sqrt(gm * (pL / fmax(uL[0], T(kNumLimit))));
+        const T cR = sqrt(gm * (pR / fmax(uR[0], T(kNumLimit))));
+        const T unL = uL[d + 1] * riL, unR = uR[d + 1] * riR;
+        const T dfL = fmax(fmax(fabs(unL - cL), fabs(unL)), fabs(unL + cL));
+        const T dfR = fmax(fmax(fabs(unR - cR), fabs(unR)), fabs(unR + cR));
+        const T C = fmax(fabs(dfL), fabs(dfR));
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const T fL = flux_q<D>(d, q, uL, uL[q], riL, pL);
+            const T fR = flux_q<D>(d, q, uR, uR[q], riR, pR);
+            const T F = T(0.5) * ((fL + fR) - C * (uR[q] - uL[q]));
+            a.Fnum[d][((long)q * Mbd + mb) * NFd + f] = F;
+#pragma unroll
+            for (int k = 0; k < Ns; ++k) S[q][k] = fma(T(Mbp[k * Mbd + mb]), F, S[q][k]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int k = 0; k < Ns; ++k) a.Smom[d][((long)q * Ns + k) * NFd + f] = S[q][k];
+}
+
+// ---------------------------------------------------------------------------
+// Corrector on valid cells (AmrDG.tpp:652-742) from the per-face moments:
+//   U_w <- [M U_w + Vol - sum_d (Dt/dx_d) (S_hi - (-1)^{k_d} S_lo)] / M
+// with each face product subtracted / added on its own, in FP64.
+template <typename T>
 struct CorrArgs {
     const double* __restrict__ tab;
     TabOff to;
     double* __restrict__ Uw;
-    const double* __restrict__ Vol;
-    const double* __restrict__ Fnum[3];
+    const T* __restrict__ Vol;
+    const T* __restrict__ Smom[3];
     const int* __restrict__ cell_face[3][2];
     const int* __restrict__ valid_list;
     const uint8_t* __restrict__ slot_lev;
@@ -319,10 +586,10 @@ struct CorrArgs {
     long nvalid;
 };
 
-template <int P, int D>
-__global__ void __launch_bounds__(256) k_corrector(CorrArgs a) {
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nvalid * Q) return;
     const int q = (int)(t / a.nvalid);
@@ -332,23 +599,20 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs a) {
     const double Dt = a.ts->Dt;
     const long NS = a.NS;
     const double* __restrict__ Mkc = a.tab + a.to.Mkc;
-    const double* __restrict__ Mbm = a.tab + a.to.Mkbdm;
-    const double* __restrict__ Mbp = a.tab + a.to.Mkbdp;
+    const double* __restrict__ sgn = a.tab + a.to.sgn;
     double r[Ns];
 #pragma unroll
     for (int k = 0; k < Ns; ++k)
-        r[k] = Mkc[k] * a.Uw[((long)q * Ns + k) * NS + slot] + a.Vol[((long)q * Ns + k) * NS + slot];
+        r[k] = Mkc[k] * a.Uw[((long)q * Ns + k) * NS + slot] + double(a.Vol[slot * Dm::VES + vol_off<T, P, D>(k, q)]);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const double cdd = Dt / a.geo.dx[lev][d];
         const long flo = a.cell_face[d][0][i], fhi = a.cell_face[d][1][i];
-        const double* __restrict__ F = a.Fnum[d] + (long)q * Mbd * a.NF[d];
-        for (int mb = 0; mb < Mbd; ++mb) {
-            const double Fh = F[(long)mb * a.NF[d] + fhi];
-            const double Fl = F[(long)mb * a.NF[d] + flo];
+        const T* __restrict__ S = a.Smom[d] + (long)q * Ns * a.NF[d];
 #pragma unroll
-            for (int k = 0; k < Ns; ++k)
-                r[k] -= cdd * (Mbp[((long)d * Ns + k) * Mbd + mb] * Fh - Mbm[((long)d * Ns + k) * Mbd + mb] * Fl);
+        for (int k = 0; k < Ns; ++k) {
+            r[k] -= cdd * double(S[(long)k * a.NF[d] + fhi]);
+            r[k] += cdd * (sgn[d * Ns + k] * double(S[(long)k * a.NF[d] + flo]));
         }
     }
 #pragma unroll
@@ -356,12 +620,16 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Flux-register face moments: coarse faces J * Mkbd{p|m}[d] Fnum, fine faces
-// J * P_flux_{high|low}[d][child] Fnum (AmrDG.tpp:604-647).
+// Flux-register face moments (FP64): coarse faces J * Mkbd{p|m}[d] Fnum =
+// J (+-1)^{k_d} S, fine faces J * P_flux_{high|low}[d][child] Fnum whose mode
+// 0 row equals the quadrature weights, so it is taken from S as well
+// (AmrDG.tpp:604-647).
+template <typename T>
 struct MomArgs {
     const double* __restrict__ tab;
     TabOff to;
-    const double* __restrict__ Fnum[3];
+    const T* __restrict__ Fnum[3];
+    const T* __restrict__ Smom[3];
     double* __restrict__ Mom;
     const int* __restrict__ mom_face;
     const uint8_t* __restrict__ mom_d;
@@ -374,8 +642,8 @@ struct MomArgs {
     long NM;
 };
 
-template <int P, int D>
-__global__ void __launch_bounds__(256) k_moments(MomArgs a) {
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -385,21 +653,24 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs a) {
     const int d = a.mom_d[j], lev = a.mom_lev[j], sel = a.mom_sel[j];
     const long f = a.mom_face[j];
     const double jac = (a.ts->Dt / 2.0) * (a.geo.area[lev][d] / (double)(1 << (D - 1)));
-    const double* __restrict__ M;
+    const T* __restrict__ S = a.Smom[d] + (long)q * Ns * a.NF[d] + f;
+    double s[Ns];
     if (a.mom_kind[j] == 0) {
-        M = a.tab + ((sel & 1) ? a.to.Mkbdp : a.to.Mkbdm) + (long)d * Ns * Mbd;
+        const double* __restrict__ sgn = a.tab + a.to.sgn + d * Ns;
+#pragma unroll
+        for (int n = 0; n < Ns; ++n) s[n] = (sel & 1) ? double(S[(long)n * a.NF[d]]) : sgn[n] * double(S[(long)n * a.NF[d]]);
     } else {
         const int child = sel >> 1;
-        M = a.tab + ((sel & 1) ? a.to.Pfh : a.to.Pfl) + ((long)d * NFC + child) * Ns * Mbd;
-    }
-    const double* __restrict__ F = a.Fnum[d] + (long)q * Mbd * a.NF[d] + f;
-    double s[Ns];
+        const double* __restrict__ M = a.tab + ((sel & 1) ? a.to.Pfh : a.to.Pfl) + ((long)d * NFC + child) * Ns * Mbd;
+        const T* __restrict__ F = a.Fnum[d] + (long)q * Mbd * a.NF[d] + f;
 #pragma unroll
-    for (int n = 0; n < Ns; ++n) s[n] = 0.0;
-    for (int mb = 0; mb < Mbd; ++mb) {
-        const double fv = F[(long)mb * a.NF[d]];
+        for (int n = 0; n < Ns; ++n) s[n] = 0.0;
+        for (int mb = 0; mb < Mbd; ++mb) {
+            const double fv = double(F[(long)mb * a.NF[d]]);
 #pragma unroll
-        for (int n = 0; n < Ns; ++n) s[n] = fma(M[n * Mbd + mb], fv, s[n]);
+            for (int n = 1; n < Ns; ++n) s[n] = fma(M[n * Mbd + mb], fv, s[n]);
+        }
+        s[0] = double(S[0]);
     }
 #pragma unroll
     for (int n = 0; n < Ns; ++n) a.Mom[((long)q * Ns + n) * a.NM + j] = s[n] * jac;
@@ -621,4 +892,22 @@ __global__ void __launch_bounds__(256) k_totals(TotArgs a) {
     }
 }
 
+}  // namespace adg
+
+namespace adg {
+// Layout probe of the 16x16x4 MFMA builtins (one wavefront, in-bounds):
+// mode 0: A[lane] = lane+1, B = 1 on lanes 0..15 (kidx 0) else 0
+// mode 1: A = 1 on lanes 0..15 else 0, B[lane] = lane+1
+// out[lane*4 + r] = accumulator register r of lane.
+template <typename T>
+__global__ void k_probe_mfma(T* out, int mode) {
+    using M = MF<T>;
+    const int l = threadIdx.x & 63;
+    T a, b;
+    if (mode == 0) { a = T(l + 1); b = (l < 16) ? T(1) : T(0); }
+    else { a = (l < 16) ? T(1) : T(0); b = T(l + 1); }
+    typename M::v4 c = {0, 0, 0, 0};
+    c = M::mma(a, b, c);
+    for (int r = 0; r < 4; ++r) out[l * 4 + r] = c[r];
+}
 }  // namespace adg

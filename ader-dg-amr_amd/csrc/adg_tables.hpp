@@ -43,6 +43,18 @@ struct ElementTables {
 
 ElementTables build_element_tables(int P, int D);
 
+// A operands of the 16x16x4 MFMA contractions of the predictor, in fragment
+// order: fragment f holds 64 values, lane l = row (l%16) + 16 * kidx (l/16).
+// Rows/columns beyond the matrix are zero.  Offsets count fragments.
+struct Fragments {
+    int A0 = 0, V = 0, B = 0, S = 0, T = 0, total = 0;
+    int MB2 = 0, KSU = 0, NB = 0, KSH = 0, MBS = 0, MBD = 0;
+    std::vector<double> data;  // [total][64]
+};
+// fp64_rows selects the accumulator row order of the FP64 MFMA (row = 4r+g)
+// instead of the FP32 one (row = 4g+r) for the K order of B and S.
+Fragments build_fragments(const ElementTables& t, bool fp64_rows);
+
 // Named access for tests (returns empty vector for unknown names).
 const std::vector<double>* table_by_name(const ElementTables& t, const std::string& name);
 

@@ -1,4 +1,5 @@
-"""Box layouts used by the tests (level index space, ratio 2)."""
+"""Patch layouts (level index space, refinement ratio 2) for the host AMR
+substrate: uniform tilings and the Kelvin-Helmholtz band hierarchy."""
 
 
 def tiles(n, mg, D=2, lo0=None, hi0=None):

@@ -23,9 +23,9 @@ EXPORTS = [
     "adg_create", "adg_set_geometry", "adg_set_levels", "adg_level_nslot", "adg_level_nface",
     "adg_upload_Uw", "adg_download_Uw", "adg_fill_ghosts", "adg_average_down", "adg_set_dt",
     "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
-    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_step", "adg_set_time",
+    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_set_time",
     "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
-    "adg_project_ic", "adg_last_error", "adg_destroy",
+    "adg_project_ic", "adg_set_predictor_kernel", "adg_last_error", "adg_destroy",
 ]
 
 
@@ -59,6 +59,7 @@ def lib():
     L.adg_corrector.argtypes = [C.c_void_p, C.c_int]
     L.adg_reflux.argtypes = [C.c_void_p, C.c_int]
     L.adg_ader.argtypes = [C.c_void_p]
+    L.adg_advance.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     L.adg_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     L.adg_set_time.argtypes = [C.c_void_p, C.c_double]
     L.adg_conserved_totals.argtypes = [C.c_void_p, _dp]
@@ -68,6 +69,7 @@ def lib():
     L.adg_sizes.argtypes = [C.c_void_p, _ip]
     L.adg_get_table.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_int]
     L.adg_project_ic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.adg_set_predictor_kernel.argtypes = [C.c_void_p, C.c_int]
     L.adg_last_error.argtypes = [C.c_void_p]
     L.adg_last_error.restype = C.c_char_p
     L.adg_destroy.argtypes = [C.c_void_p]
@@ -140,6 +142,10 @@ class Context:
         self._chk(self.L.adg_download_Uw(self.h, lev, _d(out)), "adg_download_Uw")
         return out
 
+    def set_predictor_kernel(self, kind):
+        """0 automatic, 1 VALU (FP64), 2 MFMA."""
+        self._chk(self.L.adg_set_predictor_kernel(self.h, kind), "adg_set_predictor_kernel")
+
     def keep_Hw(self, on=True):
         self._chk(self.L.adg_keep_Hw(self.h, 1 if on else 0), "adg_keep_Hw")
 
@@ -186,6 +192,9 @@ class Context:
 
     def ader(self):
         self._chk(self.L.adg_ader(self.h), "adg_ader")
+
+    def advance(self, c_dt, T):
+        self._chk(self.L.adg_advance(self.h, c_dt, T, None), "adg_advance")
 
     def step(self, c_dt, T, want_t=False):
         if want_t:

@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Throughput of the ADER-DG hot path on one MI355X.
+
+Workload (the metric's configuration): 2D compressible Euler (gamma=1.4)
+Kelvin-Helmholtz shear layer on periodic [0,1]^2, DG p=3 (Np_s=10,
+Np_st=20), three levels 64^2 -> 128^2 -> 256^2 (ratio 2; level 1 covers the
+bands |y-1/4|,|y-3/4| < 1/8, level 2 the bands < 1/16), v = 0.01 sin(4 pi x)
+plus seeded per-cell uniform noise of amplitude 1e-3.  Inputs are synthetic
+and resident in HBM before the timed region.
+
+A step is one iteration of the evolve loop on the device: predictor, face
+flux, corrector on every level, flux-register reflux, average down, ghost
+fill (same level, periodic, coarse-fine projection), CFL reduction and time
+advance.  value = sum over levels of valid elements * Q * Np_st per second.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "ader-dg-amr_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "space-time DOF-updates/s at p=3, learned+FP32 vs plain FP64, matched L1 error"
+PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix)
+PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix
+PEAK_HBM = 8.0e12
+KH_PARAMS = [0.01, 1e-3, 20240611]
+
+
+def algorithmic_flops(P, D, Q, Ns, Nst, Mst, Mbd):
+    """Per valid element per step (2 flops per FMA): P Picard iterations plus
+    the final evaluation, traces, corrector volume and boundary terms."""
+    f_iter = 2 * Q * (Mst * Nst + Nst * Ns + D * Nst * Mst + Nst * Nst)
+    f_corr = 2 * Q * (Mst * Nst + 2 * D * Mbd * Nst + D * Ns * Mst + 2 * D * Ns * Mbd)
+    f_pred_kernel = P * f_iter + f_corr - 2 * Q * 2 * D * Ns * Mbd  # the predictor kernel's share
+    return P * f_iter + f_corr, f_pred_kernel
+
+
+def cpu_baseline(levels, P, nsteps, threads):
+    """The oracle (reference-structured FP64 CPU restatement) on the same
+    workload, OpenMP over cells, a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    T = O.Tables(P, 2)
+    H = O.Hier(2, levels, [64, 64], [0, 0], [1, 1])
+    Uw = O.project_ic(T, H, 1, KH_PARAMS)
+    O.average_down(T, H, Uw)
+    O.fill_patch(T, H, Uw)
+    Dt = O.set_dt(T, H, Uw, 0.5)
+    t0 = time.perf_counter()
+    n, _, _ = O.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=nsteps)
+    el = time.perf_counter() - t0
+    return n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--kernel", type=int, default=0, help="predictor kernel: 0 auto, 1 VALU, 2 MFMA")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the hot path is single-GPU by design "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import pyadg
+    from hiptime import Event
+    from layouts import kh_bands
+
+    P, D = 3, 2
+    levels = kh_bands(64, 3, 32)
+    prec = pyadg.ADG_FP64 if args.form == "fp64" else pyadg.ADG_FP32_MFMA
+    ctx = pyadg.Context(D, P, 1.4, prec)
+    if args.kernel:
+        ctx.set_predictor_kernel(args.kernel)
+    ctx.set_geometry([64, 64], [0, 0], [1, 1])
+    ctx.set_levels(levels)
+    for l in range(len(levels)):
+        ctx.project_ic(l, 1, KH_PARAMS)
+    ctx.average_down()
+    ctx.fill_ghosts()
+    c_dt, Tend = 0.5, 1.0e9   # no clamp inside the timed window
+    ctx.set_dt(c_dt, Tend, 0.0)
+    n_valid = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) for lev in levels for lo, hi in lev)
+    dof_per_step = n_valid * ctx.Q * ctx.Np_st
+    stream = ctx.stream()
+
+    def one_step(ev_a=None, ev_b=None):
+        if ev_a is not None:
+            ev_a.record(stream)
+        ctx.predictor()
+        if ev_b is not None:
+            ev_b.record(stream)
+        ctx.face_flux()
+        ctx.corrector()
+        ctx.reflux()
+        ctx.average_down()
+        ctx.fill_ghosts()
+        ctx.advance(c_dt, Tend)
+
+    for _ in range(args.warmup):
+        one_step()
+    ctx.sync()
+    pa = [Event() for _ in range(args.steps)]
+    pb = [Event() for _ in range(args.steps)]
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for k in range(args.steps):
+        one_step(pa[k], pb[k])
+    e1.record(stream)
+    e1.synchronize()
+    ctx.sync()
+    total_ms = e0.elapsed_ms(e1)
+    pred_ms = sum(pa[k].elapsed_ms(pb[k]) for k in range(args.steps)) / args.steps
+    ms_per_step = total_ms / args.steps
+    value = dof_per_step / (ms_per_step * 1e-3)
+
+    f_step, f_pred = algorithmic_flops(P, D, ctx.Q, ctx.Np_s, ctx.Np_st, ctx.qMp_st, ctx.qMp_st_bd)
+    achieved = f_pred * n_valid / (pred_ms * 1e-3)
+    peak = PEAK_FP64 if args.form == "fp64" else PEAK_FP32
+    roofline = {"bound": "mfma", "kernel": f"predictor p=3 2D ({args.form}, kernel={args.kernel})",
+                "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "avg_launch_ms": pred_ms, "flops_per_launch": f_pred * n_valid}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        n, el = cpu_baseline(levels, P, args.cpu_steps, threads)
+        cpu = {"value": dof_per_step * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{n} full steps of the same 3-level workload ({n_valid} elements) by the FP64 "
+                         f"CPU oracle (OpenMP, {threads} threads)"}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.form == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"kelvin-helmholtz 2D p=3, 3 levels 64^2/128^2/256^2 (bands), {args.form} form",
+                   "elements": n_valid, "dof_per_step": dof_per_step, "form": args.form,
+                   "algorithmic_flops_per_step": f_step * n_valid},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

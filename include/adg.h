@@ -105,6 +105,11 @@ int adg_set_dt_value(adg_ctx* ctx, double Dt);
  * mode 0: exactly P Picard iterations (the reference); other modes are
  * reserved.  n_fallback (nullable) receives 0 in mode 0. */
 int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
+/* Predictor kernel variant: 0 = automatic (MFMA where the element matrices
+ * fit in LDS, else VALU), 1 = VALU (FP64 only), 2 = MFMA.  No counterpart in
+ * the reference: a tuning knob of this implementation. */
+int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
+
 /* Debug readback of the last predictor H_w of a level, [q][Np_st][slot]
  * (only after adg_keep_Hw(ctx,1)). */
 int adg_keep_Hw(adg_ctx* ctx, int on);
@@ -131,6 +136,11 @@ int adg_reflux(adg_ctx* ctx, int crse_lev);
 /* Replaces AmrDG::ADER (AmrDG.h:1071-1165): predictor, face flux,
  * corrector on all levels, then flux correction. Uses the device Dt. */
 int adg_ader(adg_ctx* ctx);
+
+/* The tail of one AmrDG::evolve iteration (AmrDG.h:1017-1029): t += Dt,
+ * then set_Dt on the updated U_w and the clamp Dt = min(Dt, T - t), all on
+ * the device.  t_out (nullable) gets a blocking copy of the new t. */
+int adg_advance(adg_ctx* ctx, double c_dt, double T, double* t_out);
 
 /* One iteration of the AmrDG::evolve loop without regrid, limiter and
  * plots (AmrDG.h:988-1029): ADER, average_down, FillPatch, t+=Dt, set_Dt,

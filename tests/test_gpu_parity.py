@@ -4,7 +4,7 @@ conserved variable (the north-star tolerance for U_w modes)."""
 import numpy as np
 import pytest
 
-from hierarchies import kh_bands, single_level, two_level_centre
+from layouts import kh_bands, single_level, two_level_centre
 
 pytestmark = pytest.mark.gpu
 
@@ -21,13 +21,15 @@ def rel_err(a, b):
     return out
 
 
-def setup(adg, orc, D, P, levels, dom, plo, phi, ic_case, params, gamma=1.4):
+def setup(adg, orc, D, P, levels, dom, plo, phi, ic_case, params, gamma=1.4, kind=0, precision=0):
     T = orc.Tables(P, D)
     H = orc.Hier(D, levels, dom, plo, phi)
     Uw = orc.project_ic(T, H, ic_case, params, gamma)
     orc.average_down(T, H, Uw)
     orc.fill_patch(T, H, Uw)
-    ctx = adg.Context(D, P, gamma)
+    ctx = adg.Context(D, P, gamma, precision)
+    if kind:
+        ctx.set_predictor_kernel(kind)
     ctx.set_geometry(dom, plo, phi)
     ctx.set_levels(levels)
     for l in range(len(levels)):
@@ -36,11 +38,15 @@ def setup(adg, orc, D, P, levels, dom, plo, phi, ic_case, params, gamma=1.4):
     return T, H, Uw, ctx
 
 
+KINDS = [1, 2]  # VALU, MFMA predictor
+
+
+@pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("P", [1, 2, 3])
-def test_predictor_single_patch(adg, orc, P):
-    print(f"case predictor vortex P={P} 16^2", flush=True)
+def test_predictor_single_patch(adg, orc, P, kind):
+    print(f"case predictor vortex P={P} 16^2 kind={kind}", flush=True)
     levels = single_level(16)
-    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [10, 10], 0, VORTEX)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [10, 10], 0, VORTEX, kind=kind)
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.keep_Hw(True)
     ctx.set_dt_value(Dt)
@@ -59,11 +65,12 @@ def test_set_dt_matches(adg, orc):
     assert abs(dt - dt_ref) <= 1e-14 * dt_ref
 
 
+@pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("P,mg", [(2, 32), (2, 16), (3, 8)])
-def test_ader_step_single_level(adg, orc, P, mg):
-    print(f"case ader vortex P={P} 32^2 patches {mg}", flush=True)
+def test_ader_step_single_level(adg, orc, P, mg, kind):
+    print(f"case ader vortex P={P} 32^2 patches {mg} kind={kind}", flush=True)
     levels = single_level(32, mg)
-    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [32, 32], [0, 0], [10, 10], 0, VORTEX)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [32, 32], [0, 0], [10, 10], 0, VORTEX, kind=kind)
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.set_dt_value(Dt)
     ctx.ader()
@@ -120,10 +127,11 @@ def test_two_level_step(adg, orc, P):
         assert rel_err(U, Uw[l]) < TOL, f"level {l}"
 
 
-def test_kh_three_levels(adg, orc):
-    print("case KH P=3 16^2 3 levels", flush=True)
+@pytest.mark.parametrize("kind", KINDS)
+def test_kh_three_levels(adg, orc, kind):
+    print(f"case KH P=3 16^2 3 levels kind={kind}", flush=True)
     levels = kh_bands(16, 3, 16)
-    T, H, Uw, ctx = setup(adg, orc, 2, 3, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7])
+    T, H, Uw, ctx = setup(adg, orc, 2, 3, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7], kind=kind)
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.set_dt(0.5, 1.0, 0.0)
     m0 = ctx.totals()
@@ -150,3 +158,51 @@ def test_gpu_ic_matches_oracle(adg, orc):
     ctx.project_ic(0, 1, [0.01, 1e-3, 7])
     ref = orc.project_ic(T, H, 1, [0.01, 1e-3, 7])[0]
     assert rel_err(ctx.download_Uw(0), ref) < 1e-13
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_fp32_form_kh(adg, orc, P):
+    """Precision form: FP32 MFMA contractions and FP32 face fluxes, FP64 U_w
+    accumulation, reflux and reductions.  Stated error vs the FP64 oracle
+    after 10 steps on a 3-level KH hierarchy: per variable, max|dU| over the
+    largest |U| of all variables (the FP32 round-off of 10 increments) below
+    1e-6; composite mass and energy conserved to 1e-13."""
+    print(f"case FP32 form KH P={P} 16^2 3 levels", flush=True)
+    levels = kh_bands(16, 3, 16)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7], precision=1)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    m0 = ctx.totals()
+    for _ in range(10):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=10)
+    for l in range(3):
+        got = ctx.download_Uw(l)
+        scale = np.abs(Uw[l]).max()
+        errs = [np.abs(got[q] - Uw[l][q]).max() / scale for q in range(T.Q)]
+        print(f"  level {l} FP32-vs-FP64 error per variable: {['%.2e' % e for e in errs]}", flush=True)
+        assert max(errs) < 1e-6, f"level {l}"
+    m1 = ctx.totals()
+    assert abs(m1[0] - m0[0]) / abs(m0[0]) < 1e-13
+    assert abs(m1[3] - m0[3]) / abs(m0[3]) < 1e-13
+
+
+def test_mfma_layout(adg):
+    """Pins the 16x16x4 MFMA lane layouts the predictor relies on:
+    A[i][k] on lane i+16k, B[k][j] on lane j+16k, accumulator register r of
+    lane group g holds row 4g+r (FP32) and row 4r+g (FP64)."""
+    import ctypes as C
+    L = adg.lib()
+    L.adg_probe_mfma_layout.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
+    for prec in (0, 1):
+        o = np.zeros(256)
+        assert L.adg_probe_mfma_layout(prec, 0, o.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        o = o.reshape(64, 4)
+        for lane in range(64):
+            g = lane // 16
+            rows = [4 * r + g if prec == 0 else 4 * g + r for r in range(4)]
+            assert list(o[lane]) == [row + 1 for row in rows]
+        assert L.adg_probe_mfma_layout(prec, 1, o.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        o = o.reshape(64, 4)
+        for lane in range(64):
+            assert list(o[lane]) == [lane % 16 + 1] * 4
