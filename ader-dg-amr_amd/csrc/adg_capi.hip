@@ -44,7 +44,6 @@ struct adg_ctx {
     double *Uw = nullptr, *Hw_dbg = nullptr, *Mom = nullptr, *partial = nullptr;
     void *Vol = nullptr, *Tr = nullptr;     // element type of the form (double / float)
     void* Fnum[3] = {nullptr, nullptr, nullptr};
-    void* Smom[3] = {nullptr, nullptr, nullptr};
     size_t esz = sizeof(double);
     adg::Fragments frags;
     adg::FragOff fo{};
@@ -52,6 +51,10 @@ struct adg_ctx {
     float* d_frag32 = nullptr;
     int pred_kind = 0;       // 0 auto, 1 VALU, 2 MFMA
     int ncu = 256;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    double graph_cdt = -1.0, graph_T = -1.0;
+    int graph_kind = -1;
     int* valid_list = nullptr;
     int* cell_face[3][2] = {};
     int* face_lr[3] = {};
@@ -116,16 +119,26 @@ int dev_alloc(adg_ctx* c, double** dst, size_t n) {
 
 // LDS bytes of the MFMA predictor for a 512-thread workgroup.
 size_t mfma_lds_bytes(const adg_ctx* c) {
-    return ((size_t)c->frags.total * 64 + (size_t)8 * c->frags.MB2 * 256) * c->esz;
+    // fragments, plus the per-wave H_w re-layout buffer of the FP32 form
+    const size_t xch = c->esz == sizeof(double) ? 0 : (size_t)8 * c->frags.MB2 * 256;
+    return ((size_t)c->frags.total * 64 + xch) * c->esz;
 }
 bool use_mfma(const adg_ctx* c) {
     const bool fits = mfma_lds_bytes(c) <= 160 * 1024;
-    if (c->pred_kind == 1) return false;
+    if (c->pred_kind == 1) return c->precision != ADG_FP64 && fits;
     if (c->pred_kind == 2) return fits;
     return fits && !(c->P == 1);
 }
 
+void drop_graph(adg_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
 void free_levels(adg_ctx* c) {
+    drop_graph(c);
     for (void* p : c->level_allocs) (void)hipFree(p);
     c->level_allocs.clear();
     c->levels_set = false;
@@ -241,7 +254,7 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
     void* tsp = nullptr;
     if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->ts = static_cast<adg::TimeState*>(tsp);
-    adg::TimeState t0{0.0, 0.0, 0x7ff0000000000000ull, 0.0};
+    adg::TimeState t0{0.0, 0.0, 0x7ff0000000000000ull, 0u, 0u};
     if (hipMemcpy(c->ts, &t0, sizeof(t0), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
     *out = c;
     return ADG_OK;
@@ -260,6 +273,8 @@ int adg_set_geometry(adg_ctx* c, const int* dom_n, const double* prob_lo, const 
     c->geo_set = true;
     return ADG_OK;
 }
+
+static int prepare_kernels(adg_ctx* c);
 
 int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if (!c) return ADG_E_ARG;
@@ -327,14 +342,32 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if ((rc = dev_alloc_bytes(c, &c->Vol, ves * NS * c->esz)) != ADG_OK) return rc;
     if ((rc = dev_alloc_bytes(c, &c->Tr, tes * NS * c->esz)) != ADG_OK) return rc;
     for (int d = 0; d < D; ++d)
-        if ((rc = dev_alloc_bytes(c, &c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d] * c->esz)) != ADG_OK ||
-            (rc = dev_alloc_bytes(c, &c->Smom[d], (size_t)Q * t.Np_s * H.NF[d] * c->esz)) != ADG_OK)
+        if ((rc = dev_alloc_bytes(c, &c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d] * c->esz)) != ADG_OK)
             return rc;
     if ((rc = dev_alloc(c, &c->Mom, (size_t)Q * t.Np_s * H.mom_face.size())) != ADG_OK) return rc;
     if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->levels_set = true;
+    return prepare_kernels(c);
+}
+
+// Raises the dynamic-LDS limit of the MFMA predictor once, outside any
+// stream capture.
+static int prepare_kernels(adg_ctx* c) {
+    const int lds = (int)mfma_lds_bytes(c);
+    if (lds > 160 * 1024) return ADG_OK;
+#define L_ATTR(PP, DD)                                                                                    \
+    {                                                                                                     \
+        if (c->precision == ADG_FP64)                                                                     \
+            HIPCHK(c, hipFuncSetAttribute((const void*)adg::k_predictor_mfma<double, PP, DD>,             \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds));              \
+        else                                                                                              \
+            HIPCHK(c, hipFuncSetAttribute((const void*)adg::k_predictor_mfma<float, PP, DD>,              \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds));              \
+    }
+    ADG_DISPATCH(c, L_ATTR);
+#undef L_ATTR
     return ADG_OK;
 }
 
@@ -401,14 +434,17 @@ int adg_download_Fnum(adg_ctx* c, int lev, int d, double* host) {
     if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev || d < 0 || d >= c->D || !host)
         return fail(c, ADG_E_ARG, "bad arguments");
     const size_t nf = (size_t)c->H.lev[lev].nface[d], off = (size_t)c->H.lev[lev].face_off[d];
-    const size_t NF = (size_t)c->H.NF[d], rows = (size_t)c->Q * c->tab.qMp_st_bd, es = c->esz;
-    std::vector<unsigned char> tmp(rows * nf * es);
-    HIPCHK(c, hipMemcpy2DAsync(tmp.data(), nf * es, (unsigned char*)c->Fnum[d] + off * es, NF * es, nf * es, rows,
-                               hipMemcpyDeviceToHost, c->stream));
+    const size_t row = (size_t)c->Q * c->tab.qMp_st_bd, es = c->esz;  // values per face
+    std::vector<unsigned char> tmp(row * nf * es);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), (unsigned char*)c->Fnum[d] + off * row * es, tmp.size(),
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < rows * nf; ++i)
-        host[i] = es == sizeof(double) ? reinterpret_cast<double*>(tmp.data())[i]
-                                       : (double)reinterpret_cast<float*>(tmp.data())[i];
+    for (size_t f = 0; f < nf; ++f)
+        for (size_t i = 0; i < row; ++i) {
+            const size_t src = f * row + i;
+            host[i * nf + f] = es == sizeof(double) ? reinterpret_cast<double*>(tmp.data())[src]
+                                                    : (double)reinterpret_cast<float*>(tmp.data())[src];
+        }
     return ADG_OK;
 }
 
@@ -417,7 +453,7 @@ int adg_fill_ghosts(adg_ctx* c) {
     const long ng = (long)c->H.g_slot.size();
     if (ng == 0) return ADG_OK;
     adg::GhostArgs a{c->d_tab, c->to, c->Uw, c->g_slot, c->g_src, c->g_kind, c->g_child, c->H.NS, ng};
-#define L_GH(PP, DD) hipLaunchKernelGGL((adg::k_ghosts<PP, DD>), dim3(nblocks(ng * c->Q)), dim3(256), 0, c->stream, a)
+#define L_GH(PP, DD) hipLaunchKernelGGL((adg::k_ghosts<PP, DD>), dim3(nblocks(ng * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_GH);
 #undef L_GH
     HIPCHK(c, hipGetLastError());
@@ -430,7 +466,7 @@ int adg_average_down(adg_ctx* c) {
         const long ng = (long)c->H.ad_coarse[l].size();
         if (ng == 0) continue;
         adg::AvgArgs a{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
-#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(ng * c->Q)), dim3(256), 0, c->stream, a)
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(ng * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, a)
         ADG_DISPATCH(c, L_AD);
 #undef L_AD
         HIPCHK(c, hipGetLastError());
@@ -497,7 +533,6 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n) {
     a.gamma = c->gamma;
     const size_t lds = mfma_lds_bytes(c);
     auto kern = adg::k_predictor_mfma<T, P, D>;
-    HIPCHK(c, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long per_batch = 8L * adg::Dims<P, D>::EPM;
     const long nbatch = (n + per_batch - 1) / per_batch;
     const long per_cu = std::max<long>(1, (long)((160 * 1024) / lds));
@@ -556,19 +591,18 @@ extern "C++" {
 template <typename T>
 static int launch_faceflux(adg_ctx* c) {
     adg::FaceArgs<T> a{};
-    a.tab = c->d_tab;
-    a.to = c->to;
     a.Tr = (const T*)c->Tr;
     long tot = 0;
     for (int d = 0; d < c->D; ++d) {
         a.Fnum[d] = (T*)c->Fnum[d];
-        a.Smom[d] = (T*)c->Smom[d];
         a.face_lr[d] = c->face_lr[d];
         a.NF[d] = c->H.NF[d];
         tot += c->H.NF[d];
     }
     a.gamma = c->gamma;
-#define L_FF(PP, DD) hipLaunchKernelGGL((adg::k_faceflux<T, PP, DD>), dim3(nblocks(tot, 64)), dim3(64), 0, c->stream, a)
+    const long tpf = 64 * ((c->tab.qMp_st_bd * c->Q + 63) / 64);
+    tot *= tpf;
+#define L_FF(PP, DD) hipLaunchKernelGGL((adg::k_faceflux<T, PP, DD>), dim3(nblocks(tot)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_FF);
 #undef L_FF
     HIPCHK(c, hipGetLastError());
@@ -591,7 +625,7 @@ static int launch_corrector(adg_ctx* c, long begin, long n) {
     a.Uw = c->Uw;
     a.Vol = (const T*)c->Vol;
     for (int d = 0; d < c->D; ++d) {
-        a.Smom[d] = (const T*)c->Smom[d];
+        a.Fnum[d] = (const T*)c->Fnum[d];
         a.cell_face[d][0] = c->cell_face[d][0] + begin;
         a.cell_face[d][1] = c->cell_face[d][1] + begin;
         a.NF[d] = c->H.NF[d];
@@ -630,7 +664,6 @@ static int launch_moments(adg_ctx* c, long NM) {
     m.to = c->to;
     for (int d = 0; d < c->D; ++d) {
         m.Fnum[d] = (const T*)c->Fnum[d];
-        m.Smom[d] = (const T*)c->Smom[d];
         m.NF[d] = c->H.NF[d];
     }
     m.Mom = c->Mom;
@@ -642,7 +675,7 @@ static int launch_moments(adg_ctx* c, long NM) {
     m.ts = c->ts;
     m.geo = c->geo;
     m.NM = NM;
-#define L_MO(PP, DD) hipLaunchKernelGGL((adg::k_moments<T, PP, DD>), dim3(nblocks(NM * c->Q)), dim3(256), 0, c->stream, m)
+#define L_MO(PP, DD) hipLaunchKernelGGL((adg::k_moments<T, PP, DD>), dim3(nblocks(NM * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, m)
     ADG_DISPATCH(c, L_MO);
 #undef L_MO
     HIPCHK(c, hipGetLastError());
@@ -659,10 +692,113 @@ int adg_reflux(adg_ctx* c, int crse_lev) {
     if (rc != ADG_OK) return rc;
     adg::RefluxArgs r{c->d_tab, c->to, c->Uw, c->Mom, c->rc_slot, c->rc_start, c->rc_count, c->rc_lev,
                       c->re_mom, c->re_coef, c->geo, c->H.NS, NM, nrc};
-#define L_RF(PP, DD) hipLaunchKernelGGL((adg::k_reflux<PP, DD>), dim3(nblocks(nrc * c->Q)), dim3(256), 0, c->stream, r)
+#define L_RF(PP, DD) hipLaunchKernelGGL((adg::k_reflux<PP, DD>), dim3(nblocks(nrc * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, r)
     ADG_DISPATCH(c, L_RF);
 #undef L_RF
     HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+extern "C++" {
+template <typename T>
+static adg::EpiArgs<T> epi_args(adg_ctx* c, double c_dt, double T_end) {
+    adg::EpiArgs<T> a{};
+    a.tab = c->d_tab;
+    a.to = c->to;
+    a.Uw = c->Uw;
+    for (int d = 0; d < c->D; ++d) {
+        a.Fnum[d] = (const T*)c->Fnum[d];
+        a.NF[d] = c->H.NF[d];
+    }
+    a.mom_face = c->mom_face; a.mom_d = c->mom_d; a.mom_kind = c->mom_kind; a.mom_sel = c->mom_sel;
+    a.mom_lev = c->mom_lev; a.rc_slot = c->rc_slot; a.rc_start = c->rc_start; a.rc_count = c->rc_count;
+    a.rc_lev = c->rc_lev; a.re_mom = c->re_mom; a.re_coef = c->re_coef;
+    a.nrc = (long)c->H.rc_slot.size();
+    a.g_slot = c->g_slot; a.g_src = c->g_src; a.g_kind = c->g_kind; a.g_child = c->g_child;
+    a.nghost = (long)c->H.g_slot.size();
+    a.valid_list = c->valid_list;
+    a.slot_lev = c->slot_lev;
+    a.nvalid = (long)c->H.valid_list.size();
+    a.ncfl_blocks = nblocks(a.nvalid);
+    a.cfl = (1.0 / (2.0 * (double)c->P + 1.0)) * (1.0 / (double)c->D);
+    a.c_dt = c_dt;
+    a.Tend = T_end;
+    a.ts = c->ts;
+    a.geo = c->geo;
+    a.NS = c->H.NS;
+    a.gamma = c->gamma;
+    return a;
+}
+
+// reflux || finest average down, coarser average downs, ghosts || CFL + advance
+template <typename T>
+static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
+    const int L = c->H.nlev;
+    adg::EpiArgs<T> a = epi_args<T>(c, c_dt, T_end);
+    if (L > 1) {
+        a.ad_coarse = c->ad_coarse[L - 1];
+        a.ad_fine = c->ad_fine[L - 1];
+        a.ngroup = (long)c->H.ad_coarse[L - 1].size();
+        a.nblk_a = nblocks(a.nrc * c->Q * c->tab.Np_s);
+        const unsigned nb = a.nblk_a + nblocks(a.ngroup * c->Q * c->tab.Np_s);
+        if (nb > 0) {
+#define L_RA(PP, DD) hipLaunchKernelGGL((adg::k_reflux_avg<T, PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)
+            ADG_DISPATCH(c, L_RA);
+#undef L_RA
+            HIPCHK(c, hipGetLastError());
+        }
+        for (int l = L - 2; l > 0; --l) {
+            const long ng = (long)c->H.ad_coarse[l].size();
+            if (ng == 0) continue;
+            adg::AvgArgs g{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(ng * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, g)
+            ADG_DISPATCH(c, L_AD);
+#undef L_AD
+            HIPCHK(c, hipGetLastError());
+        }
+    }
+    a.nblk_a = nblocks(a.nghost * c->Q * c->tab.Np_s);
+    const unsigned nb = a.nblk_a + a.ncfl_blocks;
+#define L_GC(PP, DD) hipLaunchKernelGGL((adg::k_ghost_cfl<T, PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_GC);
+#undef L_GC
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+}  // extern "C++"
+
+static int step_launches(adg_ctx* c, double c_dt, double T_end) {
+    int rc;
+    if ((rc = adg_predictor(c, -1, 0, nullptr)) != ADG_OK) return rc;
+    if ((rc = adg_face_flux(c, -1)) != ADG_OK) return rc;
+    if ((rc = adg_corrector(c, -1)) != ADG_OK) return rc;
+    return c->precision == ADG_FP64 ? launch_epilogue<double>(c, c_dt, T_end) : launch_epilogue<float>(c, c_dt, T_end);
+}
+
+int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_out) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (nsteps < 0) return fail(c, ADG_E_ARG, "nsteps must be >= 0");
+    const int kind = use_mfma(c) ? 2 : 1;
+    if (!c->graph_exec || c->graph_cdt != c_dt || c->graph_T != T_end || c->graph_kind != kind) {
+        drop_graph(c);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = step_launches(c, c_dt, T_end);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != ADG_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(c, ADG_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_cdt = c_dt;
+        c->graph_T = T_end;
+        c->graph_kind = kind;
+    }
+    for (int i = 0; i < nsteps; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if (t_out) {
+        HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return ADG_OK;
 }
 
@@ -687,11 +823,9 @@ int adg_advance(adg_ctx* c, double c_dt, double T, double* t_out) {
 }
 
 int adg_step(adg_ctx* c, double c_dt, double T, double* t_out) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     int rc;
-    if ((rc = adg_ader(c)) != ADG_OK) return rc;
-    if ((rc = adg_average_down(c)) != ADG_OK) return rc;
-    if ((rc = adg_fill_ghosts(c)) != ADG_OK) return rc;
-    if ((rc = launch_cfl(c, c_dt, T, 1)) != ADG_OK) return rc;
+    if ((rc = step_launches(c, c_dt, T)) != ADG_OK) return rc;
     if (t_out) {
         HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -739,6 +873,32 @@ int adg_get_table(adg_ctx* c, const char* name, double* out, int cap) {
     if (out)
         for (size_t i = 0; i < v->size() && (int)i < cap; ++i) out[i] = (*v)[i];
     return (int)v->size();
+}
+
+// Host-only inspection of the hierarchy lists built for a box layout (no
+// device work; used by CPU tests).  kind 0: b_coarse, 1: b_fine, 2: child
+// index of the faces of direction d of level lev, [face] in level order.
+// kind 3: counts {NS, ghosts, faces d0, faces d1, faces d2, moments,
+// reflux rows, uncovered}.  Returns the number of ints written or < 0.
+int adg_host_inspect(int dim, const int* dom_n, const int* periodic, int nlev, const int* npatch, const int* boxes,
+                     int lev, int d, int kind, int* out, int cap) {
+    try {
+        double plo[3] = {0, 0, 0}, phi[3] = {1, 1, 1};
+        adg::Hierarchy H = adg::build_hierarchy(dim, dom_n, plo, phi, periodic, nlev, npatch, boxes);
+        if (kind == 3) {
+            const long v[8] = {H.NS, (long)H.g_slot.size(), H.NF[0], H.NF[1], H.NF[2], (long)H.mom_face.size(),
+                               (long)H.rc_slot.size(), (long)H.uncovered.size()};
+            for (int i = 0; i < 8 && i < cap; ++i) out[i] = (int)v[i];
+            return 8;
+        }
+        if (lev < 0 || lev >= nlev || d < 0 || d >= dim || kind < 0 || kind > 2) return ADG_E_ARG;
+        const std::vector<int8_t>& m = kind == 0 ? H.bcoarse[d] : kind == 1 ? H.bfine[d] : H.cidx[d];
+        const long off = H.lev[lev].face_off[d], n = H.lev[lev].nface[d];
+        for (long i = 0; i < n && i < cap; ++i) out[i] = m[off + i];
+        return (int)n;
+    } catch (const std::exception&) {
+        return ADG_E_ARG;
+    }
 }
 
 // Lane-layout probe of the MFMA builtins used by the predictor (diagnostic;

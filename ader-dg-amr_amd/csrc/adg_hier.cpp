@@ -374,9 +374,13 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         }
         // corrected coarse cells: reference L2ProjInterp::reflux applies the
         // correction only where a face of the cell is flagged in its own
-        // patch's (ownership-filtered) cf_face_b_coarse.
+        // patch's (ownership-filtered) cf_face_b_coarse.  Cells covered by
+        // the fine level are left out: the average down that follows the
+        // reflux (AmrDG.h:1000) overwrites them, so their correction has no
+        // effect, and leaving them out lets the reflux run beside it.
         for (int cp = h.lev[lc].p_begin; cp < h.lev[lc].p_end; ++cp)
             for_box(D, h.patch[cp].lo, h.patch[cp].hi, [&](const int* iv) {
+                if (covered(lc, iv)) return;
                 bool at_if = false;
                 std::map<int, double> acc;
                 for (int d = 0; d < D; ++d) {

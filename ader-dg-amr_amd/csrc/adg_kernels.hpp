@@ -5,8 +5,7 @@
 //   Vol  per slot, [MBS][4][4][Q]   corrector volume term sum_d (Dt/dx_d) Sk_corr[d] F_d
 //   Tr   per slot, [2D][MBD][4][4][Q] predictor traces on the faces
 //                                    (side 0: xi_d=-1, side 1: xi_d=+1)
-//   Fnum [d]: [Q][qMp_st_bd][NF_d]   LLF flux at the face quadrature nodes
-//   Smom [d]: [Q][Np_s][NF_d]        face moments sum_m Mkbdp[d](k,m) Fnum[m]
+//   Fnum [d]: [NF_d][Q][qMp_st_bd]   LLF flux at the face quadrature nodes (face-major)
 //   Mom  [Q][Np_s][NM]  FP64         flux-register face moments
 // Vol and Tr are stored in the order the 16x16 MFMA accumulators hold them
 // (row = 16*block + 4*lane_group + register), so a wavefront writes each
@@ -99,7 +98,8 @@ struct TimeState {
     double t;
     double Dt;
     unsigned long long dtmin_bits;
-    double pad;
+    unsigned int done;   // CFL blocks finished in the current reduction
+    unsigned int pad;
 };
 
 static constexpr double kNumLimit = 10.0 * 2.220446049250313e-16;  // 10 * DBL_EPSILON
@@ -417,15 +417,22 @@ __global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
                             acc[mb] = M::mma(fr[(a.fo.B + ((mb * D + d) * NB + nb) * 4 + r) * 64 + lane], gd[d][r],
                                              acc[mb]);
             }
-            // accumulator layout (mode 16mb+4g+r) -> B operand layout (mode 4ks+g)
+            // accumulator layout -> B operand layout (mode 4ks+g).  FP64: register
+            // r of block mb holds mode 16mb+4r+g = 4(4mb+r)+g already; FP32
+            // (mode 16mb+4g+r) goes through the per-wave LDS buffer.
+            if constexpr (sizeof(T) == 8) {
 #pragma unroll
-            for (int mb = 0; mb < MB2; ++mb)
+                for (int ks = 0; ks < KSH; ++ks) hb[ks] = acc[ks >> 2][ks & 3];
+            } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) xch[(16 * mb + acc_row<T>(g, r)) * 16 + c] = acc[mb][r];
-            wave_lds_sync();
+                for (int mb = 0; mb < MB2; ++mb)
 #pragma unroll
-            for (int ks = 0; ks < KSH; ++ks) hb[ks] = xch[(4 * ks + g) * 16 + c];
-            wave_lds_sync();
+                    for (int r = 0; r < 4; ++r) xch[(16 * mb + acc_row<T>(g, r)) * 16 + c] = acc[mb][r];
+                wave_lds_sync();
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) hb[ks] = xch[(4 * ks + g) * 16 + c];
+                wave_lds_sync();
+            }
         }
 
         // final evaluation: corrector volume term
@@ -491,20 +498,30 @@ __global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Face moment row k of direction d: S_k = sum_m Mkbdp[d](k,m) Fnum[m], one
+// FMA chain over the face nodes in a fixed order.  The corrector of both
+// neighbours of a face and the flux registers all call this, so they see
+// bitwise equal moments: the left cell (its + face) subtracts S, the right
+// cell (its - face) adds (-1)^{k_d} S since Mkbdm[d](k,m) = (-1)^{k_d}
+// Mkbdp[d](k,m) exactly, and the mode-0 rows telescope.
+template <typename T, int Mbd>
+__device__ __forceinline__ T face_moment(const double* __restrict__ Mrow, const T* F) {
+    T s = T(0);
+#pragma unroll
+    for (int mb = 0; mb < Mbd; ++mb) s = fma(T(Mrow[mb]), F[mb], s);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
 // Face flux: LLF at every face quadrature node (AmrDG_NumericalFlux.tpp:4-41)
 // from the traces of the left (side +) and right (side -) cells, computed
-// once per face.  The face moments S[q][k] = sum_m Mkbdp[d](k,m) Fnum[q][m]
-// are summed here, once: the left cell (its + face) subtracts S, the right
-// cell (its - face) adds (-1)^{k_d} S since Mkbdm[d](k,m) = (-1)^{k_d}
-// Mkbdp[d](k,m) exactly, and the flux registers reuse the same S.  Both
-// neighbours and the reflux therefore see bitwise equal mode-0 moments.
+// once per face.  A face owns TPF = 64*ceil(Mbd*Q/64) lanes, lane = mb*Q + q:
+// the trace reads of a face are contiguous, the Q variables of a node are
+// gathered from neighbouring lanes, and Fnum is face-major [face][q][mb].
 template <typename T>
 struct FaceArgs {
-    const double* __restrict__ tab;
-    TabOff to;
     const T* __restrict__ Tr;
     T* __restrict__ Fnum[3];
-    T* __restrict__ Smom[3];
     const int* __restrict__ face_lr[3];
     long NF[3];
     double gamma;
@@ -513,60 +530,46 @@ struct FaceArgs {
 template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_faceflux(FaceArgs<T> a) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Mbd = Dm::Mbd, Ns = Dm::Ns;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int Q = Dm::Q, Mbd = Dm::Mbd;
+    constexpr int TPF = 64 * cdiv(Mbd * Q, 64);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long fg = t / TPF;
+    const int w = (int)(t - fg * TPF);
     int d = 0;
     long base = 0, tot = 0;
 #pragma unroll
     for (int dd = 0; dd < D; ++dd) {
-        if (t >= tot + a.NF[dd]) base = tot + a.NF[dd], d = dd + 1;
+        if (fg >= tot + a.NF[dd]) base = tot + a.NF[dd], d = dd + 1;
         tot += a.NF[dd];
     }
-    if (t >= tot) return;
-    const long f = t - base;
-    const long NFd = a.NF[d];
+    if (fg >= tot) return;   // whole face groups exit together
+    const long f = fg - base;
+    const bool ok = w < Mbd * Q;
+    const int mb = ok ? w / Q : 0, q = ok ? w - (w / Q) * Q : 0;
     const long sl = a.face_lr[d][2 * f], sr = a.face_lr[d][2 * f + 1];
-    const T* __restrict__ Mp = nullptr;
-    const double* __restrict__ Mbp = a.tab + a.to.Mkbdp + (long)d * Ns * Mbd;
+    const T vL = a.Tr[sl * Dm::TES + tr_off<T, P, D>(2 * d + 1, mb, q)];
+    const T vR = a.Tr[sr * Dm::TES + tr_off<T, P, D>(2 * d + 0, mb, q)];
+    T uL[Q], uR[Q];
+    const int lane = threadIdx.x & 63;
+    gather_state<Q>(vL, lane - q, uL);
+    gather_state<Q>(vR, lane - q, uR);
+    if (!ok) return;
     const T gm = T(a.gamma), gm1 = T(a.gamma - 1.0);
-    T S[Q][Ns];
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-#pragma unroll
-        for (int k = 0; k < Ns; ++k) S[q][k] = T(0);
-    (void)Mp;
-    for (int mb = 0; mb < Mbd; ++mb) {
-        T uL[Q], uR[Q];
-        const T* pl = a.Tr + sl * Dm::TES + tr_off<T, P, D>(2 * d + 1, mb, 0);
-        const T* pr = a.Tr + sr * Dm::TES + tr_off<T, P, D>(2 * d + 0, mb, 0);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) { uL[q] = pl[q]; uR[q] = pr[q]; }
-        const T riL = T(1) / uL[0], riR = T(1) / uR[0];
-        const T pL = pressure<D>(uL, riL, gm1), pR = pressure<D>(uR, riR, gm1);
-        const T cL = sqrt(gm * (pL / fmax(uL[0], T(kNumLimit))));
-        const T cR = sqrt(gm * (pR / fmax(uR[0], T(kNumLimit))));
-        const T unL = uL[d + 1] * riL, unR = uR[d + 1] * riR;
-        const T dfL = fmax(fmax(fabs(unL - cL), fabs(unL)), fabs(unL + cL));
-        const T dfR = fmax(fmax(fabs(unR - cR), fabs(unR)), fabs(unR + cR));
-        const T C = fmax(fabs(dfL), fabs(dfR));
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const T fL = flux_q<D>(d, q, uL, uL[q], riL, pL);
-            const T fR = flux_q<D>(d, q, uR, uR[q], riR, pR);
-            const T F = T(0.5) * ((fL + fR) - C * (uR[q] - uL[q]));
-            a.Fnum[d][((long)q * Mbd + mb) * NFd + f] = F;
-#pragma unroll
-            for (int k = 0; k < Ns; ++k) S[q][k] = fma(T(Mbp[k * Mbd + mb]), F, S[q][k]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-#pragma unroll
-        for (int k = 0; k < Ns; ++k) a.Smom[d][((long)q * Ns + k) * NFd + f] = S[q][k];
+    const T riL = T(1) / uL[0], riR = T(1) / uR[0];
+    const T pL = pressure<D>(uL, riL, gm1), pR = pressure<D>(uR, riR, gm1);
+    const T cL = sqrt(gm * (pL / fmax(uL[0], T(kNumLimit))));
+    const T cR = sqrt(gm * (pR / fmax(uR[0], T(kNumLimit))));
+    const T unL = uL[d + 1] * riL, unR = uR[d + 1] * riR;
+    const T dfL = fmax(fmax(fabs(unL - cL), fabs(unL)), fabs(unL + cL));
+    const T dfR = fmax(fmax(fabs(unR - cR), fabs(unR)), fabs(unR + cR));
+    const T C = fmax(fabs(dfL), fabs(dfR));
+    const T fL = flux_q<D>(d, q, uL, vL, riL, pL);
+    const T fR = flux_q<D>(d, q, uR, vR, riR, pR);
+    a.Fnum[d][(f * Q + q) * Mbd + mb] = T(0.5) * ((fL + fR) - C * (vR - vL));
 }
 
 // ---------------------------------------------------------------------------
-// Corrector on valid cells (AmrDG.tpp:652-742) from the per-face moments:
+// Corrector on valid cells (AmrDG.tpp:652-742):
 //   U_w <- [M U_w + Vol - sum_d (Dt/dx_d) (S_hi - (-1)^{k_d} S_lo)] / M
 // with each face product subtracted / added on its own, in FP64.
 template <typename T>
@@ -575,7 +578,7 @@ struct CorrArgs {
     TabOff to;
     double* __restrict__ Uw;
     const T* __restrict__ Vol;
-    const T* __restrict__ Smom[3];
+    const T* __restrict__ Fnum[3];
     const int* __restrict__ cell_face[3][2];
     const int* __restrict__ valid_list;
     const uint8_t* __restrict__ slot_lev;
@@ -589,7 +592,7 @@ struct CorrArgs {
 template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nvalid * Q) return;
     const int q = (int)(t / a.nvalid);
@@ -608,11 +611,18 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     for (int d = 0; d < D; ++d) {
         const double cdd = Dt / a.geo.dx[lev][d];
         const long flo = a.cell_face[d][0][i], fhi = a.cell_face[d][1][i];
-        const T* __restrict__ S = a.Smom[d] + (long)q * Ns * a.NF[d];
+        const double* __restrict__ Mbp = a.tab + a.to.Mkbdp + (long)d * Ns * Mbd;
+        T Fh[Mbd], Fl[Mbd];
+        const T* __restrict__ ph = a.Fnum[d] + (fhi * Q + q) * Mbd;
+        const T* __restrict__ pl = a.Fnum[d] + (flo * Q + q) * Mbd;
+#pragma unroll
+        for (int mb = 0; mb < Mbd; ++mb) { Fh[mb] = ph[mb]; Fl[mb] = pl[mb]; }
 #pragma unroll
         for (int k = 0; k < Ns; ++k) {
-            r[k] -= cdd * double(S[(long)k * a.NF[d] + fhi]);
-            r[k] += cdd * (sgn[d * Ns + k] * double(S[(long)k * a.NF[d] + flo]));
+            const T sh = face_moment<T, Mbd>(Mbp + k * Mbd, Fh);
+            const T sl = face_moment<T, Mbd>(Mbp + k * Mbd, Fl);
+            r[k] -= cdd * double(sh);
+            r[k] += cdd * (sgn[d * Ns + k] * double(sl));
         }
     }
 #pragma unroll
@@ -624,12 +634,33 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
 // J (+-1)^{k_d} S, fine faces J * P_flux_{high|low}[d][child] Fnum whose mode
 // 0 row equals the quadrature weights, so it is taken from S as well
 // (AmrDG.tpp:604-647).
+template <typename T, int P, int D>
+__device__ __forceinline__ double register_moment(const double* __restrict__ tab, const TabOff& to,
+                                                  const T* __restrict__ Fnum_d, int d, int kind, int sel, long f,
+                                                  int q, int n, double jac) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
+    T F[Mbd];
+    const T* __restrict__ pf = Fnum_d + (f * Q + q) * Mbd;
+#pragma unroll
+    for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb];
+    const double* __restrict__ Mbp = tab + to.Mkbdp + (long)d * Ns * Mbd;
+    if (kind == 0 || n == 0) {
+        const double sv = double(face_moment<T, Mbd>(Mbp + n * Mbd, F));
+        return jac * ((kind == 1 || (sel & 1)) ? sv : tab[to.sgn + d * Ns + n] * sv);
+    }
+    const double* __restrict__ M = tab + ((sel & 1) ? to.Pfh : to.Pfl) + ((long)d * NFC + (sel >> 1)) * Ns * Mbd;
+    double s = 0.0;
+#pragma unroll
+    for (int mb = 0; mb < Mbd; ++mb) s = fma(M[n * Mbd + mb], double(F[mb]), s);
+    return jac * s;
+}
+
 template <typename T>
 struct MomArgs {
     const double* __restrict__ tab;
     TabOff to;
     const T* __restrict__ Fnum[3];
-    const T* __restrict__ Smom[3];
     double* __restrict__ Mom;
     const int* __restrict__ mom_face;
     const uint8_t* __restrict__ mom_d;
@@ -645,35 +676,16 @@ struct MomArgs {
 template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.NM * Q) return;
-    const int q = (int)(t / a.NM);
-    const long j = t - (long)q * a.NM;
-    const int d = a.mom_d[j], lev = a.mom_lev[j], sel = a.mom_sel[j];
-    const long f = a.mom_face[j];
+    if (t >= a.NM * Q * Ns) return;
+    const int qn = (int)(t / a.NM);
+    const long j = t - (long)qn * a.NM;
+    const int q = qn / Ns, n = qn - q * Ns;
+    const int d = a.mom_d[j], lev = a.mom_lev[j];
     const double jac = (a.ts->Dt / 2.0) * (a.geo.area[lev][d] / (double)(1 << (D - 1)));
-    const T* __restrict__ S = a.Smom[d] + (long)q * Ns * a.NF[d] + f;
-    double s[Ns];
-    if (a.mom_kind[j] == 0) {
-        const double* __restrict__ sgn = a.tab + a.to.sgn + d * Ns;
-#pragma unroll
-        for (int n = 0; n < Ns; ++n) s[n] = (sel & 1) ? double(S[(long)n * a.NF[d]]) : sgn[n] * double(S[(long)n * a.NF[d]]);
-    } else {
-        const int child = sel >> 1;
-        const double* __restrict__ M = a.tab + ((sel & 1) ? a.to.Pfh : a.to.Pfl) + ((long)d * NFC + child) * Ns * Mbd;
-        const T* __restrict__ F = a.Fnum[d] + (long)q * Mbd * a.NF[d] + f;
-#pragma unroll
-        for (int n = 0; n < Ns; ++n) s[n] = 0.0;
-        for (int mb = 0; mb < Mbd; ++mb) {
-            const double fv = double(F[(long)mb * a.NF[d]]);
-#pragma unroll
-            for (int n = 1; n < Ns; ++n) s[n] = fma(M[n * Mbd + mb], fv, s[n]);
-        }
-        s[0] = double(S[0]);
-    }
-#pragma unroll
-    for (int n = 0; n < Ns; ++n) a.Mom[((long)q * Ns + n) * a.NM + j] = s[n] * jac;
+    a.Mom[((long)q * Ns + n) * a.NM + j] =
+        register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
 }
 
 // Reflux: u_c += (2^D / vol_c) * Minv * sum(coef * moment) on corrected cells
@@ -698,25 +710,16 @@ __global__ void __launch_bounds__(256) k_reflux(RefluxArgs a) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nrc * Q) return;
-    const int q = (int)(t / a.nrc);
-    const long r = t - (long)q * a.nrc;
+    if (t >= a.nrc * Q * Ns) return;
+    const int qn = (int)(t / a.nrc);
+    const long r = t - (long)qn * a.nrc;
+    const int q = qn / Ns, n = qn - q * Ns;
     const long slot = a.rc_slot[r];
-    const int lev = a.rc_lev[r];
-    const double inv_jac = (double)(1 << D) / a.geo.vol[lev];
-    const double* __restrict__ Minv = a.tab + a.to.Minv;
-    double corr[Ns];
-#pragma unroll
-    for (int n = 0; n < Ns; ++n) corr[n] = 0.0;
+    const double inv_jac = (double)(1 << D) / a.geo.vol[a.rc_lev[r]];
+    double corr = 0.0;
     const int st = a.rc_start[r], cnt = a.rc_count[r];
-    for (int e = 0; e < cnt; ++e) {
-        const long j = a.re_mom[st + e];
-        const double cf = a.re_coef[st + e];
-#pragma unroll
-        for (int n = 0; n < Ns; ++n) corr[n] += cf * a.Mom[((long)q * Ns + n) * a.NM + j];
-    }
-#pragma unroll
-    for (int n = 0; n < Ns; ++n) a.Uw[((long)q * Ns + n) * a.NS + slot] += inv_jac * (Minv[n] * corr[n]);
+    for (int e = 0; e < cnt; ++e) corr += a.re_coef[st + e] * a.Mom[((long)q * Ns + n) * a.NM + a.re_mom[st + e]];
+    a.Uw[((long)q * Ns + n) * a.NS + slot] += inv_jac * (a.tab[a.to.Minv + n] * corr);
 }
 
 // ---------------------------------------------------------------------------
@@ -731,34 +734,14 @@ struct AvgArgs {
 };
 
 template <int P, int D>
+__device__ __forceinline__ void avgdown_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
+                                             const int* __restrict__ coarse, const int* __restrict__ fine,
+                                             long NS, long ngroup, long t);
+
+template <int P, int D>
 __global__ void __launch_bounds__(256) k_avgdown(AvgArgs a) {
-    using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, NCH = Dm::NCH;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.ngroup * Q) return;
-    const int q = (int)(t / a.ngroup);
-    const long g = t - (long)q * a.ngroup;
-    const double* __restrict__ Pfc = a.tab + a.to.Pfc;
-    const double* __restrict__ Minv = a.tab + a.to.Minv;
-    double s[Ns];
-#pragma unroll
-    for (int n = 0; n < Ns; ++n) s[n] = 0.0;
-    for (int c = 0; c < NCH; ++c) {
-        const long fs = a.fine[g * NCH + c];
-        double uf[Ns];
-#pragma unroll
-        for (int k = 0; k < Ns; ++k) uf[k] = a.Uw[((long)q * Ns + k) * a.NS + fs];
-#pragma unroll
-        for (int n = 0; n < Ns; ++n) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = 0; k < Ns; ++k) v = fma(Pfc[(c * Ns + n) * Ns + k], uf[k], v);
-            s[n] += v;
-        }
-    }
-    const long cs = a.coarse[g];
-#pragma unroll
-    for (int n = 0; n < Ns; ++n) a.Uw[((long)q * Ns + n) * a.NS + cs] = (1.0 / NCH) * (Minv[n] * s[n]);
+    avgdown_item<P, D>(a.tab, a.to, a.Uw, a.coarse, a.fine, a.NS, a.ngroup,
+                       (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Ghost fill: same-level / periodic copy, or u_f = Minv P_cf[child] u_c
@@ -775,31 +758,15 @@ struct GhostArgs {
 };
 
 template <int P, int D>
+__device__ __forceinline__ void ghost_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
+                                           const int* __restrict__ g_slot, const int* __restrict__ g_src,
+                                           const uint8_t* __restrict__ g_kind, const uint8_t* __restrict__ g_child,
+                                           long NS, long nghost, long t);
+
+template <int P, int D>
 __global__ void __launch_bounds__(256) k_ghosts(GhostArgs a) {
-    using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nghost * Q) return;
-    const int q = (int)(t / a.nghost);
-    const long g = t - (long)q * a.nghost;
-    const long s = a.g_slot[g], src = a.g_src[g];
-    double uc[Ns];
-#pragma unroll
-    for (int k = 0; k < Ns; ++k) uc[k] = a.Uw[((long)q * Ns + k) * a.NS + src];
-    if (a.g_kind[g] == 0) {
-#pragma unroll
-        for (int k = 0; k < Ns; ++k) a.Uw[((long)q * Ns + k) * a.NS + s] = uc[k];
-        return;
-    }
-    const double* __restrict__ Pcf = a.tab + a.to.Pcf + (long)a.g_child[g] * Ns * Ns;
-    const double* __restrict__ Minv = a.tab + a.to.Minv;
-#pragma unroll
-    for (int n = 0; n < Ns; ++n) {
-        double v = 0.0;
-#pragma unroll
-        for (int k = 0; k < Ns; ++k) v = fma(Pcf[n * Ns + k], uc[k], v);
-        a.Uw[((long)q * Ns + n) * a.NS + s] = Minv[n] * v;
-    }
+    ghost_item<P, D>(a.tab, a.to, a.Uw, a.g_slot, a.g_src, a.g_kind, a.g_child, a.NS, a.nghost,
+                     (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -858,6 +825,200 @@ __global__ void k_dt_finalize(TimeState* ts, double c_dt, double T, int advance)
     if (T - ts->t < Dt) Dt = T - ts->t;
     ts->Dt = Dt;
     ts->dtmin_bits = 0x7ff0000000000000ull;  // +inf for the next reduction
+}
+
+// ---------------------------------------------------------------------------
+// Fused step epilogue.  Device functions of the reflux, average-down, ghost
+// and CFL work items, and two kernels that run disjoint work items side by
+// side so that one step needs few launches:
+//   k_reflux_avg : reflux rows (moments computed inline) || average down of
+//                  the finest level (coarse cells written are disjoint: the
+//                  reflux touches uncovered cells only)
+//   k_ghost_cfl  : ghost fill || CFL reduction; the last CFL block to finish
+//                  advances t and sets the next Dt (no extra launch).
+template <typename T>
+struct EpiArgs {
+    const double* __restrict__ tab;
+    TabOff to;
+    double* __restrict__ Uw;
+    const T* __restrict__ Fnum[3];
+    long NF[3];
+    // reflux rows
+    const int* __restrict__ mom_face;
+    const uint8_t* __restrict__ mom_d;
+    const uint8_t* __restrict__ mom_kind;
+    const uint8_t* __restrict__ mom_sel;
+    const uint8_t* __restrict__ mom_lev;
+    const int* __restrict__ rc_slot;
+    const int* __restrict__ rc_start;
+    const int* __restrict__ rc_count;
+    const uint8_t* __restrict__ rc_lev;
+    const int* __restrict__ re_mom;
+    const double* __restrict__ re_coef;
+    long nrc;
+    // average down groups
+    const int* __restrict__ ad_coarse;
+    const int* __restrict__ ad_fine;
+    long ngroup;
+    // ghosts
+    const int* __restrict__ g_slot;
+    const int* __restrict__ g_src;
+    const uint8_t* __restrict__ g_kind;
+    const uint8_t* __restrict__ g_child;
+    long nghost;
+    // CFL
+    const int* __restrict__ valid_list;
+    const uint8_t* __restrict__ slot_lev;
+    long nvalid;
+    unsigned ncfl_blocks;
+    double cfl, c_dt, Tend;
+    TimeState* ts;
+    LevelGeo geo;
+    long NS;
+    double gamma;
+    unsigned nblk_a;   // blocks of the first work type
+};
+
+template <typename T, int P, int D>
+__device__ __forceinline__ void reflux_item(const EpiArgs<T>& a, long t) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
+    if (t >= a.nrc * Q * Ns) return;
+    const int qn = (int)(t / a.nrc);
+    const long r = t - (long)qn * a.nrc;
+    const int q = qn / Ns, n = qn - q * Ns;
+    const long slot = a.rc_slot[r];
+    const int lev = a.rc_lev[r];
+    const double inv_jac = (double)(1 << D) / a.geo.vol[lev];
+    const double Dt = a.ts->Dt;
+    double corr = 0.0;
+    const int st = a.rc_start[r], cnt = a.rc_count[r];
+    for (int e = 0; e < cnt; ++e) {
+        const long j = a.re_mom[st + e];
+        const int d = a.mom_d[j], ml = a.mom_lev[j];
+        const double jac = (Dt / 2.0) * (a.geo.area[ml][d] / (double)(1 << (D - 1)));
+        corr += a.re_coef[st + e] *
+                register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
+    }
+    a.Uw[((long)q * Ns + n) * a.NS + slot] += inv_jac * (a.tab[a.to.Minv + n] * corr);
+}
+
+// u_c[n] = 2^-D Minv[n] sum_c sum_k P_fc[c](n,k) u_f[c][k], one thread per (group, q, n)
+template <int P, int D>
+__device__ __forceinline__ void avgdown_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
+                                             const int* __restrict__ coarse, const int* __restrict__ fine,
+                                             long NS, long ngroup, long t) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, NCH = Dm::NCH;
+    if (t >= ngroup * Q * Ns) return;
+    const int qn = (int)(t / ngroup);
+    const long g = t - (long)qn * ngroup;
+    const int q = qn / Ns, n = qn - q * Ns;
+    const double* __restrict__ Pfc = tab + to.Pfc;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const long fs = fine[g * NCH + c];
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < Ns; ++k) v = fma(Pfc[(c * Ns + n) * Ns + k], Uw[((long)q * Ns + k) * NS + fs], v);
+        s += v;
+    }
+    Uw[((long)q * Ns + n) * NS + coarse[g]] = (1.0 / NCH) * (tab[to.Minv + n] * s);
+}
+
+// ghost slot mode n: copy, or Minv[n] sum_k P_cf[child](n,k) u_c[k]
+template <int P, int D>
+__device__ __forceinline__ void ghost_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
+                                           const int* __restrict__ g_slot, const int* __restrict__ g_src,
+                                           const uint8_t* __restrict__ g_kind, const uint8_t* __restrict__ g_child,
+                                           long NS, long nghost, long t) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
+    if (t >= nghost * Q * Ns) return;
+    const int qn = (int)(t / nghost);
+    const long g = t - (long)qn * nghost;
+    const int q = qn / Ns, n = qn - q * Ns;
+    const long s = g_slot[g], src = g_src[g];
+    if (g_kind[g] == 0) {
+        Uw[((long)q * Ns + n) * NS + s] = Uw[((long)q * Ns + n) * NS + src];
+        return;
+    }
+    const double* __restrict__ Pcf = tab + to.Pcf + (long)g_child[g] * Ns * Ns;
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < Ns; ++k) v = fma(Pcf[n * Ns + k], Uw[((long)q * Ns + k) * NS + src], v);
+    Uw[((long)q * Ns + n) * NS + s] = tab[to.Minv + n] * v;
+}
+
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(256) k_reflux_avg(EpiArgs<T> a) {
+    const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x < a.nblk_a) {
+        reflux_item<T, P, D>(a, t0);
+    } else {
+        const long t = t0 - (long)a.nblk_a * blockDim.x;
+        avgdown_item<P, D>(a.tab, a.to, a.Uw, a.ad_coarse, a.ad_fine, a.NS, a.ngroup, t);
+    }
+}
+
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(256) k_ghost_cfl(EpiArgs<T> a) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
+    if (blockIdx.x < a.nblk_a) {
+        const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        ghost_item<P, D>(a.tab, a.to, a.Uw, a.g_slot, a.g_src, a.g_kind, a.g_child, a.NS, a.nghost, t);
+        return;
+    }
+    // CFL (AmrDG.h:1300-1314) + last-block finalize (AmrDG.h:1018-1029)
+    const long i = (long)(blockIdx.x - a.nblk_a) * blockDim.x + threadIdx.x;
+    double dt = 1.0e300;
+    if (i < a.nvalid) {
+        const long slot = a.valid_list[i];
+        const int lev = a.slot_lev[slot];
+        const double* __restrict__ phi0 = a.tab + a.to.phi0;
+        double s[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            double v = 0.0;
+#pragma unroll
+            for (int n = 0; n < Ns; ++n) v += a.Uw[((long)q * Ns + n) * a.NS + slot] * phi0[n];
+            s[q] = v;
+        }
+        const double rinv = 1.0 / s[0];
+        const double prs = pressure<D>(s, rinv, a.gamma - 1.0);
+        const double c = sqrt(a.gamma * (prs / fmax(s[0], kNumLimit)));
+        double lmax = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) lmax = fmax(lmax, fabs(s[d + 1] / s[0]) + c);
+        dt = a.cfl * (a.geo.dx_avg[lev] / lmax);
+    }
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
+    __shared__ double wmin[4];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin(m, wmin[w]);
+        if (m < 1.0e300) atomicMin(&a.ts->dtmin_bits, (unsigned long long)__double_as_longlong(m));
+    }
+    __threadfence();
+    __shared__ unsigned last;
+    if (threadIdx.x == 0) last = (atomicAdd(&a.ts->done, 1u) == a.ncfl_blocks - 1) ? 1u : 0u;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        const unsigned long long bits = atomicAdd(&a.ts->dtmin_bits, 0ull);
+        const double tn = a.ts->t + a.ts->Dt;
+        double Dt = a.c_dt * __longlong_as_double((long long)bits);
+        if (a.Tend - tn < Dt) Dt = a.Tend - tn;
+        a.ts->t = tn;
+        a.ts->Dt = Dt;
+        a.ts->dtmin_bits = 0x7ff0000000000000ull;
+        a.ts->done = 0u;
+        __threadfence();
+    }
 }
 
 // Conserved totals: per-block partial sums of vol * Uw[q][0] (deterministic

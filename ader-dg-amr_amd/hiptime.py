@@ -55,3 +55,22 @@ class Event:
             _hip().hipEventDestroy(self.h)
         except Exception:
             pass
+
+
+def copy_probe(nbytes, reps=3):
+    """Device-to-device copies of nbytes (a plain copy kernel of known size,
+    used to calibrate the FETCH_SIZE / WRITE_SIZE counters)."""
+    h = _hip()
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipDeviceSynchronize.argtypes = []
+    a, b = C.c_void_p(), C.c_void_p()
+    if h.hipMalloc(C.byref(a), nbytes) or h.hipMalloc(C.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    for _ in range(reps):
+        if h.hipMemcpy(b, a, nbytes, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    h.hipDeviceSynchronize()
+    h.hipFree(a)
+    h.hipFree(b)

@@ -23,7 +23,7 @@ EXPORTS = [
     "adg_create", "adg_set_geometry", "adg_set_levels", "adg_level_nslot", "adg_level_nface",
     "adg_upload_Uw", "adg_download_Uw", "adg_fill_ghosts", "adg_average_down", "adg_set_dt",
     "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
-    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_set_time",
+    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_set_time",
     "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
     "adg_project_ic", "adg_set_predictor_kernel", "adg_last_error", "adg_destroy",
 ]
@@ -61,6 +61,7 @@ def lib():
     L.adg_ader.argtypes = [C.c_void_p]
     L.adg_advance.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     L.adg_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+    L.adg_step_graph.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp]
     L.adg_set_time.argtypes = [C.c_void_p, C.c_double]
     L.adg_conserved_totals.argtypes = [C.c_void_p, _dp]
     L.adg_sync.argtypes = [C.c_void_p]
@@ -203,6 +204,9 @@ class Context:
             return float(out[0])
         self._chk(self.L.adg_step(self.h, c_dt, T, None), "adg_step")
         return None
+
+    def step_graph(self, c_dt, T, nsteps):
+        self._chk(self.L.adg_step_graph(self.h, c_dt, T, nsteps, None), "adg_step_graph")
 
     def totals(self):
         out = np.zeros(self.Q)

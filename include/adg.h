@@ -120,7 +120,7 @@ int adg_download_Hw(adg_ctx* ctx, int lev, double* host);
  * AmrDG_NumericalFlux.tpp:4-41): face-centred, once per face.
  * lev = -1: every level. */
 int adg_face_flux(adg_ctx* ctx, int lev);
-/* Readback of Fnum for (lev, d): [q][qMp_st_bd][face]. */
+/* Readback of Fnum for (lev, d), returned as [q][qMp_st_bd][face]. */
 int adg_download_Fnum(adg_ctx* ctx, int lev, int d, double* host);
 
 /* Replaces AmrDG::update_U_w (AmrDG.tpp:652-742) on valid cells.
@@ -146,6 +146,10 @@ int adg_advance(adg_ctx* ctx, double c_dt, double T, double* t_out);
  * plots (AmrDG.h:988-1029): ADER, average_down, FillPatch, t+=Dt, set_Dt,
  * clamp to T.  t stays on the device; t_out (nullable) gets a blocking copy. */
 int adg_step(adg_ctx* ctx, double c_dt, double T, double* t_out);
+/* nsteps iterations of adg_step replayed from one captured hipGraph (the
+ * graph is captured on first use and re-captured when c_dt, T or the
+ * predictor kernel change).  t_out (nullable) gets a blocking copy of t. */
+int adg_step_graph(adg_ctx* ctx, double c_dt, double T, int nsteps, double* t_out);
 /* Sets the device-resident time. */
 int adg_set_time(adg_ctx* ctx, double t);
 

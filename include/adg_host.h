@@ -1,0 +1,31 @@
+/*
+ * adg_host.h -- host-only diagnostics of libadg.so (no device work).
+ *
+ * No counterpart in the reference: these expose the host AMR substrate
+ * (the flattened hierarchy lists of adg_set_levels) and the MFMA lane
+ * layouts the kernels assume, for tests.
+ */
+#ifndef ADG_HOST_H
+#define ADG_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Coarse-fine face masks of a box layout as AMR_set_flux_registers
+ * (AmrDG_Level.tpp:240-414) defines them: kind 0 b_coarse (with the
+ * ownership rule), 1 b_fine, 2 child index, for the faces of direction d of
+ * level lev; kind 3: counts {slots, ghosts, faces d0..d2, register moments,
+ * reflux rows, uncovered cells}.  Returns the number of ints or < 0. */
+int adg_host_inspect(int dim, const int* dom_n, const int* periodic, int nlev, const int* npatch,
+                     const int* boxes, int lev, int d, int kind, int* out, int cap);
+
+/* Runs one 16x16x4 MFMA on the current device and returns the 64x4
+ * accumulator (prec 0: FP64, 1: FP32; mode 0 probes A rows, mode 1 B
+ * columns).  Needs a GPU. */
+int adg_probe_mfma_layout(int prec, int mode, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
