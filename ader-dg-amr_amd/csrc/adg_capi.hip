@@ -45,12 +45,15 @@ struct adg_ctx {
     void *Vol = nullptr, *Tr = nullptr;     // element type of the form (double / float)
     void* Fnum[3] = {nullptr, nullptr, nullptr};
     size_t esz = sizeof(double);
-    adg::Fragments frags;
-    adg::FragOff fo{};
-    double* d_frag64 = nullptr;
-    float* d_frag32 = nullptr;
-    int pred_kind = 0;       // 0 auto, 1 VALU, 2 MFMA
+    // MFMA A-operand fragments in the element type of the form:
+    // [0] quad-gather predictor, [1] node-lane predictor
+    adg::Fragments frags[2];
+    adg::FragOff fo[2]{};
+    void* d_frag[2] = {nullptr, nullptr};
+    int pred_kind = 0;       // 0 auto, 1 VALU, 2 MFMA quad-gather, 3 MFMA node-lane
+    int nl_wpb = 8;          // wavefronts per workgroup of the node-lane predictor (8 or 16)
     int ncu = 256;
+    int pred_per_cu[3] = {1, 1, 1};   // resident workgroups per CU: quad, node-lane x8, node-lane x16
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     double graph_cdt = -1.0, graph_T = -1.0;
@@ -117,17 +120,34 @@ int dev_alloc(adg_ctx* c, double** dst, size_t n) {
     return rc;
 }
 
-// LDS bytes of the MFMA predictor for a 512-thread workgroup.
-size_t mfma_lds_bytes(const adg_ctx* c) {
-    // fragments, plus the per-wave H_w re-layout buffer of the FP32 form
-    const size_t xch = c->esz == sizeof(double) ? 0 : (size_t)8 * c->frags.MB2 * 256;
-    return ((size_t)c->frags.total * 64 + xch) * c->esz;
+enum { PV_VALU = 0, PV_QUAD = 1, PV_NL = 2 };
+constexpr size_t kLdsMax = 160 * 1024;
+
+// Dynamic LDS bytes of an MFMA predictor workgroup: the fragments, plus the
+// per-wave H_w re-layout buffer of the FP32 quad-gather form (8 waves) or the
+// per-wave flux tile of the node-lane form (wpb waves).
+size_t mfma_lds_bytes(const adg_ctx* c, int variant, int wpb = 8) {
+    if (variant == PV_QUAD) {
+        const size_t xch = c->esz == sizeof(double) ? 0 : (size_t)8 * c->frags[0].MB2 * 256;
+        return ((size_t)c->frags[0].total * 64 + xch) * c->esz;
+    }
+    const size_t tst = c->esz == sizeof(double) ? adg::nl_tile_stride<double>() : adg::nl_tile_stride<float>();
+    return ((size_t)c->frags[1].total * 64 + (size_t)wpb * c->D * 16 * tst) * c->esz;
 }
-bool use_mfma(const adg_ctx* c) {
-    const bool fits = mfma_lds_bytes(c) <= 160 * 1024;
-    if (c->pred_kind == 1) return c->precision != ADG_FP64 && fits;
-    if (c->pred_kind == 2) return fits;
-    return fits && !(c->P == 1);
+bool nl_available(const adg_ctx* c, int wpb) { return c->Q == 4 && mfma_lds_bytes(c, PV_NL, wpb) <= kLdsMax; }
+bool quad_available(const adg_ctx* c) { return mfma_lds_bytes(c, PV_QUAD) <= kLdsMax; }
+// Predictor kernel of the next launch.
+int pred_variant(const adg_ctx* c) {
+    switch (c->pred_kind) {
+        case 1: return PV_VALU;
+        case 2: return PV_QUAD;
+        case 3: return PV_NL;
+        default: break;
+    }
+    if (c->precision == ADG_FP64 && c->P == 1) return PV_VALU;
+    if (nl_available(c, c->nl_wpb)) return PV_NL;
+    if (quad_available(c)) return PV_QUAD;
+    return PV_VALU;
 }
 
 void drop_graph(adg_ctx* c) {
@@ -197,19 +217,21 @@ int pack_tables(adg_ctx* c) {
     HIPCHK(c, hipMalloc(&p, buf.size() * sizeof(double)));
     c->d_tab = static_cast<double*>(p);
     HIPCHK(c, hipMemcpy(c->d_tab, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
-    // MFMA A-operand fragments in both element types (K order follows each
-    // type's accumulator row order)
-    c->frags = adg::build_fragments(t, true);
-    c->fo = adg::FragOff{c->frags.A0, c->frags.V, c->frags.B, c->frags.S, c->frags.T, c->frags.total};
-    const std::vector<double>& fd = c->frags.data;
-    adg::Fragments f32 = adg::build_fragments(t, false);
-    std::vector<float> ff(f32.data.begin(), f32.data.end());
-    HIPCHK(c, hipMalloc(&p, fd.size() * sizeof(double)));
-    c->d_frag64 = static_cast<double*>(p);
-    HIPCHK(c, hipMemcpy(c->d_frag64, fd.data(), fd.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMalloc(&p, ff.size() * sizeof(float)));
-    c->d_frag32 = static_cast<float*>(p);
-    HIPCHK(c, hipMemcpy(c->d_frag32, ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
+    // MFMA A-operand fragments of both predictor layouts in the element type
+    // of the form (K order / row permutation follow its accumulator rows)
+    const bool fp64 = c->precision == ADG_FP64;
+    for (int v = 0; v < 2; ++v) {
+        adg::Fragments& f = c->frags[v];
+        f = adg::build_fragments(t, fp64, v == 1);
+        c->fo[v] = adg::FragOff{f.A0, f.V, f.B, f.S, f.T, f.total};
+        HIPCHK(c, hipMalloc(&c->d_frag[v], f.data.size() * c->esz));
+        if (fp64) {
+            HIPCHK(c, hipMemcpy(c->d_frag[v], f.data.data(), f.data.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> ff(f.data.begin(), f.data.end());
+            HIPCHK(c, hipMemcpy(c->d_frag[v], ff.data(), ff.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
     return ADG_OK;
 }
 
@@ -250,7 +272,7 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
             c->ncu = ncu;
     }
-    if (precision == ADG_FP32_MFMA && mfma_lds_bytes(c) > 160 * 1024) { adg_destroy(c); return ADG_E_UNSUP; }
+    if (precision == ADG_FP32_MFMA && !quad_available(c) && !nl_available(c, 8)) { adg_destroy(c); return ADG_E_UNSUP; }
     void* tsp = nullptr;
     if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->ts = static_cast<adg::TimeState*>(tsp);
@@ -338,7 +360,7 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
 #undef UP
     const size_t NS = (size_t)H.NS;
     if ((rc = dev_alloc(c, &c->Uw, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
-    const size_t ves = (size_t)c->frags.MBS * 16 * Q, tes = (size_t)2 * D * c->frags.MBD * 16 * Q;
+    const size_t ves = (size_t)Q * t.Np_s, tes = (size_t)2 * D * t.qMp_st_bd * Q;   // per slot
     if ((rc = dev_alloc_bytes(c, &c->Vol, ves * NS * c->esz)) != ADG_OK) return rc;
     if ((rc = dev_alloc_bytes(c, &c->Tr, tes * NS * c->esz)) != ADG_OK) return rc;
     for (int d = 0; d < D; ++d)
@@ -352,23 +374,43 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     return prepare_kernels(c);
 }
 
-// Raises the dynamic-LDS limit of the MFMA predictor once, outside any
-// stream capture.
-static int prepare_kernels(adg_ctx* c) {
-    const int lds = (int)mfma_lds_bytes(c);
-    if (lds > 160 * 1024) return ADG_OK;
-#define L_ATTR(PP, DD)                                                                                    \
-    {                                                                                                     \
-        if (c->precision == ADG_FP64)                                                                     \
-            HIPCHK(c, hipFuncSetAttribute((const void*)adg::k_predictor_mfma<double, PP, DD>,             \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds));              \
-        else                                                                                              \
-            HIPCHK(c, hipFuncSetAttribute((const void*)adg::k_predictor_mfma<float, PP, DD>,              \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds));              \
+// Raises the dynamic-LDS limit of the MFMA predictors once, outside any
+// stream capture, and records how many of their workgroups a CU holds.
+extern "C++" {
+template <typename T, int P, int D>
+static int prepare_pred(adg_ctx* c) {
+    auto setup = [&](const void* kf, int threads, size_t lds, int& per_cu) -> int {
+        per_cu = 0;
+        if (lds > kLdsMax) return ADG_OK;
+        HIPCHK(c, hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int nb = 0;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kf, threads, lds));
+        per_cu = nb > 0 ? nb : 1;
+        return ADG_OK;
+    };
+    int rc = setup(adg::predictor_mfma_kernel<T, P, D>(), 512, mfma_lds_bytes(c, PV_QUAD), c->pred_per_cu[0]);
+    if (rc == ADG_OK && c->Q == 4) {
+        rc = setup((const void*)adg::k_predictor_nl<T, P, D, 8>, 512, mfma_lds_bytes(c, PV_NL, 8), c->pred_per_cu[1]);
+        if (rc == ADG_OK)
+            rc = setup((const void*)adg::k_predictor_nl<T, P, D, 16>, 1024, mfma_lds_bytes(c, PV_NL, 16),
+                       c->pred_per_cu[2]);
     }
-    ADG_DISPATCH(c, L_ATTR);
-#undef L_ATTR
-    return ADG_OK;
+    return rc;
+}
+}  // extern "C++"
+
+static int prepare_kernels(adg_ctx* c) {
+    int rc = ADG_OK;
+    if (c->precision == ADG_FP64) {
+#define L_PP(PP, DD) rc = prepare_pred<double, PP, DD>(c)
+        ADG_DISPATCH(c, L_PP);
+#undef L_PP
+    } else {
+#define L_PP(PP, DD) rc = prepare_pred<float, PP, DD>(c)
+        ADG_DISPATCH(c, L_PP);
+#undef L_PP
+    }
+    return rc;
 }
 
 int adg_level_nslot(adg_ctx* c, int lev) {
@@ -434,14 +476,13 @@ int adg_download_Fnum(adg_ctx* c, int lev, int d, double* host) {
     if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev || d < 0 || d >= c->D || !host)
         return fail(c, ADG_E_ARG, "bad arguments");
     const size_t nf = (size_t)c->H.lev[lev].nface[d], off = (size_t)c->H.lev[lev].face_off[d];
-    const size_t row = (size_t)c->Q * c->tab.qMp_st_bd, es = c->esz;  // values per face
-    std::vector<unsigned char> tmp(row * nf * es);
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), (unsigned char*)c->Fnum[d] + off * row * es, tmp.size(),
-                             hipMemcpyDeviceToHost, c->stream));
+    const size_t NF = (size_t)c->H.NF[d], rows = (size_t)c->Q * c->tab.qMp_st_bd, es = c->esz;
+    std::vector<unsigned char> tmp(rows * NF * es);   // device layout [q][mb][face]
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->Fnum[d], tmp.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t f = 0; f < nf; ++f)
-        for (size_t i = 0; i < row; ++i) {
-            const size_t src = f * row + i;
+    for (size_t i = 0; i < rows; ++i)
+        for (size_t f = 0; f < nf; ++f) {
+            const size_t src = i * NF + off + f;
             host[i * nf + f] = es == sizeof(double) ? reinterpret_cast<double*>(tmp.data())[src]
                                                     : (double)reinterpret_cast<float*>(tmp.data())[src];
         }
@@ -516,10 +557,11 @@ int adg_set_dt_value(adg_ctx* c, double Dt) {
 
 extern "C++" {
 template <typename T, int P, int D>
-static int launch_pred_mfma(adg_ctx* c, long begin, long n) {
+static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant) {
+    const int v = variant == PV_NL ? 1 : 0;
     adg::PredMArgs<T> a{};
-    a.frags = std::is_same<T, double>::value ? (const T*)c->d_frag64 : (const T*)c->d_frag32;
-    a.fo = c->fo;
+    a.frags = (const T*)c->d_frag[v];
+    a.fo = c->fo[v];
     a.Uw = c->Uw;
     a.Vol = (T*)c->Vol;
     a.Tr = (T*)c->Tr;
@@ -531,13 +573,20 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n) {
     a.slot_begin = begin;
     a.nelem = n;
     a.gamma = c->gamma;
-    const size_t lds = mfma_lds_bytes(c);
-    auto kern = adg::k_predictor_mfma<T, P, D>;
-    const long per_batch = 8L * adg::Dims<P, D>::EPM;
+    const int wpb = variant == PV_NL ? c->nl_wpb : 8;
+    const size_t lds = mfma_lds_bytes(c, variant, wpb);
+    const void* kern = variant == PV_NL ? (wpb == 16 ? (const void*)adg::k_predictor_nl<T, P, D, 16>
+                                                     : (const void*)adg::k_predictor_nl<T, P, D, 8>)
+                                        : adg::predictor_mfma_kernel<T, P, D>();
+    const int per_cu = c->pred_per_cu[variant == PV_NL ? (wpb == 16 ? 2 : 1) : 0];
+    if (per_cu <= 0) return fail(c, ADG_E_UNSUP, "predictor workgroup does not fit a CU");
+    const long per_batch = (long)wpb * (variant == PV_NL ? 4 : adg::Dims<P, D>::EPM);
     const long nbatch = (n + per_batch - 1) / per_batch;
-    const long per_cu = std::max<long>(1, (long)((160 * 1024) / lds));
-    const long grid = std::min<long>(nbatch, per_cu * c->ncu);
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(512), lds, c->stream, a);
+    // persistent grid: exactly the workgroups that are resident at once
+    // (registers and LDS both limit it), so no workgroup waits for a CU
+    const long grid = std::min<long>(nbatch, (long)per_cu * c->ncu);
+    void* kargs[] = {&a};
+    HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(64 * wpb), kargs, lds, c->stream));
     HIPCHK(c, hipGetLastError());
     return ADG_OK;
 }
@@ -545,10 +594,22 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n) {
 
 int adg_set_predictor_kernel(adg_ctx* c, int kind) {
     if (!c) return ADG_E_ARG;
-    if (kind < 0 || kind > 2) return fail(c, ADG_E_ARG, "kind must be 0, 1 or 2");
+    if (kind < 0 || kind > 3) return fail(c, ADG_E_ARG, "kind must be 0, 1, 2 or 3");
     if (kind == 1 && c->precision != ADG_FP64) return fail(c, ADG_E_UNSUP, "the VALU predictor is FP64 only");
-    if (kind == 2 && mfma_lds_bytes(c) > 160 * 1024) return fail(c, ADG_E_UNSUP, "MFMA fragments exceed LDS");
+    if (kind == 2 && !quad_available(c)) return fail(c, ADG_E_UNSUP, "MFMA fragments exceed LDS");
+    if (kind == 3 && !nl_available(c, c->nl_wpb))
+        return fail(c, ADG_E_UNSUP, "the node-lane predictor needs Q = 4 and its fragments in LDS");
     c->pred_kind = kind;
+    drop_graph(c);
+    return ADG_OK;
+}
+
+int adg_set_predictor_waves(adg_ctx* c, int wpb) {
+    if (!c) return ADG_E_ARG;
+    if (wpb != 8 && wpb != 16) return fail(c, ADG_E_ARG, "waves per block must be 8 or 16");
+    if (c->pred_kind == 3 && !nl_available(c, wpb)) return fail(c, ADG_E_UNSUP, "node-lane tile exceeds LDS");
+    c->nl_wpb = wpb;
+    drop_graph(c);
     return ADG_OK;
 }
 
@@ -560,14 +621,15 @@ int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     long begin = 0, n = c->H.NS;
     if (lev >= 0) { begin = c->H.lev[lev].slot_off; n = c->H.lev[lev].nslot; }
     if (n == 0) return ADG_OK;
-    if (use_mfma(c) || c->precision == ADG_FP32_MFMA) {
+    const int variant = pred_variant(c);
+    if (variant != PV_VALU) {
         int rc = ADG_OK;
         if (c->precision == ADG_FP64) {
-#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n)
+#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n, variant)
             ADG_DISPATCH(c, L_PM);
 #undef L_PM
         } else {
-#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n)
+#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n, variant)
             ADG_DISPATCH(c, L_PM);
 #undef L_PM
         }
@@ -599,9 +661,9 @@ static int launch_faceflux(adg_ctx* c) {
         a.NF[d] = c->H.NF[d];
         tot += c->H.NF[d];
     }
+    a.NS = c->H.NS;
     a.gamma = c->gamma;
-    const long tpf = 64 * ((c->tab.qMp_st_bd * c->Q + 63) / 64);
-    tot *= tpf;
+    tot *= c->tab.qMp_st_bd;   // one thread per (face, node)
 #define L_FF(PP, DD) hipLaunchKernelGGL((adg::k_faceflux<T, PP, DD>), dim3(nblocks(tot)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_FF);
 #undef L_FF
@@ -778,7 +840,7 @@ static int step_launches(adg_ctx* c, double c_dt, double T_end) {
 int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_out) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     if (nsteps < 0) return fail(c, ADG_E_ARG, "nsteps must be >= 0");
-    const int kind = use_mfma(c) ? 2 : 1;
+    const int kind = pred_variant(c) * 32 + c->nl_wpb;   // launch shape baked into the graph
     if (!c->graph_exec || c->graph_cdt != c_dt || c->graph_T != T_end || c->graph_kind != kind) {
         drop_graph(c);
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -930,8 +992,8 @@ void adg_destroy(adg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_levels(c);
     if (c->d_tab) (void)hipFree(c->d_tab);
-    if (c->d_frag64) (void)hipFree(c->d_frag64);
-    if (c->d_frag32) (void)hipFree(c->d_frag32);
+    for (int v = 0; v < 2; ++v)
+        if (c->d_frag[v]) (void)hipFree(c->d_frag[v]);
     if (c->ts) (void)hipFree(c->ts);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

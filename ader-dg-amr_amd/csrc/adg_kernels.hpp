@@ -49,8 +49,6 @@ struct Dims {
     static constexpr int KSH = cdiv(Nst, 4);   // K steps over H_w modes
     static constexpr int MBS = cdiv(Ns, 16);   // row blocks of the corrector volume term
     static constexpr int MBD = cdiv(Mbd, 16);  // row blocks of a face trace
-    static constexpr int TES = 2 * D * MBD * 16 * Q;  // Tr entries per slot
-    static constexpr int VES = MBS * 16 * Q;          // Vol entries per slot
 };
 
 // Row of a 16x16x4 MFMA accumulator held by register r of lane group g
@@ -60,22 +58,18 @@ __host__ __device__ constexpr int acc_row(int g, int r) {
     return sizeof(T) == 8 ? 4 * r + g : 4 * g + r;
 }
 
-// Index of trace node mb, variable q, direction/side ds inside a slot's Tr
-// block, and of mode k inside its Vol block: the position a wavefront's
-// accumulator store puts it at (register r, lane group g -> ((r*4)+g)*Q+q).
-template <typename T, int P, int D>
-__host__ __device__ __forceinline__ int tr_off(int ds, int mb, int q) {
+// Structure-of-arrays layouts, slot (or face) index fastest:
+//   Tr  [ds][mb][q][slot]   face traces, ds = 2*d + side (0 = lo, 1 = hi)
+//   Vol [q][k][slot]        corrector volume term, like U_w
+//   Fnum[d][q][mb][face]    numerical flux at the face quadrature nodes
+template <int P, int D>
+__host__ __device__ __forceinline__ long tr_idx(int ds, int mb, int q, long slot, long NS) {
     using Dm = Dims<P, D>;
-    const int blk = mb >> 4, row = mb & 15;
-    const int r = sizeof(T) == 8 ? row >> 2 : row & 3, g = sizeof(T) == 8 ? row & 3 : row >> 2;
-    return (((ds * Dm::MBD + blk) * 4 + r) * 4 + g) * Dm::Q + q;
+    return ((long)(ds * Dm::Mbd + mb) * Dm::Q + q) * NS + slot;
 }
-template <typename T, int P, int D>
-__host__ __device__ __forceinline__ int vol_off(int k, int q) {
-    using Dm = Dims<P, D>;
-    const int blk = k >> 4, row = k & 15;
-    const int r = sizeof(T) == 8 ? row >> 2 : row & 3, g = sizeof(T) == 8 ? row & 3 : row >> 2;
-    return ((blk * 4 + r) * 4 + g) * Dm::Q + q;
+template <int P, int D>
+__host__ __device__ __forceinline__ long vol_idx(int k, int q, long slot, long NS) {
+    return ((long)q * Dims<P, D>::Ns + k) * NS + slot;
 }
 
 // Offsets (in doubles) of every table inside the device table buffer.
@@ -267,7 +261,7 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
     }
     if (active) {
 #pragma unroll
-        for (int k = 0; k < Ns; ++k) a.Vol[slot * Dm::VES + vol_off<double, P, D>(k, q)] = vol[k];
+        for (int k = 0; k < Ns; ++k) a.Vol[vol_idx<P, D>(k, q, slot, NS)] = vol[k];
         if (a.Hw_dbg) {
 #pragma unroll
             for (int n = 0; n < Nst; ++n) a.Hw_dbg[((long)q * Nst + n) * NS + slot] = hw[n];
@@ -279,7 +273,7 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
             double tr = 0.0;
 #pragma unroll
             for (int n = 0; n < Nst; ++n) tr = fma(Vb[mb * Nst + n], hw[n], tr);
-            if (active) a.Tr[slot * Dm::TES + tr_off<double, P, D>(ds, mb, q)] = tr;
+            if (active) a.Tr[tr_idx<P, D>(ds, mb, q, slot, NS)] = tr;
         }
     }
 }
@@ -313,6 +307,13 @@ struct MF<float> {
     }
 };
 
+// Row stride (elements) of the node-lane predictor's per-wave LDS tile:
+// 16 nodes plus padding that keeps 16-byte alignment of each row.
+template <typename T>
+__host__ __device__ constexpr int nl_tile_stride() {
+    return sizeof(T) == 8 ? 18 : 20;
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -337,11 +338,11 @@ struct PredMArgs {
 };
 
 template <typename T, int P, int D>
-__global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
+__device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
     using Dm = Dims<P, D>;
     using M = MF<T>;
     using v4 = typename M::v4;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst, Mst = Dm::Mst, EPM = Dm::EPM;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst, Mst = Dm::Mst, Mbd = Dm::Mbd, EPM = Dm::EPM;
     constexpr int MB2 = Dm::MB2, KSU = Dm::KSU, NB = Dm::NB, KSH = Dm::KSH, MBS = Dm::MBS, MBD = Dm::MBD;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* fr = reinterpret_cast<T*>(smem_raw);
@@ -378,19 +379,21 @@ __global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
             const int k = 4 * ks + g;
             hb[ks] = (k < Ns) ? T(a.Uw[((long)q * Ns + k) * a.NS + slot]) : T(0);
         }
-        v4 c0[MB2];
+        // C0 is recomputed from U_w every iteration: KSU B registers instead
+        // of MB2 accumulators stay live across the iterations.
+        T hu[KSU];
 #pragma unroll
-        for (int mb = 0; mb < MB2; ++mb) {
-            c0[mb] = v4{0, 0, 0, 0};
-#pragma unroll
-            for (int ks = 0; ks < KSU; ++ks)
-                c0[mb] = M::mma(fr[(a.fo.A0 + mb * KSU + ks) * 64 + lane], hb[ks], c0[mb]);
-        }
+        for (int ks = 0; ks < KSU; ++ks) hu[ks] = hb[ks];
 
         for (int it = 0; it < P; ++it) {
             v4 acc[MB2];
 #pragma unroll
-            for (int mb = 0; mb < MB2; ++mb) acc[mb] = c0[mb];
+            for (int mb = 0; mb < MB2; ++mb) acc[mb] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KSU; ++ks)
+#pragma unroll
+                for (int mb = 0; mb < MB2; ++mb)
+                    acc[mb] = M::mma(fr[(a.fo.A0 + mb * KSU + ks) * 64 + lane], hu[ks], acc[mb]);
 #pragma unroll 1
             for (int nb = 0; nb < NB; ++nb) {
                 v4 h = v4{0, 0, 0, 0};
@@ -466,11 +469,13 @@ __global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
                                           vol[mbs]);
         }
         if (active) {
-            T* vo = a.Vol + slot * Dm::VES;
 #pragma unroll
             for (int mbs = 0; mbs < MBS; ++mbs)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) vo[((mbs * 4 + r) * 4 + g) * Q + q] = vol[mbs][r];
+                for (int r = 0; r < 4; ++r) {
+                    const int mode = 16 * mbs + acc_row<T>(g, r);
+                    if (mode < Ns) a.Vol[vol_idx<P, D>(mode, q, slot, a.NS)] = vol[mbs][r];
+                }
             if (a.Hw_dbg) {
 #pragma unroll
                 for (int ks = 0; ks < KSH; ++ks) {
@@ -480,7 +485,6 @@ __global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
             }
         }
         // face traces: Tr[ds][mb] = Vbd[ds] * H_w
-        T* to = a.Tr + slot * Dm::TES;
 #pragma unroll 1
         for (int ds = 0; ds < 2 * D; ++ds)
 #pragma unroll
@@ -491,10 +495,209 @@ __global__ void __launch_bounds__(512) k_predictor_mfma(PredMArgs<T> a) {
                     t = M::mma(fr[(a.fo.T + (ds * MBD + mbd) * KSH + ks) * 64 + lane], hb[ks], t);
                 if (active) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) to[(((ds * MBD + mbd) * 4 + r) * 4 + g) * Q + q] = t[r];
+                    for (int r = 0; r < 4; ++r) {
+                        const int fn = 16 * mbd + acc_row<T>(g, r);
+                        if (fn < Mbd) a.Tr[tr_idx<P, D>(ds, fn, q, slot, a.NS)] = t[r];
+                    }
                 }
             }
     }
+}
+
+// 2D: capped at 128 VGPRs so that two 512-thread workgroups (4 waves per
+// SIMD) share a CU; the 3D tiles need more registers than that cap allows
+// without spilling and run uncapped.
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_predictor_mfma(PredMArgs<T> a) {
+    predictor_mfma_body<T, P, D>(a);
+}
+template <typename T, int P, int D>
+__global__ void __launch_bounds__(512) k_predictor_mfma_wide(PredMArgs<T> a) {
+    predictor_mfma_body<T, P, D>(a);
+}
+template <typename T, int P, int D>
+inline const void* predictor_mfma_kernel() {
+    if constexpr (D == 2) return (const void*)k_predictor_mfma<T, P, D>;
+    else return (const void*)k_predictor_mfma_wide<T, P, D>;
+}
+
+// ---------------------------------------------------------------------------
+// Node-lane MFMA predictor (Q = 4).  A wavefront owns 4 elements; its 16
+// matrix columns are the (element, variable) pairs, column acc_row<T>(e, q),
+// so that the nodal product with swapped operands leaves the 4 variables of
+// one (element, node) in the 4 accumulator registers of one lane:
+//   C0    = A0 U_w                         rows modes, columns (e, q)
+//   H^T   = H_w^T V^T                      lane (e = g, node = c), register q
+//   g_d   = (Dt/dx_d) F_d(H)               once per (element, node)
+//   H_w'  = C0 + sum_d B_d g_d             g_d re-laid through a per-wave LDS
+//                                          tile as the B operand
+// Register r of lane group g of every modal accumulator holds logical row
+// 16*blk + 4r + g (the fragment rows of A0, B, S and T are permuted for that),
+// which is the operand layout of K step 4*blk + r of the next nodal product.
+// Final pass: Vol = sum_d S_d g_d and the face traces Tr = Vbd H_w.
+template <typename T, int P, int D>
+__device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
+    using Dm = Dims<P, D>;
+    using M = MF<T>;
+    using v4 = typename M::v4;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst, Mst = Dm::Mst, Mbd = Dm::Mbd;
+    constexpr int MB2 = Dm::MB2, KSU = Dm::KSU, NB = Dm::NB, KSH = Dm::KSH, MBS = Dm::MBS, MBD = Dm::MBD;
+    constexpr int TST = nl_tile_stride<T>();
+    constexpr bool FULL = Mst % 16 == 0;
+    static_assert(Q == 4, "the node-lane layout keeps the 4 variables of a node in one lane");
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    T* fr = reinterpret_cast<T*>(smem_raw);
+    const int nfr = a.fo.total * 64;
+    for (int i = threadIdx.x; i < nfr; i += blockDim.x) fr[i] = a.frags[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    T* tile = fr + nfr + wave * (D * 16 * TST);
+    const int g = lane >> 4, c = lane & 15;
+    // column c of the modal data holds variable cq of element ce
+    const int ce = sizeof(T) == 8 ? (c & 3) : (c >> 2);
+    const int cq = sizeof(T) == 8 ? (c >> 2) : (c & 3);
+    const T gm1 = T(a.gamma - 1.0);
+    const long per_batch = (long)nwave * 4;
+    const long nbatch = (a.nelem + per_batch - 1) / per_batch;
+
+    for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
+        const long e0 = batch * per_batch + (long)wave * 4;
+        const bool active = e0 + ce < a.nelem;
+        const long slot = a.slot_begin + (active ? e0 + ce : a.nelem - 1);
+        const long slot_f = a.slot_begin + (e0 + g < a.nelem ? e0 + g : a.nelem - 1);
+        const int lev = a.slot_lev[slot_f];
+        const double Dt = a.ts->Dt;
+        T cd[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) cd[d] = T(Dt / a.geo.dx[lev][d]);
+
+        T hu[KSU];
+#pragma unroll
+        for (int ks = 0; ks < KSU; ++ks) {
+            const int k = 4 * ks + g;
+            hu[ks] = (k < Ns) ? T(a.Uw[((long)cq * Ns + k) * a.NS + slot]) : T(0);
+        }
+        v4 c0[MB2];
+#pragma unroll
+        for (int mb = 0; mb < MB2; ++mb) {
+            c0[mb] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KSU; ++ks)
+                c0[mb] = M::mma(fr[(a.fo.A0 + mb * KSU + ks) * 64 + lane], hu[ks], c0[mb]);
+        }
+        // initial guess H_w[mode] = U_w[mode] for mode < Np_s (AmrDG.tpp:522-533)
+        T hb[KSH];
+#pragma unroll
+        for (int ks = 0; ks < KSH; ++ks) hb[ks] = ks < KSU ? hu[ks] : T(0);
+
+        // nodal state of node block nb -> (Dt/dx_d) F_d in the LDS tile
+        // [d][column][node]; returns the B operands of the 4 K steps
+        auto nodal_flux = [&](int nb, T (&gv)[D][4]) {
+            v4 h = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) h = M::mma(hb[ks], fr[(a.fo.V + nb * KSH + ks) * 64 + lane], h);
+            T s[Q] = {h[0], h[1], h[2], h[3]};
+            const bool ok = FULL || (16 * nb + c < Mst);
+            if (!ok) s[0] = T(1);
+            const T rinv = T(1) / s[0];
+            const T prs = pressure<D>(s, rinv, gm1);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const T ud = s[d + 1] * rinv;
+                T f[Q];
+                f[0] = s[d + 1];
+#pragma unroll
+                for (int b = 0; b < D; ++b) f[b + 1] = ud * s[b + 1] + ((b == d) ? prs : T(0));
+                f[D + 1] = (s[D + 1] + prs) * ud;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) tile[(d * 16 + acc_row<T>(g, q)) * TST + c] = ok ? cd[d] * f[q] : T(0);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) gv[d][s4] = tile[(d * 16 + c) * TST + 4 * g + s4];
+            wave_lds_sync();
+        };
+
+        for (int it = 0; it < P; ++it) {
+            v4 acc[MB2];
+#pragma unroll
+            for (int mb = 0; mb < MB2; ++mb) acc[mb] = c0[mb];
+#pragma unroll 1
+            for (int nb = 0; nb < NB; ++nb) {
+                T gv[D][4];
+                nodal_flux(nb, gv);
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int mb = 0; mb < MB2; ++mb)
+                            acc[mb] = M::mma(fr[(a.fo.B + ((mb * D + d) * NB + nb) * 4 + s4) * 64 + lane], gv[d][s4],
+                                             acc[mb]);
+            }
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) hb[ks] = acc[ks >> 2][ks & 3];
+        }
+
+        // final evaluation: corrector volume term
+        v4 vol[MBS];
+#pragma unroll
+        for (int mbs = 0; mbs < MBS; ++mbs) vol[mbs] = v4{0, 0, 0, 0};
+#pragma unroll 1
+        for (int nb = 0; nb < NB; ++nb) {
+            T gv[D][4];
+            nodal_flux(nb, gv);
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+#pragma unroll
+                    for (int mbs = 0; mbs < MBS; ++mbs)
+                        vol[mbs] = M::mma(fr[(a.fo.S + ((mbs * D + d) * NB + nb) * 4 + s4) * 64 + lane], gv[d][s4],
+                                          vol[mbs]);
+        }
+        if (active) {
+#pragma unroll
+            for (int mbs = 0; mbs < MBS; ++mbs)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int mode = 16 * mbs + 4 * r + g;
+                    if (mode < Ns) a.Vol[vol_idx<P, D>(mode, cq, slot, a.NS)] = vol[mbs][r];
+                }
+            if (a.Hw_dbg) {
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) {
+                    const int mode = 4 * ks + g;
+                    if (mode < Nst) a.Hw_dbg[((long)cq * Nst + mode) * a.NS + slot] = double(hb[ks]);
+                }
+            }
+        }
+        // face traces: Tr[ds][node] = Vbd[ds] * H_w
+#pragma unroll 1
+        for (int ds = 0; ds < 2 * D; ++ds)
+#pragma unroll
+            for (int mbd = 0; mbd < MBD; ++mbd) {
+                v4 t = v4{0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks)
+                    t = M::mma(fr[(a.fo.T + (ds * MBD + mbd) * KSH + ks) * 64 + lane], hb[ks], t);
+                if (active) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int fn = 16 * mbd + 4 * r + g;
+                        if (fn < Mbd) a.Tr[tr_idx<P, D>(ds, fn, cq, slot, a.NS)] = t[r];
+                    }
+                }
+            }
+    }
+}
+
+template <typename T, int P, int D, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_predictor_nl(PredMArgs<T> a) {
+    if constexpr (Dims<P, D>::Q == 4) predictor_nl_body<T, P, D>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,15 +718,16 @@ __device__ __forceinline__ T face_moment(const double* __restrict__ Mrow, const 
 // ---------------------------------------------------------------------------
 // Face flux: LLF at every face quadrature node (AmrDG_NumericalFlux.tpp:4-41)
 // from the traces of the left (side +) and right (side -) cells, computed
-// once per face.  A face owns TPF = 64*ceil(Mbd*Q/64) lanes, lane = mb*Q + q:
-// the trace reads of a face are contiguous, the Q variables of a node are
-// gathered from neighbouring lanes, and Fnum is face-major [face][q][mb].
+// once per face.  One thread per (direction, face node, face), face index
+// fastest: the trace reads and the flux writes of a wavefront are
+// consecutive slots / faces of one SoA row.
 template <typename T>
 struct FaceArgs {
     const T* __restrict__ Tr;
     T* __restrict__ Fnum[3];
     const int* __restrict__ face_lr[3];
     long NF[3];
+    long NS;
     double gamma;
 };
 
@@ -531,29 +735,23 @@ template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_faceflux(FaceArgs<T> a) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Mbd = Dm::Mbd;
-    constexpr int TPF = 64 * cdiv(Mbd * Q, 64);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long fg = t / TPF;
-    const int w = (int)(t - fg * TPF);
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int d = 0;
-    long base = 0, tot = 0;
 #pragma unroll
-    for (int dd = 0; dd < D; ++dd) {
-        if (fg >= tot + a.NF[dd]) base = tot + a.NF[dd], d = dd + 1;
-        tot += a.NF[dd];
-    }
-    if (fg >= tot) return;   // whole face groups exit together
-    const long f = fg - base;
-    const bool ok = w < Mbd * Q;
-    const int mb = ok ? w / Q : 0, q = ok ? w - (w / Q) * Q : 0;
+    for (int dd = 0; dd < D - 1; ++dd)
+        if (d == dd && t >= a.NF[dd] * Mbd) { t -= a.NF[dd] * Mbd; d = dd + 1; }
+    const long nf = a.NF[d];
+    if (t >= nf * Mbd) return;
+    const int mb = (int)(t / nf);
+    const long f = t - (long)mb * nf;
     const long sl = a.face_lr[d][2 * f], sr = a.face_lr[d][2 * f + 1];
-    const T vL = a.Tr[sl * Dm::TES + tr_off<T, P, D>(2 * d + 1, mb, q)];
-    const T vR = a.Tr[sr * Dm::TES + tr_off<T, P, D>(2 * d + 0, mb, q)];
+    const long NS = a.NS;
     T uL[Q], uR[Q];
-    const int lane = threadIdx.x & 63;
-    gather_state<Q>(vL, lane - q, uL);
-    gather_state<Q>(vR, lane - q, uR);
-    if (!ok) return;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uL[q] = a.Tr[tr_idx<P, D>(2 * d + 1, mb, q, sl, NS)];
+        uR[q] = a.Tr[tr_idx<P, D>(2 * d + 0, mb, q, sr, NS)];
+    }
     const T gm = T(a.gamma), gm1 = T(a.gamma - 1.0);
     const T riL = T(1) / uL[0], riR = T(1) / uR[0];
     const T pL = pressure<D>(uL, riL, gm1), pR = pressure<D>(uR, riR, gm1);
@@ -563,9 +761,13 @@ __global__ void __launch_bounds__(256) k_faceflux(FaceArgs<T> a) {
     const T dfL = fmax(fmax(fabs(unL - cL), fabs(unL)), fabs(unL + cL));
     const T dfR = fmax(fmax(fabs(unR - cR), fabs(unR)), fabs(unR + cR));
     const T C = fmax(fabs(dfL), fabs(dfR));
-    const T fL = flux_q<D>(d, q, uL, vL, riL, pL);
-    const T fR = flux_q<D>(d, q, uR, vR, riR, pR);
-    a.Fnum[d][(f * Q + q) * Mbd + mb] = T(0.5) * ((fL + fR) - C * (vR - vL));
+    T* __restrict__ out = a.Fnum[d] + (long)mb * nf + f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const T fL = flux_q<D>(d, q, uL, uL[q], riL, pL);
+        const T fR = flux_q<D>(d, q, uR, uR[q], riR, pR);
+        out[(long)q * Mbd * nf] = T(0.5) * ((fL + fR) - C * (uR[q] - uL[q]));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -606,17 +808,18 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     double r[Ns];
 #pragma unroll
     for (int k = 0; k < Ns; ++k)
-        r[k] = Mkc[k] * a.Uw[((long)q * Ns + k) * NS + slot] + double(a.Vol[slot * Dm::VES + vol_off<T, P, D>(k, q)]);
+        r[k] = Mkc[k] * a.Uw[((long)q * Ns + k) * NS + slot] + double(a.Vol[vol_idx<P, D>(k, q, slot, NS)]);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const double cdd = Dt / a.geo.dx[lev][d];
         const long flo = a.cell_face[d][0][i], fhi = a.cell_face[d][1][i];
         const double* __restrict__ Mbp = a.tab + a.to.Mkbdp + (long)d * Ns * Mbd;
         T Fh[Mbd], Fl[Mbd];
-        const T* __restrict__ ph = a.Fnum[d] + (fhi * Q + q) * Mbd;
-        const T* __restrict__ pl = a.Fnum[d] + (flo * Q + q) * Mbd;
+        const long nf = a.NF[d];
+        const T* __restrict__ ph = a.Fnum[d] + (long)q * Mbd * nf + fhi;
+        const T* __restrict__ pl = a.Fnum[d] + (long)q * Mbd * nf + flo;
 #pragma unroll
-        for (int mb = 0; mb < Mbd; ++mb) { Fh[mb] = ph[mb]; Fl[mb] = pl[mb]; }
+        for (int mb = 0; mb < Mbd; ++mb) { Fh[mb] = ph[mb * nf]; Fl[mb] = pl[mb * nf]; }
 #pragma unroll
         for (int k = 0; k < Ns; ++k) {
             const T sh = face_moment<T, Mbd>(Mbp + k * Mbd, Fh);
@@ -636,14 +839,14 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
 // (AmrDG.tpp:604-647).
 template <typename T, int P, int D>
 __device__ __forceinline__ double register_moment(const double* __restrict__ tab, const TabOff& to,
-                                                  const T* __restrict__ Fnum_d, int d, int kind, int sel, long f,
-                                                  int q, int n, double jac) {
+                                                  const T* __restrict__ Fnum_d, long nf, int d, int kind, int sel,
+                                                  long f, int q, int n, double jac) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
     T F[Mbd];
-    const T* __restrict__ pf = Fnum_d + (f * Q + q) * Mbd;
+    const T* __restrict__ pf = Fnum_d + (long)q * Mbd * nf + f;
 #pragma unroll
-    for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb];
+    for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb * nf];
     const double* __restrict__ Mbp = tab + to.Mkbdp + (long)d * Ns * Mbd;
     if (kind == 0 || n == 0) {
         const double sv = double(face_moment<T, Mbd>(Mbp + n * Mbd, F));
@@ -685,7 +888,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     const int d = a.mom_d[j], lev = a.mom_lev[j];
     const double jac = (a.ts->Dt / 2.0) * (a.geo.area[lev][d] / (double)(1 << (D - 1)));
     a.Mom[((long)q * Ns + n) * a.NM + j] =
-        register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
+        register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], a.NF[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
 }
 
 // Reflux: u_c += (2^D / vol_c) * Minv * sum(coef * moment) on corrected cells
@@ -898,7 +1101,7 @@ __device__ __forceinline__ void reflux_item(const EpiArgs<T>& a, long t) {
         const int d = a.mom_d[j], ml = a.mom_lev[j];
         const double jac = (Dt / 2.0) * (a.geo.area[ml][d] / (double)(1 << (D - 1)));
         corr += a.re_coef[st + e] *
-                register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
+                register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], a.NF[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
     }
     a.Uw[((long)q * Ns + n) * a.NS + slot] += inv_jac * (a.tab[a.to.Minv + n] * corr);
 }

@@ -340,15 +340,25 @@ ElementTables build_element_tables(int P, int D) {
     return t;
 }
 
-Fragments build_fragments(const ElementTables& t, bool fp64_rows) {
+Fragments build_fragments(const ElementTables& t, bool fp64_rows, bool node_lane) {
     Fragments f;
     const int D = t.D, Ns = t.Np_s, Nst = t.Np_st, Mst = t.qMp_st, Mbd = t.qMp_st_bd;
     auto cdiv = [](int a, int b) { return (a + b - 1) / b; };
     f.MB2 = cdiv(Nst, 16); f.KSU = cdiv(Ns, 4); f.NB = cdiv(Mst, 16);
     f.KSH = cdiv(Nst, 4); f.MBS = cdiv(Ns, 16); f.MBD = cdiv(Mbd, 16);
     std::vector<double>& v = f.data;
-    // accumulator row held by register r of lane group k (see acc_row)
+    // accumulator row held by register r of lane group k
     auto acc_row = [&](int k, int r) { return fp64_rows ? 4 * r + k : 4 * k + r; };
+    // Output-row permutation.  Quad layout: identity.  Node-lane layout: the
+    // physical row i of a fragment carries logical row 4r + g where (g, r) is
+    // the lane group and register the accumulator puts row i in, so register
+    // r of lane group g holds logical row 4r + g in both precisions.
+    auto lrow = [&](int i) {
+        if (!node_lane || fp64_rows) return i;
+        return 4 * (i & 3) + (i >> 2);
+    };
+    // node of K step (nb, r), lane group k, in the products with nodal data
+    auto knode = [&](int nb, int r, int k) { return 16 * nb + (node_lane ? 4 * k + r : acc_row(k, r)); };
     auto frag = [&](auto&& value) {  // value(row_in_block, kidx) -> double
         for (int l = 0; l < 64; ++l) v.push_back(value(l % 16, l / 16));
     };
@@ -357,10 +367,11 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows) {
     for (int mb = 0; mb < f.MB2; ++mb)
         for (int ks = 0; ks < f.KSU; ++ks)
             frag([&](int i, int k) {
-                int row = 16 * mb + i, col = 4 * ks + k;
+                int row = 16 * mb + lrow(i), col = 4 * ks + k;
                 return (row < Nst && col < Ns) ? t.A0[(size_t)row * Ns + col] : 0.0;
             });
-    // H = V * H_w: rows nodes (blocks nb), K = H_w modes 4ks+k
+    // H = V * H_w: rows nodes (blocks nb), K = H_w modes 4ks+k.  The node-lane
+    // kernel uses it as the B operand of the swapped product (same registers).
     f.V = (int)(v.size() / 64);
     for (int nb = 0; nb < f.NB; ++nb)
         for (int ks = 0; ks < f.KSH; ++ks)
@@ -368,14 +379,14 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows) {
                 int node = 16 * nb + i, mode = 4 * ks + k;
                 return (node < Mst && mode < Nst) ? t.V[(size_t)node * Nst + mode] : 0.0;
             });
-    // H_w update: rows modes (blocks mb), K step (nb, r) takes node 16nb+acc_row(k,r)
+    // H_w update: rows modes (blocks mb), K step (nb, r) takes knode(nb, r, k)
     f.B = (int)(v.size() / 64);
     for (int mb = 0; mb < f.MB2; ++mb)
         for (int d = 0; d < D; ++d)
             for (int nb = 0; nb < f.NB; ++nb)
                 for (int r = 0; r < 4; ++r)
                     frag([&](int i, int k) {
-                        int mode = 16 * mb + i, node = 16 * nb + acc_row(k, r);
+                        int mode = 16 * mb + lrow(i), node = knode(nb, r, k);
                         return (mode < Nst && node < Mst) ? t.B[((size_t)d * Nst + mode) * Mst + node] : 0.0;
                     });
     // corrector volume term: rows spatial modes, same K order as B
@@ -385,7 +396,7 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows) {
             for (int nb = 0; nb < f.NB; ++nb)
                 for (int r = 0; r < 4; ++r)
                     frag([&](int i, int k) {
-                        int row = 16 * mb + i, node = 16 * nb + acc_row(k, r);
+                        int row = 16 * mb + lrow(i), node = knode(nb, r, k);
                         return (row < Ns && node < Mst) ? t.Sk_corr[((size_t)d * Ns + row) * Mst + node] : 0.0;
                     });
     // face traces: rows face nodes (blocks mbd) per (d, side), K = H_w modes
@@ -394,7 +405,7 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows) {
         for (int mbd = 0; mbd < f.MBD; ++mbd)
             for (int ks = 0; ks < f.KSH; ++ks)
                 frag([&](int i, int k) {
-                    int row = 16 * mbd + i, mode = 4 * ks + k;
+                    int row = 16 * mbd + lrow(i), mode = 4 * ks + k;
                     return (row < Mbd && mode < Nst) ? t.Vbd[((size_t)ds * Mbd + row) * Nst + mode] : 0.0;
                 });
     f.total = (int)(v.size() / 64);

@@ -52,8 +52,11 @@ struct Fragments {
     std::vector<double> data;  // [total][64]
 };
 // fp64_rows selects the accumulator row order of the FP64 MFMA (row = 4r+g)
-// instead of the FP32 one (row = 4g+r) for the K order of B and S.
-Fragments build_fragments(const ElementTables& t, bool fp64_rows);
+// instead of the FP32 one (row = 4g+r).  node_lane selects the fragment set
+// of the node-lane predictor (permuted output rows, K step (nb, r) over nodes
+// 16nb + 4k + r) instead of the quad-gather one (K step over the nodes the
+// accumulator of a nodal product puts in register r).
+Fragments build_fragments(const ElementTables& t, bool fp64_rows, bool node_lane);
 
 // Named access for tests (returns empty vector for unknown names).
 const std::vector<double>* table_by_name(const ElementTables& t, const std::string& name);

@@ -25,7 +25,7 @@ EXPORTS = [
     "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
     "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_set_time",
     "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
-    "adg_project_ic", "adg_set_predictor_kernel", "adg_last_error", "adg_destroy",
+    "adg_project_ic", "adg_set_predictor_kernel", "adg_set_predictor_waves", "adg_last_error", "adg_destroy",
 ]
 
 
@@ -71,6 +71,7 @@ def lib():
     L.adg_get_table.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_int]
     L.adg_project_ic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
     L.adg_set_predictor_kernel.argtypes = [C.c_void_p, C.c_int]
+    L.adg_set_predictor_waves.argtypes = [C.c_void_p, C.c_int]
     L.adg_last_error.argtypes = [C.c_void_p]
     L.adg_last_error.restype = C.c_char_p
     L.adg_destroy.argtypes = [C.c_void_p]
@@ -144,8 +145,12 @@ class Context:
         return out
 
     def set_predictor_kernel(self, kind):
-        """0 automatic, 1 VALU (FP64), 2 MFMA."""
+        """0 automatic, 1 VALU (FP64), 2 MFMA quad-gather, 3 MFMA node-lane (2D)."""
         self._chk(self.L.adg_set_predictor_kernel(self.h, kind), "adg_set_predictor_kernel")
+
+    def set_predictor_waves(self, waves):
+        """Wavefronts per workgroup of the node-lane predictor (8 or 16)."""
+        self._chk(self.L.adg_set_predictor_waves(self.h, waves), "adg_set_predictor_waves")
 
     def keep_Hw(self, on=True):
         self._chk(self.L.adg_keep_Hw(self.h, 1 if on else 0), "adg_keep_Hw")

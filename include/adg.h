@@ -105,10 +105,16 @@ int adg_set_dt_value(adg_ctx* ctx, double Dt);
  * mode 0: exactly P Picard iterations (the reference); other modes are
  * reserved.  n_fallback (nullable) receives 0 in mode 0. */
 int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
-/* Predictor kernel variant: 0 = automatic (MFMA where the element matrices
- * fit in LDS, else VALU), 1 = VALU (FP64 only), 2 = MFMA.  No counterpart in
- * the reference: a tuning knob of this implementation. */
+/* Predictor kernel variant: 0 = automatic, 1 = VALU (FP64 only), 2 = MFMA
+ * quad-gather (every Q), 3 = MFMA node-lane (Q = 4, i.e. 2D).  Automatic
+ * picks node-lane where it applies, else quad-gather, else VALU; FP64 at
+ * p = 1 uses VALU.  No counterpart in the reference: a tuning knob of this
+ * implementation. */
 int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
+/* Wavefronts per workgroup of the node-lane predictor: 8 (default) or 16.
+ * No counterpart in the reference: a launch-shape knob of this
+ * implementation, set by the autotuner. */
+int adg_set_predictor_waves(adg_ctx* ctx, int waves_per_block);
 
 /* Debug readback of the last predictor H_w of a level, [q][Np_st][slot]
  * (only after adg_keep_Hw(ctx,1)). */

@@ -38,7 +38,7 @@ def setup(adg, orc, D, P, levels, dom, plo, phi, ic_case, params, gamma=1.4, kin
     return T, H, Uw, ctx
 
 
-KINDS = [1, 2]  # VALU, MFMA predictor
+KINDS = [1, 2, 3]  # VALU, MFMA quad-gather, MFMA node-lane predictor
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -160,16 +160,19 @@ def test_gpu_ic_matches_oracle(adg, orc):
     assert rel_err(ctx.download_Uw(0), ref) < 1e-13
 
 
+@pytest.mark.parametrize("kind,waves", [(2, 8), (3, 8), (3, 16)])
 @pytest.mark.parametrize("P", [2, 3])
-def test_fp32_form_kh(adg, orc, P):
+def test_fp32_form_kh(adg, orc, P, kind, waves):
     """Precision form: FP32 MFMA contractions and FP32 face fluxes, FP64 U_w
     accumulation, reflux and reductions.  Stated error vs the FP64 oracle
     after 10 steps on a 3-level KH hierarchy: per variable, max|dU| over the
     largest |U| of all variables (the FP32 round-off of 10 increments) below
     1e-6; composite mass and energy conserved to 1e-13."""
-    print(f"case FP32 form KH P={P} 16^2 3 levels", flush=True)
+    print(f"case FP32 form KH P={P} 16^2 3 levels kind={kind} waves={waves}", flush=True)
     levels = kh_bands(16, 3, 16)
-    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7], precision=1)
+    T, H, Uw, ctx = setup(adg, orc, 2, P, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7], precision=1,
+                          kind=kind)
+    ctx.set_predictor_waves(waves)
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.set_dt(0.5, 1.0, 0.0)
     m0 = ctx.totals()
