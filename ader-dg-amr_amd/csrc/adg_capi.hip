@@ -18,6 +18,7 @@
 #include "adg.h"
 #include "adg_hier.hpp"
 #include "adg_kernels.hpp"
+#include "adg_mlp.hpp"
 #include "adg_tables.hpp"
 
 namespace adg {
@@ -71,6 +72,30 @@ struct adg_ctx {
     int* uncovered = nullptr;
     adg::TimeState* ts = nullptr;
     bool keep_hw = false;
+    int iters_override = 0;    // > 0: Picard iterations of the next plain predictor launches (collection)
+
+    // learned warm start
+    struct Mlp {
+        bool init = false;
+        int nin = 0, nin_p = 0, nout = 0, nout_p = 0, nh = adg::kMlpHidden;
+        long npar = 0, oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0;
+        float *par = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr, *norm = nullptr;
+        uint2* frag = nullptr;
+        int f1 = 0, f2 = 0, f3 = 0, nfrag = 0;
+        bool frag_ok = false;
+        float *X = nullptr, *Y = nullptr;   // dataset rows (raw features / targets)
+        long ndata = 0, cap = 0;
+        float* work = nullptr;              // minibatch activations and gradients
+        long work_rows = 0;
+        double* dacc = nullptr;             // loss accumulator and column statistics
+        long adam_t = 0;
+    } mlp;
+    float* warm = nullptr;                    // learned H_w increments [Q*Np_st][slot]
+    unsigned long long* lstats = nullptr;     // [elements, fallbacks] of learned predictor launches
+    bool learned = false;
+    float learned_tol = 1e-2f;
+    int warm_per_cu = 0;
+    int learned_iters = 1;
     std::string err;
     std::vector<void*> level_allocs;
 };
@@ -163,6 +188,7 @@ void free_levels(adg_ctx* c) {
     c->level_allocs.clear();
     c->levels_set = false;
     c->Hw_dbg = nullptr;
+    c->warm = nullptr;
 }
 
 bool supported(int P, int D) { return (D == 2 || D == 3) && P >= 1 && P <= 3; }
@@ -276,6 +302,11 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
     void* tsp = nullptr;
     if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->ts = static_cast<adg::TimeState*>(tsp);
+    void* lsp = nullptr;
+    if (hipMalloc(&lsp, 2 * sizeof(unsigned long long)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
+    c->lstats = static_cast<unsigned long long*>(lsp);
+    const unsigned long long z2[2] = {0ull, 0ull};
+    if (hipMemcpy(c->lstats, z2, sizeof(z2), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
     adg::TimeState t0{0.0, 0.0, 0x7ff0000000000000ull, 0u, 0u};
     if (hipMemcpy(c->ts, &t0, sizeof(t0), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
     *out = c;
@@ -369,6 +400,7 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if ((rc = dev_alloc(c, &c->Mom, (size_t)Q * t.Np_s * H.mom_face.size())) != ADG_OK) return rc;
     if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
+    if ((rc = dev_alloc_bytes(c, (void**)&c->warm, (size_t)Q * t.Np_st * NS * sizeof(float))) != ADG_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->levels_set = true;
     return prepare_kernels(c);
@@ -449,7 +481,7 @@ int adg_download_Uw(adg_ctx* c, int lev, double* host) {
 int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
     if (!c) return ADG_E_ARG;
     if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 3)
+    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 4)
         return fail(c, ADG_E_ARG, "bad level, case or params");
     if (ic_case <= 1 && c->D != 2) return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
     std::vector<double> host;
@@ -557,7 +589,7 @@ int adg_set_dt_value(adg_ctx* c, double Dt) {
 
 extern "C++" {
 template <typename T, int P, int D>
-static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant) {
+static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool learned) {
     const int v = variant == PV_NL ? 1 : 0;
     adg::PredMArgs<T> a{};
     a.frags = (const T*)c->d_frag[v];
@@ -573,6 +605,11 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant) {
     a.slot_begin = begin;
     a.nelem = n;
     a.gamma = c->gamma;
+    a.iters = c->iters_override > 0 ? c->iters_override : P;
+    a.warm = learned ? c->warm : nullptr;
+    a.tol = c->learned_tol;
+    a.learned_iters = c->learned_iters;
+    a.stats = c->lstats;
     const int wpb = variant == PV_NL ? c->nl_wpb : 8;
     const size_t lds = mfma_lds_bytes(c, variant, wpb);
     const void* kern = variant == PV_NL ? (wpb == 16 ? (const void*)adg::k_predictor_nl<T, P, D, 16>
@@ -613,30 +650,106 @@ int adg_set_predictor_waves(adg_ctx* c, int wpb) {
     return ADG_OK;
 }
 
+extern "C++" {
+// Warm-start increments of the slots [begin, begin + n) from the packed MLP.
+template <int P, int D>
+static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
+    constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
+    constexpr int NINP = 16 * ((Q * Ns + 1 + D + 15) / 16), NOUTP = 16 * ((Q * Nst + 15) / 16);
+    auto& m = c->mlp;
+    if (m.nin_p != NINP || m.nout_p != NOUTP) return fail(c, ADG_E_STATE, "MLP widths do not match the tables");
+    adg::WarmArgs a{};
+    a.frag = m.frag;
+    a.f1 = m.f1;
+    a.f2 = m.f2;
+    a.f3 = m.f3;
+    a.nfrag = m.nfrag;
+    a.b1 = m.par + m.ob1;
+    a.b2 = m.par + m.ob2;
+    a.b3 = m.par + m.ob3;
+    a.norm = m.norm;
+    a.Uw = c->Uw;
+    a.slot_lev = c->slot_lev;
+    a.ts = c->ts;
+    a.geo = c->geo;
+    a.NS = c->H.NS;
+    a.slot_begin = begin;
+    a.nelem = n;
+    a.gamma = c->gamma;
+    a.warm = c->warm;
+    const size_t lds = (size_t)m.nfrag * 64 * sizeof(uint2);
+    const void* kern = (const void*)adg::k_mlp_warm<P, D, NINP, NOUTP>;
+    if (c->warm_per_cu <= 0) return fail(c, ADG_E_STATE, "warm-start kernel not configured");
+    const long nbatch = (n + 127) / 128;
+    const long grid = std::min<long>(nbatch, (long)c->warm_per_cu * c->ncu);
+    void* kargs[] = {&a};
+    HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(512), kargs, lds, c->stream));
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+}  // extern "C++"
+
+static int mlp_pack(adg_ctx* c);
+
+// Raises the LDS limit of the warm-start kernel (outside any capture).
+extern "C++" {
+template <int P, int D>
+static int configure_mlp_warm(adg_ctx* c) {
+    constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
+    constexpr int NINP = 16 * ((Q * Ns + 1 + D + 15) / 16), NOUTP = 16 * ((Q * Nst + 15) / 16);
+    const void* kern = (const void*)adg::k_mlp_warm<P, D, NINP, NOUTP>;
+    const size_t lds = (size_t)c->mlp.nfrag * 64 * sizeof(uint2);
+    if (lds > kLdsMax) return fail(c, ADG_E_UNSUP, "network fragments exceed LDS");
+    HIPCHK(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int nb = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 512, lds));
+    c->warm_per_cu = nb > 0 ? nb : 1;
+    return ADG_OK;
+}
+}  // extern "C++"
+
 int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    if (mode != 0) return fail(c, ADG_E_UNSUP, "only mode 0 (P Picard iterations) is available");
+    if (mode != 0 && mode != 1) return fail(c, ADG_E_ARG, "mode must be 0 (plain) or 1 (learned warm start)");
     if (lev < -1 || lev >= c->H.nlev) return fail(c, ADG_E_ARG, "bad level");
     if (n_fallback) *n_fallback = 0;
     long begin = 0, n = c->H.NS;
     if (lev >= 0) { begin = c->H.lev[lev].slot_off; n = c->H.lev[lev].nslot; }
     if (n == 0) return ADG_OK;
     const int variant = pred_variant(c);
+    const bool learned = mode == 1;
+    int rc = ADG_OK;
+    if (learned) {
+        if (variant != PV_NL) return fail(c, ADG_E_UNSUP, "the learned warm start runs in the node-lane predictor");
+        if (!c->mlp.init) return fail(c, ADG_E_STATE, "no network: adg_mlp_init / adg_mlp_set_params first");
+        if ((rc = mlp_pack(c)) != ADG_OK) return rc;
+        if (n_fallback) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
+#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, begin, n)
+        ADG_DISPATCH(c, L_MW);
+#undef L_MW
+        if (rc != ADG_OK) return rc;
+    }
     if (variant != PV_VALU) {
-        int rc = ADG_OK;
         if (c->precision == ADG_FP64) {
-#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n, variant)
+#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n, variant, learned)
             ADG_DISPATCH(c, L_PM);
 #undef L_PM
         } else {
-#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n, variant)
+#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n, variant, learned)
             ADG_DISPATCH(c, L_PM);
 #undef L_PM
+        }
+        if (rc == ADG_OK && learned && n_fallback) {
+            unsigned long long st[2];
+            HIPCHK(c, hipMemcpyAsync(st, c->lstats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            *n_fallback = (int)st[1];
         }
         return rc;
     }
     adg::PredArgs a{c->d_tab, c->to, c->Uw, (double*)c->Vol, (double*)c->Tr, c->keep_hw ? c->Hw_dbg : nullptr,
-                    c->slot_lev, c->ts, c->geo, c->H.NS, begin, n, c->gamma};
+                    c->slot_lev, c->ts, c->geo, c->H.NS, begin, n, c->gamma,
+                    c->iters_override > 0 ? c->iters_override : c->P};
 #define L_PR(PP, DD)                                                                              \
     {                                                                                             \
         constexpr int epw = adg::Dims<PP, DD>::EPW;                                               \
@@ -648,6 +761,431 @@ int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     HIPCHK(c, hipGetLastError());
     return ADG_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Learned warm start: network state, dataset, training.
+namespace {
+uint64_t mix64_h(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+long mlp_norm_size(const adg_ctx* c) { return 2L * c->mlp.nin_p + 2L * c->mlp.nout_p; }
+}  // namespace
+
+static int mlp_alloc(adg_ctx* c) {
+    auto& m = c->mlp;
+    if (m.par) return ADG_OK;
+    m.nin = c->Q * c->tab.Np_s + 1 + c->D;
+    m.nin_p = 16 * ((m.nin + 15) / 16);
+    m.nout = c->Q * c->tab.Np_st;
+    m.nout_p = 16 * ((m.nout + 15) / 16);
+    const long nh = m.nh;
+    m.oW1 = 0;
+    m.ob1 = m.oW1 + nh * m.nin_p;
+    m.oW2 = m.ob1 + nh;
+    m.ob2 = m.oW2 + nh * nh;
+    m.oW3 = m.ob2 + nh;
+    m.ob3 = m.oW3 + (long)m.nout_p * nh;
+    m.npar = m.ob3 + m.nout_p;
+    m.f1 = 0;
+    m.f2 = m.f1 + (int)(nh / 16) * (m.nin_p / 16);
+    m.f3 = m.f2 + (int)(nh / 16) * (int)(nh / 16);
+    m.nfrag = m.f3 + (m.nout_p / 16) * (int)(nh / 16);
+    for (float** p : {&m.par, &m.grad, &m.m, &m.v})
+        HIPCHK(c, hipMalloc((void**)p, (size_t)m.npar * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&m.norm, (size_t)mlp_norm_size(c) * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&m.frag, (size_t)m.nfrag * 64 * sizeof(uint2)));
+    HIPCHK(c, hipMalloc((void**)&m.dacc, (size_t)(1 + 2 * (m.nin_p + m.nout_p)) * sizeof(double)));
+    return ADG_OK;
+}
+
+static int mlp_fill(adg_ctx* c, float* p, long n, float v) {
+    if (n <= 0) return ADG_OK;
+    hipLaunchKernelGGL(adg::k_fill, dim3(nblocks(n)), dim3(256), 0, c->stream, p, n, v);
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+static int mlp_pack(adg_ctx* c) {
+    auto& m = c->mlp;
+    if (m.frag_ok) return ADG_OK;
+    const int nh = m.nh;
+    hipLaunchKernelGGL(adg::k_mlp_pack, dim3((nh / 16) * (m.nin_p / 16)), dim3(64), 0, c->stream, m.par + m.oW1, nh,
+                       m.nin_p, m.frag + (long)m.f1 * 64);
+    hipLaunchKernelGGL(adg::k_mlp_pack, dim3((nh / 16) * (nh / 16)), dim3(64), 0, c->stream, m.par + m.oW2, nh, nh,
+                       m.frag + (long)m.f2 * 64);
+    hipLaunchKernelGGL(adg::k_mlp_pack, dim3((m.nout_p / 16) * (nh / 16)), dim3(64), 0, c->stream, m.par + m.oW3,
+                       m.nout_p, nh, m.frag + (long)m.f3 * 64);
+    HIPCHK(c, hipGetLastError());
+    int rc = ADG_OK;
+#define L_CW(PP, DD) rc = configure_mlp_warm<PP, DD>(c)
+    ADG_DISPATCH(c, L_CW);
+#undef L_CW
+    if (rc != ADG_OK) return rc;
+    m.frag_ok = true;
+    return ADG_OK;
+}
+
+static int mlp_gemm(adg_ctx* c, const float* A, long lda, int tA, const float* B, long ldb, int tB, float* C, long ldc,
+                    int M, int N, int K, int epi, const float* bias = nullptr, float* C2 = nullptr,
+                    const float* aux = nullptr) {
+    adg::GemmArgs g{A, lda, tA, B, ldb, tB, C, ldc, M, N, K, 1, epi, bias, C2, aux};
+    if (epi == 2) g.ksplit = std::max(1, std::min(K / 16, 64));
+    const long waves = (long)(M / 16) * (N / 16) * g.ksplit;
+    hipLaunchKernelGGL(adg::k_gemm_bf16, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0, c->stream, g);
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
+// Work buffers for minibatches of up to rows rows.
+static int mlp_work(adg_ctx* c, long rows) {
+    auto& m = c->mlp;
+    if (m.work_rows >= rows) return ADG_OK;
+    if (m.work) (void)hipFree(m.work);
+    m.work = nullptr;
+    const long per = 2L * m.nin_p + 3L * m.nout_p + 6L * m.nh;
+    HIPCHK(c, hipMalloc((void**)&m.work, (size_t)rows * per * sizeof(float)));
+    m.work_rows = rows;
+    return ADG_OK;
+}
+struct MlpWork {
+    float *Xb, *Yb, *A1, *H1, *A2, *H2, *YP, *G3, *D2, *D1;
+};
+static MlpWork mlp_views(adg_ctx* c) {
+    auto& m = c->mlp;
+    const long R = m.work_rows;
+    MlpWork w;
+    float* p = m.work;
+    w.Xb = p; p += R * m.nin_p;
+    w.Yb = p; p += R * m.nout_p;
+    w.YP = p; p += R * m.nout_p;
+    w.G3 = p; p += R * m.nout_p;
+    w.A1 = p; p += R * m.nh;
+    w.H1 = p; p += R * m.nh;
+    w.A2 = p; p += R * m.nh;
+    w.H2 = p; p += R * m.nh;
+    w.D2 = p; p += R * m.nh;
+    w.D1 = p; p += R * m.nh;
+    return w;
+}
+
+// Forward and backward pass over the B rows in Xb/Yb: gradient into
+// m.grad (zeroed first), summed loss added to dacc[0].
+static int mlp_fwd_bwd(adg_ctx* c, long B) {
+    auto& m = c->mlp;
+    const MlpWork w = mlp_views(c);
+    const int nh = m.nh, Bi = (int)B;
+    int rc;
+    float* P = m.par;
+    float* G = m.grad;
+    if ((rc = mlp_fill(c, G, m.npar, 0.f)) != ADG_OK) return rc;
+    // forward: A1 = Xb W1^T + b1, H1 = gelu; A2 = H1 W2^T + b2, H2 = gelu; YP = H2 W3^T + b3
+    if ((rc = mlp_gemm(c, w.Xb, m.nin_p, 0, P + m.oW1, m.nin_p, 1, w.A1, nh, Bi, nh, m.nin_p, 1, P + m.ob1, w.H1)))
+        return rc;
+    if ((rc = mlp_gemm(c, w.H1, nh, 0, P + m.oW2, nh, 1, w.A2, nh, Bi, nh, nh, 1, P + m.ob2, w.H2))) return rc;
+    if ((rc = mlp_gemm(c, w.H2, nh, 0, P + m.oW3, nh, 1, w.YP, m.nout_p, Bi, m.nout_p, nh, 1, P + m.ob3, nullptr)))
+        return rc;
+    hipLaunchKernelGGL(adg::k_mlp_lossgrad, dim3(nblocks(B * m.nout_p)), dim3(256), 0, c->stream, w.YP, w.Yb, w.G3,
+                       B, m.nout_p, m.nout, m.dacc);
+    HIPCHK(c, hipGetLastError());
+    auto colsum = [&](const float* Gm, int ncol, float* out) -> int {
+        hipLaunchKernelGGL(adg::k_colsum, dim3((unsigned)((ncol + 255) / 256), (unsigned)((B + 255) / 256)), dim3(256),
+                           0, c->stream, Gm, B, ncol, out);
+        HIPCHK(c, hipGetLastError());
+        return ADG_OK;
+    };
+    // layer 3: dW3 = G3^T H2, db3 = sum G3, D2 = (G3 W3) * gelu'(A2)
+    if ((rc = mlp_gemm(c, w.G3, m.nout_p, 1, w.H2, nh, 0, G + m.oW3, nh, m.nout_p, nh, Bi, 2))) return rc;
+    if ((rc = colsum(w.G3, m.nout_p, G + m.ob3))) return rc;
+    if ((rc = mlp_gemm(c, w.G3, m.nout_p, 0, P + m.oW3, nh, 0, w.D2, nh, Bi, nh, m.nout_p, 3, nullptr, nullptr, w.A2)))
+        return rc;
+    // layer 2
+    if ((rc = mlp_gemm(c, w.D2, nh, 1, w.H1, nh, 0, G + m.oW2, nh, nh, nh, Bi, 2))) return rc;
+    if ((rc = colsum(w.D2, nh, G + m.ob2))) return rc;
+    if ((rc = mlp_gemm(c, w.D2, nh, 0, P + m.oW2, nh, 0, w.D1, nh, Bi, nh, nh, 3, nullptr, nullptr, w.A1))) return rc;
+    // layer 1
+    if ((rc = mlp_gemm(c, w.D1, nh, 1, w.Xb, m.nin_p, 0, G + m.oW1, m.nin_p, nh, m.nin_p, Bi, 2))) return rc;
+    return colsum(w.D1, nh, G + m.ob1);
+}
+
+extern "C" {
+
+int adg_mlp_sizes(adg_ctx* c, int* out) {
+    if (!c || !out) return ADG_E_ARG;
+    int rc = mlp_alloc(c);
+    if (rc != ADG_OK) return rc;
+    out[0] = c->mlp.nin;
+    out[1] = c->mlp.nin_p;
+    out[2] = c->mlp.nh;
+    out[3] = c->mlp.nout;
+    out[4] = c->mlp.nout_p;
+    out[5] = (int)c->mlp.npar;
+    out[6] = (int)mlp_norm_size(c);
+    return ADG_OK;
+}
+
+int adg_mlp_init(adg_ctx* c, unsigned long long seed) {
+    if (!c) return ADG_E_ARG;
+    int rc = mlp_alloc(c);
+    if (rc != ADG_OK) return rc;
+    auto& m = c->mlp;
+    std::vector<float> par((size_t)m.npar, 0.f);
+    uint64_t st = mix64_h(seed ^ 0x6d6c70ull);
+    auto uni = [&]() {
+        st = mix64_h(st);
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    // Glorot-uniform weights over the logical widths, zero biases; padded
+    // input columns and output rows stay zero
+    auto init = [&](long off, int rows, int cols, int rows_used, int cols_used) {
+        const double lim = std::sqrt(6.0 / (double)(rows_used + cols_used));
+        for (int i = 0; i < rows; ++i)
+            for (int j = 0; j < cols; ++j)
+                par[(size_t)off + (size_t)i * cols + j] =
+                    (i < rows_used && j < cols_used) ? (float)((2.0 * uni() - 1.0) * lim) : 0.f;
+    };
+    init(m.oW1, m.nh, m.nin_p, m.nh, m.nin);
+    init(m.oW2, m.nh, m.nh, m.nh, m.nh);
+    init(m.oW3, m.nout_p, m.nh, m.nout, m.nh);
+    HIPCHK(c, hipMemcpy(m.par, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> norm((size_t)mlp_norm_size(c), 0.f);
+    for (int j = 0; j < m.nin_p; ++j) norm[(size_t)m.nin_p + j] = 1.f;
+    for (int j = 0; j < m.nout_p; ++j) norm[(size_t)2 * m.nin_p + m.nout_p + j] = 1.f;
+    HIPCHK(c, hipMemcpy(m.norm, norm.data(), norm.size() * sizeof(float), hipMemcpyHostToDevice));
+    if ((rc = mlp_fill(c, m.m, m.npar, 0.f)) != ADG_OK) return rc;
+    if ((rc = mlp_fill(c, m.v, m.npar, 0.f)) != ADG_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    m.adam_t = 0;
+    m.frag_ok = false;
+    m.init = true;
+    drop_graph(c);
+    return ADG_OK;
+}
+
+int adg_mlp_get_params(adg_ctx* c, float* out, long cap) {
+    if (!c || !out || !c->mlp.init) return fail(c, ADG_E_STATE, "no network");
+    auto& m = c->mlp;
+    const long tot = m.npar + mlp_norm_size(c);
+    if (cap < tot) return fail(c, ADG_E_ARG, "buffer too small");
+    HIPCHK(c, hipMemcpy(out, m.par, (size_t)m.npar * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out + m.npar, m.norm, (size_t)mlp_norm_size(c) * sizeof(float), hipMemcpyDeviceToHost));
+    return ADG_OK;
+}
+
+int adg_mlp_set_params(adg_ctx* c, const float* in, long n) {
+    if (!c || !in) return ADG_E_ARG;
+    int rc = mlp_alloc(c);
+    if (rc != ADG_OK) return rc;
+    auto& m = c->mlp;
+    if (n != m.npar + mlp_norm_size(c)) return fail(c, ADG_E_ARG, "parameter count does not match the network");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(m.par, in, (size_t)m.npar * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m.norm, in + m.npar, (size_t)mlp_norm_size(c) * sizeof(float), hipMemcpyHostToDevice));
+    if (!m.init) {
+        if ((rc = mlp_fill(c, m.m, m.npar, 0.f)) != ADG_OK) return rc;
+        if ((rc = mlp_fill(c, m.v, m.npar, 0.f)) != ADG_OK) return rc;
+        m.adam_t = 0;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    m.frag_ok = false;
+    m.init = true;
+    drop_graph(c);
+    return ADG_OK;
+}
+
+int adg_mlp_collect(adg_ctx* c, int iters, long* n_total) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (iters < 1) return fail(c, ADG_E_ARG, "iters must be >= 1");
+    int rc = mlp_alloc(c);
+    if (rc != ADG_OK) return rc;
+    auto& m = c->mlp;
+    const long nv = (long)c->H.valid_list.size();
+    if (m.ndata + nv > m.cap) {
+        const long cap = std::max(2 * m.cap, m.ndata + nv);
+        float *X = nullptr, *Y = nullptr;
+        HIPCHK(c, hipMalloc((void**)&X, (size_t)cap * m.nin_p * sizeof(float)));
+        HIPCHK(c, hipMalloc((void**)&Y, (size_t)cap * m.nout_p * sizeof(float)));
+        if (m.ndata) {
+            HIPCHK(c, hipMemcpyAsync(X, m.X, (size_t)m.ndata * m.nin_p * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(Y, m.Y, (size_t)m.ndata * m.nout_p * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (m.X) (void)hipFree(m.X);
+        if (m.Y) (void)hipFree(m.Y);
+        m.X = X;
+        m.Y = Y;
+        m.cap = cap;
+    }
+    // converged predictor with H_w kept
+    const bool keep = c->keep_hw;
+    if ((rc = adg_keep_Hw(c, 1)) != ADG_OK) return rc;
+    c->iters_override = iters;
+    rc = adg_predictor(c, -1, 0, nullptr);
+    c->iters_override = 0;
+    if (rc != ADG_OK) return rc;
+    adg::CollectArgs a{c->d_tab, c->to, c->Uw, c->Hw_dbg, c->valid_list, c->slot_lev, c->ts, c->geo, c->H.NS, nv,
+                       c->gamma, m.X + m.ndata * m.nin_p, m.Y + m.ndata * m.nout_p, m.nin_p, m.nout_p};
+#define L_CO(PP, DD) \
+    hipLaunchKernelGGL((adg::k_mlp_collect<PP, DD>), dim3((unsigned)((nv + 63) / 64)), dim3(64), 0, c->stream, a)
+    ADG_DISPATCH(c, L_CO);
+#undef L_CO
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->keep_hw = keep;
+    m.ndata += nv;
+    if (n_total) *n_total = m.ndata;
+    return ADG_OK;
+}
+
+int adg_mlp_dataset(adg_ctx* c, long start, long n, float* x, float* y) {
+    if (!c) return ADG_E_ARG;
+    auto& m = c->mlp;
+    if (start < 0 || n < 0 || start + n > m.ndata) return fail(c, ADG_E_ARG, "rows out of range");
+    if (x) HIPCHK(c, hipMemcpy(x, m.X + start * m.nin_p, (size_t)n * m.nin_p * sizeof(float), hipMemcpyDeviceToHost));
+    if (y) HIPCHK(c, hipMemcpy(y, m.Y + start * m.nout_p, (size_t)n * m.nout_p * sizeof(float), hipMemcpyDeviceToHost));
+    return ADG_OK;
+}
+
+int adg_mlp_dataset_clear(adg_ctx* c) {
+    if (!c) return ADG_E_ARG;
+    c->mlp.ndata = 0;
+    return ADG_OK;
+}
+
+int adg_mlp_fit_normalization(adg_ctx* c) {
+    if (!c || !c->mlp.init) return fail(c, ADG_E_STATE, "no network");
+    auto& m = c->mlp;
+    if (m.ndata < 2) return fail(c, ADG_E_STATE, "dataset has fewer than 2 rows");
+    const int nc = m.nin_p + m.nout_p;
+    double* sum = m.dacc + 1;
+    double* sq = sum + nc;
+    HIPCHK(c, hipMemsetAsync(sum, 0, 2 * (size_t)nc * sizeof(double), c->stream));
+    hipLaunchKernelGGL(adg::k_colstats, dim3((unsigned)((m.nin_p + 255) / 256), (unsigned)((m.ndata + 1023) / 1024)),
+                       dim3(256), 0, c->stream, m.X, m.ndata, m.nin_p, sum, sq);
+    hipLaunchKernelGGL(adg::k_colstats, dim3((unsigned)((m.nout_p + 255) / 256), (unsigned)((m.ndata + 1023) / 1024)),
+                       dim3(256), 0, c->stream, m.Y, m.ndata, m.nout_p, sum + m.nin_p, sq + m.nin_p);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> hs(2 * (size_t)nc);
+    HIPCHK(c, hipMemcpyAsync(hs.data(), sum, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<float> norm((size_t)mlp_norm_size(c));
+    auto stat = [&](int col, float* mu, float* sd) {
+        const double mean = hs[col] / (double)m.ndata;
+        const double var = std::max(0.0, hs[nc + col] / (double)m.ndata - mean * mean);
+        const double dev = std::sqrt(var);
+        *mu = (float)mean;
+        *sd = dev > 1e-6 * std::max(1.0, std::fabs(mean)) ? (float)dev : 1.f;   // constant column
+    };
+    for (int j = 0; j < m.nin_p; ++j) stat(j, &norm[j], &norm[(size_t)m.nin_p + j]);
+    for (int j = 0; j < m.nout_p; ++j)
+        stat(m.nin_p + j, &norm[(size_t)2 * m.nin_p + j], &norm[(size_t)2 * m.nin_p + m.nout_p + j]);
+    HIPCHK(c, hipMemcpy(m.norm, norm.data(), norm.size() * sizeof(float), hipMemcpyHostToDevice));
+    m.frag_ok = false;
+    return ADG_OK;
+}
+
+int adg_mlp_train(adg_ctx* c, int nsteps, int batch, double lr, unsigned long long seed, double* loss_out) {
+    if (!c || !c->mlp.init) return fail(c, ADG_E_STATE, "no network");
+    auto& m = c->mlp;
+    if (m.ndata < 1) return fail(c, ADG_E_STATE, "empty dataset");
+    if (nsteps < 1 || batch < 16 || batch % 16) return fail(c, ADG_E_ARG, "batch must be a positive multiple of 16");
+    int rc = mlp_work(c, batch);
+    if (rc != ADG_OK) return rc;
+    const MlpWork w = mlp_views(c);
+    const int tail = std::min(nsteps, 20);
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int it = 0; it < nsteps; ++it) {
+        if (it == nsteps - tail) HIPCHK(c, hipMemsetAsync(m.dacc, 0, sizeof(double), c->stream));
+        hipLaunchKernelGGL(adg::k_gather_batch, dim3(nblocks((long)batch * (m.nin_p + m.nout_p))), dim3(256), 0,
+                           c->stream, m.X, m.Y, m.ndata, m.nin_p, m.nout_p, m.norm, (uint64_t)seed, m.adam_t,
+                           (long)batch, w.Xb, w.Yb);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = mlp_fwd_bwd(c, batch)) != ADG_OK) return rc;
+        m.adam_t += 1;
+        const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)m.adam_t)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)m.adam_t)));
+        hipLaunchKernelGGL(adg::k_adam, dim3(nblocks(m.npar)), dim3(256), 0, c->stream, m.par, m.grad, m.m, m.v, m.npar,
+                           (float)lr, b1, b2, 1e-8f, c1, c2);
+        HIPCHK(c, hipGetLastError());
+    }
+    double l = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&l, m.dacc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (loss_out) *loss_out = l / tail;
+    m.frag_ok = false;
+    drop_graph(c);
+    return ADG_OK;
+}
+
+int adg_mlp_loss_grad(adg_ctx* c, int n, const float* x, const float* y, double* loss, float* grad) {
+    if (!c || !c->mlp.init || !x || !y) return fail(c, ADG_E_STATE, "no network");
+    if (n < 16 || n % 16) return fail(c, ADG_E_ARG, "n must be a positive multiple of 16");
+    auto& m = c->mlp;
+    int rc = mlp_work(c, n);
+    if (rc != ADG_OK) return rc;
+    const MlpWork w = mlp_views(c);
+    HIPCHK(c, hipMemcpyAsync(w.Xb, x, (size_t)n * m.nin_p * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w.Yb, y, (size_t)n * m.nout_p * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(m.dacc, 0, sizeof(double), c->stream));
+    if ((rc = mlp_fwd_bwd(c, n)) != ADG_OK) return rc;
+    double l = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&l, m.dacc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, m.grad, (size_t)m.npar * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return ADG_OK;
+}
+
+int adg_mlp_warm_start(adg_ctx* c, int lev, double* host) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (!c->mlp.init) return fail(c, ADG_E_STATE, "no network");
+    if (lev < 0 || lev >= c->H.nlev || !host) return fail(c, ADG_E_ARG, "bad arguments");
+    int rc = mlp_pack(c);
+    if (rc != ADG_OK) return rc;
+    const long begin = c->H.lev[lev].slot_off, n = c->H.lev[lev].nslot;
+#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, begin, n)
+    ADG_DISPATCH(c, L_MW);
+#undef L_MW
+    if (rc != ADG_OK) return rc;
+    const long NS = c->H.NS, rows = c->mlp.nout;
+    std::vector<float> tmp((size_t)rows * NS);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->warm, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (long o = 0; o < rows; ++o)
+        for (long s = 0; s < n; ++s) host[o * n + s] = tmp[(size_t)o * NS + begin + s];
+    return ADG_OK;
+}
+
+int adg_set_learned(adg_ctx* c, int on, double tol, int max_iters) {
+    if (!c) return ADG_E_ARG;
+    if (on < 0 || on > 2) return fail(c, ADG_E_ARG, "on must be 0, 1 or 2");
+    if (on == 1 && !(tol > 0.0)) return fail(c, ADG_E_ARG, "tol must be positive");
+    if (on && (max_iters < 1 || max_iters > 8)) return fail(c, ADG_E_ARG, "max_iters must be 1..8");
+    c->learned = on != 0;
+    if (on) {
+        c->learned_tol = on == 2 ? -1.f : (float)tol;
+        c->learned_iters = max_iters;
+    }
+    drop_graph(c);
+    return ADG_OK;
+}
+
+int adg_learned_stats(adg_ctx* c, unsigned long long* out, int reset) {
+    if (!c) return ADG_E_ARG;
+    if (out) {
+        HIPCHK(c, hipMemcpyAsync(out, c->lstats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (reset) {
+        HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ADG_OK;
+}
+
+}  // extern "C"
 
 extern "C++" {
 template <typename T>
@@ -831,7 +1369,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
 
 static int step_launches(adg_ctx* c, double c_dt, double T_end) {
     int rc;
-    if ((rc = adg_predictor(c, -1, 0, nullptr)) != ADG_OK) return rc;
+    if ((rc = adg_predictor(c, -1, c->learned ? 1 : 0, nullptr)) != ADG_OK) return rc;
     if ((rc = adg_face_flux(c, -1)) != ADG_OK) return rc;
     if ((rc = adg_corrector(c, -1)) != ADG_OK) return rc;
     return c->precision == ADG_FP64 ? launch_epilogue<double>(c, c_dt, T_end) : launch_epilogue<float>(c, c_dt, T_end);
@@ -840,9 +1378,14 @@ static int step_launches(adg_ctx* c, double c_dt, double T_end) {
 int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_out) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     if (nsteps < 0) return fail(c, ADG_E_ARG, "nsteps must be >= 0");
-    const int kind = pred_variant(c) * 32 + c->nl_wpb;   // launch shape baked into the graph
+    const int kind = (pred_variant(c) * 32 + c->nl_wpb) * 2 + (c->learned ? 1 : 0);   // baked into the graph
     if (!c->graph_exec || c->graph_cdt != c_dt || c->graph_T != T_end || c->graph_kind != kind) {
         drop_graph(c);
+        if (c->learned) {
+            if (!c->mlp.init) return fail(c, ADG_E_STATE, "learned mode without a network");
+            int prc = mlp_pack(c);
+            if (prc != ADG_OK) return prc;
+        }
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = step_launches(c, c_dt, T_end);
@@ -866,7 +1409,7 @@ int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_
 
 int adg_ader(adg_ctx* c) {
     int rc;
-    if ((rc = adg_predictor(c, -1, 0, nullptr)) != ADG_OK) return rc;
+    if ((rc = adg_predictor(c, -1, c->learned ? 1 : 0, nullptr)) != ADG_OK) return rc;
     if ((rc = adg_face_flux(c, -1)) != ADG_OK) return rc;
     if ((rc = adg_corrector(c, -1)) != ADG_OK) return rc;
     if (c->H.nlev > 1 && (rc = adg_reflux(c, -1)) != ADG_OK) return rc;
@@ -995,6 +1538,10 @@ void adg_destroy(adg_ctx* c) {
     for (int v = 0; v < 2; ++v)
         if (c->d_frag[v]) (void)hipFree(c->d_frag[v]);
     if (c->ts) (void)hipFree(c->ts);
+    if (c->lstats) (void)hipFree(c->lstats);
+    for (void* p : {(void*)c->mlp.par, (void*)c->mlp.grad, (void*)c->mlp.m, (void*)c->mlp.v, (void*)c->mlp.norm,
+                    (void*)c->mlp.frag, (void*)c->mlp.X, (void*)c->mlp.Y, (void*)c->mlp.work, (void*)c->mlp.dacc})
+        if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

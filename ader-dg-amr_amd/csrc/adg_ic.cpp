@@ -12,7 +12,10 @@
 //  2  uniform state;
 //  3  radial / spherical shock tube centred in the domain (2D: the
 //     reference's radial_shock_tube, Compressible_Euler.h:336-357, with the
-//     radius and the two states as parameters; 3D has no reference IC).
+//     radius and the two states as parameters; 3D has no reference IC);
+//  4  Kelvin-Helmholtz shear layer as in 1 with the seeded multi-mode
+//     perturbation v = sum_{k=1..8} a_k sin(2 pi k x), a_k = amax * r_k,
+//     r_k a seeded uniform number (no counterpart in the reference).
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -41,6 +44,13 @@ double cell_uniform(uint64_t seed, int lev, const int* iv) {
     return (double)(k >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// uniform [0,1) for (seed, mode k) of the multi-mode KH perturbation
+double mode_uniform(uint64_t seed, int k) {
+    uint64_t h = mix64(seed ^ 0x6b68ull);
+    h = mix64(h ^ (uint64_t)(uint32_t)k);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
 void pointwise_state(int D, int ic_case, const double* prm, double gamma, const double* x, const double* plo,
                      const double* phi, int lev, const int* iv, double* u) {
     for (int q = 0; q < 5; ++q) u[q] = 0.0;
@@ -62,6 +72,15 @@ void pointwise_state(int D, int ic_case, const double* prm, double gamma, const 
         const bool in = std::fabs(x[1] - 0.5) < 0.25;
         const double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
         const double u2 = amp * std::sin(4.0 * M_PI * x[0]) + namp * (2.0 * cell_uniform(seed, lev, iv) - 1.0);
+        u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
+        u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+    } else if (ic_case == 4) {
+        const uint64_t seed = (uint64_t)prm[0];
+        const double amax = prm[1];
+        const bool in = std::fabs(x[1] - 0.5) < 0.25;
+        const double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
+        double u2 = 0.0;
+        for (int k = 1; k <= 8; ++k) u2 += amax * mode_uniform(seed, k) * std::sin(2.0 * M_PI * k * x[0]);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
         u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
     } else if (ic_case == 2) {

@@ -169,6 +169,7 @@ struct PredArgs {
     long slot_begin;
     long nelem;
     double gamma;
+    int iters;   // Picard iterations (P)
 };
 
 template <int P, int D>
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
 #pragma unroll
     for (int n = 0; n < Nst; ++n) hw[n] = (n < Ns) ? u[n] : 0.0;  // AmrDG.tpp:522-533 flat copy
 
-    for (int it = 0; it < P; ++it) {
+    for (int it = 0; it < a.iters; ++it) {
         double acc[Nst];
 #pragma unroll
         for (int n = 0; n < Nst; ++n) acc[n] = c[n];
@@ -335,6 +336,13 @@ struct PredMArgs {
     long slot_begin;
     long nelem;
     double gamma;
+    int iters;                               // Picard iterations of the plain path (P)
+    // learned warm start (node-lane kernel only; warm == nullptr: plain)
+    const float* __restrict__ warm;          // H_w start increment over A0 U_w, [q*Np_st+n][slot]
+    float tol;                               // accept when max|H_k - H_{k-1}| <= tol rho0;
+                                             // < 0: exactly learned_iters iterations, no check
+    int learned_iters;                       // Picard iterations tried from the learned start
+    unsigned long long* __restrict__ stats;  // [elements, elements that fell back]
 };
 
 template <typename T, int P, int D>
@@ -385,7 +393,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
 #pragma unroll
         for (int ks = 0; ks < KSU; ++ks) hu[ks] = hb[ks];
 
-        for (int it = 0; it < P; ++it) {
+        for (int it = 0; it < a.iters; ++it) {
             v4 acc[MB2];
 #pragma unroll
             for (int mb = 0; mb < MB2; ++mb) acc[mb] = v4{0, 0, 0, 0};
@@ -551,6 +559,9 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     for (int i = threadIdx.x; i < nfr; i += blockDim.x) fr[i] = a.frags[i];
     __syncthreads();
 
+    __shared__ unsigned long long wg_stats[2];
+    if (threadIdx.x == 0) { wg_stats[0] = 0ull; wg_stats[1] = 0ull; }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     T* tile = fr + nfr + wave * (D * 16 * TST);
     const int g = lane >> 4, c = lane & 15;
@@ -621,7 +632,8 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
             wave_lds_sync();
         };
 
-        for (int it = 0; it < P; ++it) {
+        // one Picard iteration H_w <- C0 + sum_d B_d g_d(H_w)
+        auto picard = [&]() {
             v4 acc[MB2];
 #pragma unroll
             for (int mb = 0; mb < MB2; ++mb) acc[mb] = c0[mb];
@@ -640,6 +652,80 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
             }
 #pragma unroll
             for (int ks = 0; ks < KSH; ++ks) hb[ks] = acc[ks >> 2][ks & 3];
+        };
+
+        int nplain = a.iters;
+        bool keep_learned = false;   // this column's element passed the check
+        T hl[KSH];
+        if (a.warm) {
+            // learned start H_0 = A0 U_w + Delta, then Picard iterations until
+            // the update of an element passes the tolerance.  The decision is
+            // per element (it depends on the element's data only, so a ghost
+            // copy and its source cell always agree and face fluxes stay
+            // conservative); elements that do not pass are recomputed by the
+            // plain predictor from its own start (the fallback).
+            const T inv_sc = T(1.0 / fabs(a.Uw[slot]));   // 1 / rho0 of the column's element
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) {
+                const int mode = 4 * ks + g;
+                hb[ks] = c0[ks >> 2][ks & 3] + (mode < Nst ? T(a.warm[((long)cq * Nst + mode) * a.NS + slot]) : T(0));
+            }
+            bool pass_e = false;
+            if (a.tol < 0.f) {
+                // diagnostic: exactly learned_iters iterations, no check
+                for (int k = 0; k < a.learned_iters; ++k) picard();
+                pass_e = true;
+            } else {
+                // an element that passes keeps the iterate it passed with
+                // while the rest of its wavefront iterates on, so its result
+                // does not depend on the other elements of the wavefront
+                for (int k = 0; k < a.learned_iters; ++k) {
+                    T prev[KSH];
+#pragma unroll
+                    for (int ks = 0; ks < KSH; ++ks) prev[ks] = hb[ks];
+                    picard();
+                    T r = T(0);
+#pragma unroll
+                    for (int ks = 0; ks < KSH; ++ks) r = fmax(r, fabs(hb[ks] - prev[ks]));
+                    r *= inv_sc;
+                    // max over the 16 lanes of the element: its 4 columns x 4 lane groups
+                    constexpr int QX0 = sizeof(T) == 8 ? 4 : 1, QX1 = sizeof(T) == 8 ? 8 : 2;
+                    r = fmax(r, __shfl_xor(r, QX0, 64));
+                    r = fmax(r, __shfl_xor(r, QX1, 64));
+                    r = fmax(r, __shfl_xor(r, 16, 64));
+                    r = fmax(r, __shfl_xor(r, 32, 64));
+                    if (pass_e) {
+#pragma unroll
+                        for (int ks = 0; ks < KSH; ++ks) hb[ks] = prev[ks];
+                    } else {
+                        pass_e = r <= T(a.tol);   // NaN fails
+                    }
+                    if (__all(pass_e)) break;
+                }
+            }
+            // one representative lane per element (lane group 0, variable 0)
+            const bool rep = g == 0 && cq == 0 && active;
+            const unsigned long long n_el = __popcll(__ballot(rep));
+            const unsigned long long n_fb = __popcll(__ballot(rep && !pass_e));
+            if (lane == 0 && n_el) {   // workgroup totals in LDS, one global update per workgroup
+                atomicAdd(&wg_stats[0], n_el);
+                if (n_fb) atomicAdd(&wg_stats[1], n_fb);
+            }
+            if (__all(pass_e)) {
+                nplain = 0;
+            } else {
+                keep_learned = pass_e;
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) {
+                    hl[ks] = hb[ks];
+                    hb[ks] = ks < KSU ? hu[ks] : T(0);
+                }
+            }
+        }
+        for (int it = 0; it < nplain; ++it) picard();
+        if (keep_learned) {
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) hb[ks] = hl[ks];
         }
 
         // final evaluation: corrector volume term
@@ -692,6 +778,13 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
                     }
                 }
             }
+    }
+    if (a.warm) {
+        __syncthreads();
+        if (threadIdx.x == 0 && wg_stats[0]) {
+            atomicAdd(a.stats, wg_stats[0]);
+            if (wg_stats[1]) atomicAdd(a.stats + 1, wg_stats[1]);
+        }
     }
 }
 

@@ -1,0 +1,422 @@
+// adg_mlp.hpp -- learned warm start of the space-time predictor (no
+// counterpart in the reference): a 3-layer MLP with GELU maps each element's
+// U_w modes, local CFL number and Dt/dx to a starting guess for H_w.
+//
+// Surrogate definition (per element, tau_d = Dt/dx_d, rho0 = U_w[rho][0]):
+//   x = [U_w/rho0 (q-major, Q*Np_s values), nu, tau_0 .. tau_{D-1}],
+//       nu = sum_d tau_d (|u_d| + c) at the cell-average state;
+//   y = (H_w* - A0 U_w) / (rho0 max_d tau_d), q-major (Q*Np_st values),
+//       H_w* the converged predictor, A0 = Mk_h_w^-1 Mk_pred;
+//   x and y are standardised per column (dataset mean / deviation).
+// Network: h1 = gelu(W1 x + b1), h2 = gelu(W2 h1 + b2), y = W3 h2 + b3,
+// widths nin_p -> 128 -> 128 -> nout_p (nin_p, nout_p = widths padded to 16),
+// gelu(a) = a/2 (1 + erf(a/sqrt 2)).  Every matrix product runs on the
+// 16x16x16 BF16 MFMA with FP32 accumulation; operands are rounded to BF16
+// (round to nearest even) as they are loaded, master weights stay FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "adg_kernels.hpp"
+
+namespace adg {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpHidden = 128;
+
+// BF16 bits of an FP32 value, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short bf16_bits(float v) {
+    unsigned u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)bf16_bits(a);
+    r[1] = (short)bf16_bits(b);
+    r[2] = (short)bf16_bits(c);
+    r[3] = (short)bf16_bits(d);
+    return r;
+}
+// C += A B, 16x16x16: A[i][4k'+t] on lane i+16k' (t = 0..3 packed),
+// B[4k'+t][j] on lane j+16k', C row 4g+r of lane group g in register r.
+__device__ __forceinline__ f32x4 mma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float gelu_f(float a) { return 0.5f * a * (1.0f + erff(a * 0.70710678118654752f)); }
+__device__ __forceinline__ float dgelu_f(float a) {
+    return 0.5f * (1.0f + erff(a * 0.70710678118654752f)) + a * 0.39894228040143268f * expf(-0.5f * a * a);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled GEMM C = op(A) op(B), M, N, K multiples of 16; one wavefront per
+// 16x16 output tile and K chunk.  op(A) = A (M x K, A[i*lda+k]) or A^T
+// (A[k*lda+i]); op(B) = B (K x N, B[k*ldb+j]) or B^T (B[j*ldb+k]).
+// Epilogues: 0 store; 1 C = acc + bias[j], C2 = gelu(C) (C2 nullable); 2 atomic add
+// (split K); 3 C = acc * gelu'(aux[i][j]).
+struct GemmArgs {
+    const float* A;
+    long lda;
+    int transA;
+    const float* B;
+    long ldb;
+    int transB;
+    float* C;
+    long ldc;
+    int M, N, K, ksplit, epi;
+    const float* bias;
+    float* C2;
+    const float* aux;
+};
+
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs a) {
+    const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int tm_n = a.M >> 4, tn_n = a.N >> 4;
+    const long ntile = (long)tm_n * tn_n;
+    if (w >= ntile * a.ksplit) return;   // whole wavefronts exit together
+    const int kc = (int)(w / ntile);
+    const long tile = w - (long)kc * ntile;
+    const int tm = (int)(tile / tn_n), tn = (int)(tile - (long)tm * tn_n);
+    const int lane = threadIdx.x & 63, i = lane & 15, kg = lane >> 4;
+    const int ks_n = a.K >> 4;
+    const int ks0 = (int)((long)ks_n * kc / a.ksplit), ks1 = (int)((long)ks_n * (kc + 1) / a.ksplit);
+    const long row = 16 * tm + i, col = 16 * tn + i;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s = ks0; s < ks1; ++s) {
+        const long k0 = 16 * s + 4 * kg;
+        float av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            av[t] = a.transA ? a.A[(k0 + t) * a.lda + row] : a.A[row * a.lda + k0 + t];
+            bv[t] = a.transB ? a.B[col * a.ldb + k0 + t] : a.B[(k0 + t) * a.ldb + col];
+        }
+        acc = mma_bf16(pack_bf16(av[0], av[1], av[2], av[3]), pack_bf16(bv[0], bv[1], bv[2], bv[3]), acc);
+    }
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long ii = 16 * tm + 4 * g + r, jj = 16 * tn + c;
+        float v = acc[r];
+        float* dst = a.C + ii * a.ldc + jj;
+        if (a.epi == 0) {
+            *dst = v;
+        } else if (a.epi == 1) {
+            v += a.bias[jj];
+            *dst = v;
+            if (a.C2) a.C2[ii * a.ldc + jj] = gelu_f(v);
+        } else if (a.epi == 2) {
+            atomicAdd(dst, v);
+        } else {
+            *dst = v * dgelu_f(a.aux[ii * a.ldc + jj]);
+        }
+    }
+}
+
+// g = 2 (yp - y) / (n nout) on the first nout columns (0 on padding); the
+// summed squared error is accumulated into *loss (FP64) per block.
+__global__ void __launch_bounds__(256) k_mlp_lossgrad(const float* __restrict__ yp, const float* __restrict__ y,
+                                                      float* __restrict__ g, long n, int nout_p, int nout,
+                                                      double* __restrict__ loss) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (t < n * nout_p) {
+        const int col = (int)(t % nout_p);
+        const float e = col < nout ? yp[t] - y[t] : 0.f;
+        g[t] = 2.f * e / (float)((double)n * nout);
+        e2 = (double)e * e;
+    }
+    for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = e2;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (ws[0] + ws[1] + ws[2] + ws[3]) / ((double)n * nout));
+}
+
+// out[j] += sum over rows [r0, r0 + 256) of G[r][j]  (bias gradients)
+__global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ G, long n, int ncol, float* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ncol) return;
+    const long r0 = (long)blockIdx.y * 256, r1 = r0 + 256 < n ? r0 + 256 : n;
+    float s = 0.f;
+    for (long r = r0; r < r1; ++r) s += G[r * ncol + j];
+    atomicAdd(out + j, s);
+}
+
+// per-column sum and sum of squares (FP64) of an n x ncol row-major array
+__global__ void __launch_bounds__(256) k_colstats(const float* __restrict__ X, long n, int ncol,
+                                                  double* __restrict__ sum, double* __restrict__ sq) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ncol) return;
+    const long r0 = (long)blockIdx.y * 1024, r1 = r0 + 1024 < n ? r0 + 1024 : n;
+    double s = 0.0, s2 = 0.0;
+    for (long r = r0; r < r1; ++r) {
+        const double v = X[r * ncol + j];
+        s += v;
+        s2 += v * v;
+    }
+    atomicAdd(sum + j, s);
+    atomicAdd(sq + j, s2);
+}
+
+__global__ void __launch_bounds__(256) k_fill(float* __restrict__ p, long n, float v) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ void __launch_bounds__(256) k_adam(float* __restrict__ w, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, long n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gi = g[t];
+    const float mi = b1 * m[t] + (1.f - b1) * gi;
+    const float vi = b2 * v[t] + (1.f - b2) * gi * gi;
+    m[t] = mi;
+    v[t] = vi;
+    w[t] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// Random minibatch: row b takes dataset row hash(seed, step, b) mod ndata,
+// standardised with the column statistics.
+__device__ __forceinline__ uint64_t mix64_d(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__global__ void __launch_bounds__(256) k_gather_batch(const float* __restrict__ X, const float* __restrict__ Y, long ndata,
+                                                      int nin_p, int nout_p, const float* __restrict__ norm,
+                                                      uint64_t seed, long step, long batch, float* __restrict__ Xb,
+                                                      float* __restrict__ Yb) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = nin_p + nout_p;
+    if (t >= batch * w) return;
+    const long b = t / w;
+    const int col = (int)(t - b * w);
+    const long r = (long)(mix64_d(seed ^ mix64_d((uint64_t)step * 0x100000001ull + (uint64_t)b)) % (uint64_t)ndata);
+    const float* mux = norm;
+    const float* sdx = norm + nin_p;
+    const float* muy = norm + 2 * nin_p;
+    const float* sdy = muy + nout_p;
+    if (col < nin_p) Xb[b * nin_p + col] = (X[r * nin_p + col] - mux[col]) / sdx[col];
+    else {
+        const int o = col - nin_p;
+        Yb[b * nout_p + o] = (Y[r * nout_p + o] - muy[o]) / sdy[o];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Surrogate features and targets of every valid element, from U_w and the
+// converged predictor H_w (FP64 [q][Np_st][slot]); one thread per element.
+struct CollectArgs {
+    const double* __restrict__ tab;
+    TabOff to;
+    const double* __restrict__ Uw;
+    const double* __restrict__ Hw;
+    const int* __restrict__ valid_list;
+    const uint8_t* __restrict__ slot_lev;
+    const TimeState* __restrict__ ts;
+    LevelGeo geo;
+    long NS, nvalid;
+    double gamma;
+    float* __restrict__ X;   // rows [nvalid][nin_p]
+    float* __restrict__ Y;   // rows [nvalid][nout_p]
+    int nin_p, nout_p;
+};
+
+template <int P, int D>
+__device__ __forceinline__ void mlp_features(const double* U, const double* tau, double gamma, double* x) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
+    const double rho0 = U[0];
+    for (int q = 0; q < Q; ++q)
+        for (int k = 0; k < Ns; ++k) x[q * Ns + k] = U[q * Ns + k] / rho0;
+    double ke = 0.0;
+    for (int d = 0; d < D; ++d) ke += U[(d + 1) * Ns] * U[(d + 1) * Ns];
+    const double prs = (gamma - 1.0) * (U[(D + 1) * Ns] - 0.5 * ke / rho0);
+    const double c = sqrt(gamma * prs / rho0);
+    double nu = 0.0;
+    for (int d = 0; d < D; ++d) nu += tau[d] * (fabs(U[(d + 1) * Ns] / rho0) + c);
+    x[Q * Ns] = nu;
+    for (int d = 0; d < D; ++d) x[Q * Ns + 1 + d] = tau[d];
+}
+
+template <int P, int D>
+__global__ void __launch_bounds__(64) k_mlp_collect(CollectArgs a) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst;
+    constexpr int NIN = Q * Ns + 1 + D;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nvalid) return;
+    const long slot = a.valid_list[i];
+    const int lev = a.slot_lev[slot];
+    const double Dt = a.ts->Dt;
+    double tau[D], tmax = 0.0;
+    for (int d = 0; d < D; ++d) { tau[d] = Dt / a.geo.dx[lev][d]; tmax = fmax(tmax, tau[d]); }
+    double U[Q * Ns];
+    for (int f = 0; f < Q * Ns; ++f) U[f] = a.Uw[(long)f * a.NS + slot];
+    double x[NIN];
+    mlp_features<P, D>(U, tau, a.gamma, x);
+    float* xr = a.X + i * a.nin_p;
+    for (int f = 0; f < a.nin_p; ++f) xr[f] = f < NIN ? (float)x[f] : 0.f;
+    const double* A0 = a.tab + a.to.A0;
+    const double sc = 1.0 / (U[0] * tmax);
+    float* yr = a.Y + i * a.nout_p;
+    for (int q = 0; q < Q; ++q)
+        for (int n = 0; n < Nst; ++n) {
+            double c0 = 0.0;
+            for (int m = 0; m < Ns; ++m) c0 += A0[n * Ns + m] * U[q * Ns + m];
+            yr[q * Nst + n] = (float)((a.Hw[((long)q * Nst + n) * a.NS + slot] - c0) * sc);
+        }
+    for (int o = Q * Nst; o < a.nout_p; ++o) yr[o] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Inference: one wavefront per 16 elements (matrix columns), weights as the A
+// operand (pre-packed BF16 fragments staged in LDS), activations as the B
+// operand; the accumulator of output block u (lane group g, register r =
+// unit 16u + 4g + r, column = element) is the B operand of K step u of the
+// next layer, so activations never leave registers.  Writes the warm-start
+// increment Delta = rho0 tau_max (mu_y + sd_y y) per output, [o][slot].
+struct WarmArgs {
+    const uint2* __restrict__ frag;    // [nfrag][64] packed BF16 fragments
+    int f1, f2, f3, nfrag;             // first fragment of each layer
+    const float* __restrict__ b1;
+    const float* __restrict__ b2;
+    const float* __restrict__ b3;
+    const float* __restrict__ norm;    // mu_x, sd_x [nin_p], mu_y, sd_y [nout_p]
+    const double* __restrict__ Uw;
+    const uint8_t* __restrict__ slot_lev;
+    const TimeState* __restrict__ ts;
+    LevelGeo geo;
+    long NS, slot_begin, nelem;
+    double gamma;
+    float* __restrict__ warm;          // [nout][NS]
+};
+
+template <int P, int D, int NINP, int NOUTP>
+__global__ void __launch_bounds__(512) k_mlp_warm(WarmArgs a) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst;
+    constexpr int NIN = Q * Ns + 1 + D, NOUT = Q * Nst, NH = kMlpHidden;
+    constexpr int KS1 = NINP / 16, KS2 = NH / 16, UT3 = NOUTP / 16;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    uint2* fr = reinterpret_cast<uint2*>(smem_raw);
+    for (int i = threadIdx.x; i < a.nfrag * 64; i += blockDim.x) fr[i] = a.frag[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    const float* mux = a.norm;
+    const float* sdx = a.norm + NINP;
+    const float* muy = a.norm + 2 * NINP;
+    const float* sdy = muy + NOUTP;
+    const long nbatch = (a.nelem + 16L * nwave - 1) / (16L * nwave);
+    auto bf = [&](int f) {
+        const uint2 v = fr[f * 64 + lane];
+        bf16x4 r;
+        r[0] = (short)(v.x & 0xffff);
+        r[1] = (short)(v.x >> 16);
+        r[2] = (short)(v.y & 0xffff);
+        r[3] = (short)(v.y >> 16);
+        return r;
+    };
+    for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
+        const long e = batch * 16L * nwave + 16L * wave + c;
+        const bool active = e < a.nelem;
+        const long slot = a.slot_begin + (active ? e : a.nelem - 1);
+        const int lev = a.slot_lev[slot];
+        const double Dt = a.ts->Dt;
+        double tau[D], tmax = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { tau[d] = Dt / a.geo.dx[lev][d]; tmax = fmax(tmax, tau[d]); }
+        double avg[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) avg[q] = a.Uw[(long)q * Ns * a.NS + slot];
+        const double rho0 = avg[0];
+        double ke = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) ke += avg[d + 1] * avg[d + 1];
+        const double prs = (a.gamma - 1.0) * (avg[D + 1] - 0.5 * ke / rho0);
+        const double cs = sqrt(a.gamma * prs / rho0);
+        double nu = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) nu += tau[d] * (fabs(avg[d + 1] / rho0) + cs);
+        // layer 1 input: features 16 s + 4 g + t of element c
+        bf16x4 xb[KS1];
+#pragma unroll
+        for (int s = 0; s < KS1; ++s) {
+            float v[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int f = 16 * s + 4 * g + t;
+                double xv = 0.0;
+                if (f < Q * Ns) xv = a.Uw[(long)f * a.NS + slot] / rho0;
+                else if (f == Q * Ns) xv = nu;
+                else if (f < NIN) xv = tau[f - Q * Ns - 1];
+                v[t] = f < NIN ? ((float)xv - mux[f]) / sdx[f] : 0.f;
+            }
+            xb[s] = pack_bf16(v[0], v[1], v[2], v[3]);
+        }
+        bf16x4 h1[KS2];
+#pragma unroll
+        for (int u = 0; u < KS2; ++u) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < KS1; ++s) acc = mma_bf16(bf(a.f1 + u * KS1 + s), xb[s], acc);
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[r] + a.b1[16 * u + 4 * g + r]);
+            h1[u] = pack_bf16(v[0], v[1], v[2], v[3]);
+            __builtin_amdgcn_sched_barrier(0);   // keeps fragment reads next to their MFMAs
+        }
+        bf16x4 h2[KS2];
+#pragma unroll
+        for (int u = 0; u < KS2; ++u) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < KS2; ++s) acc = mma_bf16(bf(a.f2 + u * KS2 + s), h1[s], acc);
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[r] + a.b2[16 * u + 4 * g + r]);
+            h2[u] = pack_bf16(v[0], v[1], v[2], v[3]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const float sc = (float)(rho0 * tmax);
+#pragma unroll
+        for (int u = 0; u < UT3; ++u) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < KS2; ++s) acc = mma_bf16(bf(a.f3 + u * KS2 + s), h2[s], acc);
+            __builtin_amdgcn_sched_barrier(0);
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int o = 16 * u + 4 * g + r;
+                    if (o < NOUT) a.warm[(long)o * a.NS + slot] = sc * (muy[o] + sdy[o] * (acc[r] + a.b3[o]));
+                }
+            }
+        }
+    }
+}
+
+// Packs layer weights W [rows][cols] (FP32, row-major) into A-operand
+// fragments: fragment (u, s), lane i + 16k holds W[16u + i][16s + 4k + t].
+__global__ void __launch_bounds__(64) k_mlp_pack(const float* __restrict__ W, int rows, int cols, uint2* __restrict__ out) {
+    const int f = blockIdx.x, lane = threadIdx.x;
+    const int ks = cols / 16;
+    const int u = f / ks, s = f - u * ks;
+    const int i = lane & 15, k = lane >> 4;
+    const float* w = W + (long)(16 * u + i) * cols + 16 * s + 4 * k;
+    uint2 v;
+    v.x = (unsigned)bf16_bits(w[0]) | ((unsigned)bf16_bits(w[1]) << 16);
+    v.y = (unsigned)bf16_bits(w[2]) | ((unsigned)bf16_bits(w[3]) << 16);
+    out[(long)f * 64 + lane] = v;
+    (void)rows;
+}
+
+}  // namespace adg

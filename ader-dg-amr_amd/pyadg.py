@@ -26,6 +26,9 @@ EXPORTS = [
     "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_set_time",
     "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
     "adg_project_ic", "adg_set_predictor_kernel", "adg_set_predictor_waves", "adg_last_error", "adg_destroy",
+    "adg_mlp_sizes", "adg_mlp_init", "adg_mlp_get_params", "adg_mlp_set_params", "adg_mlp_collect",
+    "adg_mlp_dataset", "adg_mlp_dataset_clear", "adg_mlp_fit_normalization", "adg_mlp_train",
+    "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats",
 ]
 
 
@@ -72,6 +75,22 @@ def lib():
     L.adg_project_ic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
     L.adg_set_predictor_kernel.argtypes = [C.c_void_p, C.c_int]
     L.adg_set_predictor_waves.argtypes = [C.c_void_p, C.c_int]
+    _fp = C.POINTER(C.c_float)
+    _lp = C.POINTER(C.c_long)
+    _up = C.POINTER(C.c_ulonglong)
+    L.adg_mlp_sizes.argtypes = [C.c_void_p, _ip]
+    L.adg_mlp_init.argtypes = [C.c_void_p, C.c_ulonglong]
+    L.adg_mlp_get_params.argtypes = [C.c_void_p, _fp, C.c_long]
+    L.adg_mlp_set_params.argtypes = [C.c_void_p, _fp, C.c_long]
+    L.adg_mlp_collect.argtypes = [C.c_void_p, C.c_int, _lp]
+    L.adg_mlp_dataset.argtypes = [C.c_void_p, C.c_long, C.c_long, _fp, _fp]
+    L.adg_mlp_dataset_clear.argtypes = [C.c_void_p]
+    L.adg_mlp_fit_normalization.argtypes = [C.c_void_p]
+    L.adg_mlp_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_ulonglong, _dp]
+    L.adg_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp, _dp, _fp]
+    L.adg_mlp_warm_start.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.adg_set_learned.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+    L.adg_learned_stats.argtypes = [C.c_void_p, _up, C.c_int]
     L.adg_last_error.argtypes = [C.c_void_p]
     L.adg_last_error.restype = C.c_char_p
     L.adg_destroy.argtypes = [C.c_void_p]
@@ -246,3 +265,77 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    # ---- learned warm start -------------------------------------------
+    def mlp_sizes(self):
+        o = np.zeros(7, dtype=np.int32)
+        self._chk(self.L.adg_mlp_sizes(self.h, _i(o)), "adg_mlp_sizes")
+        return dict(zip(("nin", "nin_p", "nh", "nout", "nout_p", "npar", "nnorm"), map(int, o)))
+
+    def mlp_init(self, seed=1):
+        self._chk(self.L.adg_mlp_init(self.h, seed), "adg_mlp_init")
+
+    def mlp_params(self):
+        sz = self.mlp_sizes()
+        out = np.zeros(sz["npar"] + sz["nnorm"], dtype=np.float32)
+        self._chk(self.L.adg_mlp_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), out.size),
+                  "adg_mlp_get_params")
+        return out
+
+    def mlp_set_params(self, par):
+        par = np.ascontiguousarray(par, dtype=np.float32)
+        self._chk(self.L.adg_mlp_set_params(self.h, par.ctypes.data_as(C.POINTER(C.c_float)), par.size),
+                  "adg_mlp_set_params")
+
+    def mlp_collect(self, iters):
+        n = C.c_long(0)
+        self._chk(self.L.adg_mlp_collect(self.h, iters, C.byref(n)), "adg_mlp_collect")
+        return n.value
+
+    def mlp_dataset(self, start, n):
+        sz = self.mlp_sizes()
+        x = np.zeros((n, sz["nin_p"]), dtype=np.float32)
+        y = np.zeros((n, sz["nout_p"]), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        self._chk(self.L.adg_mlp_dataset(self.h, start, n, x.ctypes.data_as(fp), y.ctypes.data_as(fp)),
+                  "adg_mlp_dataset")
+        return x, y
+
+    def mlp_dataset_clear(self):
+        self._chk(self.L.adg_mlp_dataset_clear(self.h), "adg_mlp_dataset_clear")
+
+    def mlp_fit_normalization(self):
+        self._chk(self.L.adg_mlp_fit_normalization(self.h), "adg_mlp_fit_normalization")
+
+    def mlp_train(self, nsteps, batch=1024, lr=1e-3, seed=7):
+        loss = C.c_double(0)
+        self._chk(self.L.adg_mlp_train(self.h, nsteps, batch, lr, seed, C.byref(loss)), "adg_mlp_train")
+        return loss.value
+
+    def mlp_loss_grad(self, x, y):
+        sz = self.mlp_sizes()
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(sz["npar"], dtype=np.float32)
+        loss = C.c_double(0)
+        fp = C.POINTER(C.c_float)
+        self._chk(self.L.adg_mlp_loss_grad(self.h, x.shape[0], x.ctypes.data_as(fp), y.ctypes.data_as(fp),
+                                           C.byref(loss), g.ctypes.data_as(fp)), "adg_mlp_loss_grad")
+        return loss.value, g
+
+    def mlp_warm_start(self, lev):
+        sz = self.mlp_sizes()
+        out = np.zeros((sz["nout"], self.nslot(lev)))
+        self._chk(self.L.adg_mlp_warm_start(self.h, lev, _d(out)), "adg_mlp_warm_start")
+        return out
+
+    def set_learned(self, on=True, tol=1e-2, max_iters=1):
+        """on: False/0 plain, True/1 learned start with early exit and
+        fallback, 2 diagnostic (exactly max_iters iterations, no check)."""
+        self._chk(self.L.adg_set_learned(self.h, int(on), tol, max_iters), "adg_set_learned")
+
+    def learned_stats(self, reset=False):
+        o = np.zeros(2, dtype=np.uint64)
+        self._chk(self.L.adg_learned_stats(self.h, o.ctypes.data_as(C.POINTER(C.c_ulonglong)), 1 if reset else 0),
+                  "adg_learned_stats")
+        return int(o[0]), int(o[1])

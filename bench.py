@@ -58,6 +58,50 @@ def cpu_baseline(levels, P, nsteps, threads):
     return n, el
 
 
+def uncovered_cells(levels):
+    """Per level: (slot indices of valid cells not covered by the next level,
+    cell volume factor 4^-l), slots of the grown boxes, x fastest."""
+    out = []
+    for l, lev in enumerate(levels):
+        fine = levels[l + 1] if l + 1 < len(levels) else []
+        slots, off = [], 0
+        for lo, hi in lev:
+            g0, g1 = hi[0] - lo[0] + 3, hi[1] - lo[1] + 3
+            for j in range(lo[1], hi[1] + 1):
+                for i in range(lo[0], hi[0] + 1):
+                    cov = any(flo[0] <= 2 * i <= fhi[0] and flo[1] <= 2 * j <= fhi[1] for flo, fhi in fine)
+                    if not cov:
+                        slots.append(off + (j - lo[1] + 1) * g0 + (i - lo[0] + 1))
+            off += g0 * g1
+        out.append((slots, 0.25 ** l))
+    return out
+
+
+def accuracy_vs_fp64(make, levels, nsteps, c_dt, Tend):
+    """Each form from the same initial state for nsteps steps: composite
+    density L1 difference (cell averages, uncovered cells) against the FP64
+    form, relative to the FP64 density L1 norm, and relative drift of the
+    conserved mass and energy over the run."""
+    unc = uncovered_cells(levels)
+    fin = {}
+    drift = {}
+    for form in ("fp64", "fp32", "learned"):
+        ctx = make(form)
+        t0 = ctx.totals()
+        ctx.step_graph(c_dt, Tend, nsteps)
+        t1 = ctx.totals()
+        fin[form] = [ctx.download_Uw(l)[0, 0] for l in range(len(levels))]
+        drift[form] = {"mass": abs(t1[0] - t0[0]) / abs(t0[0]), "energy": abs(t1[-1] - t0[-1]) / abs(t0[-1])}
+        ctx.close()
+    out = {"steps": nsteps}
+    ref_norm = sum(w * sum(abs(fin["fp64"][l][s]) for s in sl) for l, (sl, w) in enumerate(unc))
+    for form in ("fp32", "learned"):
+        num = sum(w * sum(abs(fin[form][l][s] - fin["fp64"][l][s]) for s in sl) for l, (sl, w) in enumerate(unc))
+        out[form] = {"rho_L1_vs_fp64": num / ref_norm, **{f"{k}_drift": v for k, v in drift[form].items()}}
+    out["fp64"] = {f"{k}_drift": v for k, v in drift["fp64"].items()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,8 +109,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", choices=["fp64", "fp32"], default="fp32",
-                    help="form reported as value (both are measured)")
+    ap.add_argument("--form", choices=["fp64", "fp32", "learned"], default="learned",
+                    help="form reported as value (all three are measured)")
+    ap.add_argument("--tol", type=float, default=1e-5,
+                    help="learned form: accept the warm start when max|H_k - H_k-1| <= tol * rho0")
+    ap.add_argument("--max-iters", type=int, default=1, help="learned form: Picard iterations from the warm start")
+    ap.add_argument("--weights", default=os.path.join(PKG, "weights", "kh_p3_2d.f32"))
+    ap.add_argument("--acc-steps", type=int, default=200, help="steps of the accuracy comparison (0 = skip)")
     ap.add_argument("--kernel", type=int, default=0, help="predictor kernel: 0 auto, 1 VALU, 2 MFMA")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
@@ -90,7 +139,10 @@ def main():
     n_valid = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) for lev in levels for lo, hi in lev)
     c_dt, Tend = 0.5, 1.0e9   # no clamp inside the timed window
 
-    def measure(form):
+    import numpy as np
+    weights = np.fromfile(args.weights, dtype=np.float32)
+
+    def make(form):
         prec = pyadg.ADG_FP64 if form == "fp64" else pyadg.ADG_FP32_MFMA
         ctx = pyadg.Context(D, P, 1.4, prec)
         if args.kernel:
@@ -102,15 +154,28 @@ def main():
         ctx.average_down()
         ctx.fill_ghosts()
         ctx.set_dt(c_dt, Tend, 0.0)
+        if form == "learned":
+            ctx.mlp_set_params(weights)
+            ctx.set_learned(True, tol=args.tol, max_iters=args.max_iters)
+        return ctx
+
+    def measure(form):
+        ctx = make(form)
         stream = ctx.stream()
         ctx.step_graph(c_dt, Tend, args.warmup)   # captures the step graph, then warms up
         ctx.sync()
+        if form == "learned":
+            ctx.learned_stats(reset=True)
         e0, e1 = Event(), Event()
         e0.record(stream)
         ctx.step_graph(c_dt, Tend, args.steps)
         e1.record(stream)
         e1.synchronize()
         ms_step = e0.elapsed_ms(e1) / args.steps
+        fell = None
+        if form == "learned":
+            ne, nf = ctx.learned_stats()
+            fell = nf / max(ne, 1)
         # the dominant kernel on its own, same stream, same state
         p0, p1 = Event(), Event()
         kp = max(20, args.steps // 4)
@@ -122,17 +187,19 @@ def main():
         pred_ms = p0.elapsed_ms(p1) / kp
         dof = n_valid * ctx.Q * ctx.Np_st
         out = {"ms_per_step": ms_step, "value": dof / (ms_step * 1e-3), "pred_ms": pred_ms, "dof": dof,
-               "sizes": (ctx.Q, ctx.Np_s, ctx.Np_st, ctx.qMp_st, ctx.qMp_st_bd)}
+               "sizes": (ctx.Q, ctx.Np_s, ctx.Np_st, ctx.qMp_st, ctx.qMp_st_bd), "fallback_fraction": fell}
         ctx.close()
         return out
 
-    res = {f: measure(f) for f in ("fp64", "fp32")}
+    res = {f: measure(f) for f in ("fp64", "fp32", "learned")}
     head = res[args.form]
+    accuracy = accuracy_vs_fp64(make, levels, args.acc_steps, c_dt, Tend) if args.acc_steps > 0 else None
     Q, Ns, Nst, Mst, Mbd = head["sizes"]
     f_step, f_pred = algorithmic_flops(P, D, Q, Ns, Nst, Mst, Mbd)
     achieved = f_pred * n_valid / (head["pred_ms"] * 1e-3)
     peak = PEAK_FP64 if args.form == "fp64" else PEAK_FP32
-    roofline = {"bound": "mfma", "kernel": f"k_predictor_mfma<{'double' if args.form == 'fp64' else 'float'},3,2>",
+    roofline = {"bound": "mfma",
+                "kernel": f"k_predictor_nl<{'double' if args.form == 'fp64' else 'float'},3,2,8> (P Picard iterations)",
                 "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": None, "avg_launch_ms": head["pred_ms"], "flops_per_launch": f_pred * n_valid}
 
@@ -147,13 +214,20 @@ def main():
     line = {
         "metric": METRIC, "value": head["value"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if args.form == "fp64" else "f32+f64", "data": "synthetic",
+        "vs_baseline": None, "data": "synthetic",
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "learned": "f32+f64+bf16"}[args.form],
         "config": {"workload": "kelvin-helmholtz 2D p=3, 3 levels 64^2/128^2/256^2 (bands)",
                    "form": args.form, "elements": n_valid, "dof_per_step": head["dof"],
                    "algorithmic_flops_per_step": f_step * n_valid,
                    "fp64_value": res["fp64"]["value"], "fp64_ms_per_step": res["fp64"]["ms_per_step"],
                    "fp32_value": res["fp32"]["value"], "fp32_ms_per_step": res["fp32"]["ms_per_step"],
+                   "learned_value": res["learned"]["value"],
+                   "learned_ms_per_step": res["learned"]["ms_per_step"],
+                   "learned_fallback_fraction": res["learned"]["fallback_fraction"],
+                   "learned_tol": args.tol, "learned_max_iters": args.max_iters,
                    "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
+                   "learned_over_fp64": res["learned"]["value"] / res["fp64"]["value"],
+                   "accuracy": accuracy,
                    "step": "hipGraph replay of predictor, face flux, corrector, reflux+average down, "
                            "ghost fill+CFL+advance"},
         "roofline": roofline, "cpu_baseline": cpu,

@@ -77,7 +77,10 @@ int adg_download_Uw(adg_ctx* ctx, int lev, double* host);
  *              (v = amp sin(4 pi x) + noise_amp * U[-1,1] per cell)
  *   ic_case 2: uniform state, params {rho, u, v, [w,] p}
  *   ic_case 3: spherical/radial shock tube, params {r0, rho_in, p_in,
- *              rho_out, p_out} centred in the domain */
+ *              rho_out, p_out} centred in the domain
+ *   ic_case 4: Kelvin-Helmholtz shear layer with the seeded multi-mode
+ *              perturbation v = sum_{k=1..8} a_k sin(2 pi k x),
+ *              a_k = amax * U[0,1), params {seed, amax} */
 int adg_project_ic(adg_ctx* ctx, int lev, int ic_case, const double* params);
 
 /* Replaces AmrDG::AMR_FillPatch on every level (AmrDG_Level.tpp:172-224)
@@ -102,8 +105,15 @@ int adg_set_dt_value(adg_ctx* ctx, double Dt);
  * of every level, plus the final modal->nodal evaluation, volume flux
  * (AmrDG::flux, AmrDG.h:1167-1205) and face traces (get_H_from_H_w_bd,
  * AmrDG.tpp:437-493).  lev = -1 runs every level in one launch.
- * mode 0: exactly P Picard iterations (the reference); other modes are
- * reserved.  n_fallback (nullable) receives 0 in mode 0. */
+ * mode 0: exactly P Picard iterations (the reference).
+ * mode 1: learned warm start (no counterpart in the reference): the network
+ *   of adg_mlp_* gives the start H_0 = A0 U_w + Delta, Picard iterations run
+ *   until max|H_k - H_{k-1}| <= tol rho0 (adg_set_learned, rho0 the element's
+ *   mean density, so tol is relative to the state) and
+ *   a wavefront of 4 elements that does not pass falls back to mode 0.
+ *   Node-lane predictor (2D) only.
+ * n_fallback (nullable) receives the number of elements that fell back
+ * (0 in mode 0). */
 int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
 /* Predictor kernel variant: 0 = automatic, 1 = VALU (FP64 only), 2 = MFMA
  * quad-gather (every Q), 3 = MFMA node-lane (Q = 4, i.e. 2D).  Automatic
@@ -174,6 +184,49 @@ int adg_sizes(adg_ctx* ctx, int* out7);
  * Mkbdm Mkbdp quadmat Minv P_cf P_fc P_flux_low P_flux_high).  Returns the
  * number of doubles or a negative code. */
 int adg_get_table(adg_ctx* ctx, const char* name, double* out, int cap);
+
+/* ---- learned warm start of the predictor (no counterpart in the
+ * reference).  A 3-layer GELU MLP, widths nin_p -> 128 -> 128 -> nout_p,
+ * maps x = [U_w/rho0, local CFL number, Dt/dx_d] (nin = Q Np_s + 1 + D) to
+ * y = (H_w* - A0 U_w) / (rho0 max_d Dt/dx_d) (nout = Q Np_st), H_w* the
+ * converged predictor; BF16 MFMA products with FP32 accumulation, trained on
+ * the device by its own forward / backward kernels with Adam. */
+/* out[0..6] = nin, nin_p, nh, nout, nout_p, number of weights and biases,
+ * number of normalisation values (mean/deviation of x and y). */
+int adg_mlp_sizes(adg_ctx* ctx, int* out7);
+/* Glorot-uniform weights (seeded), zero biases, identity normalisation,
+ * Adam state cleared. */
+int adg_mlp_init(adg_ctx* ctx, unsigned long long seed);
+/* Flat parameters [W1 nh x nin_p][b1][W2 nh x nh][b2][W3 nout_p x nh][b3]
+ * followed by [mu_x nin_p][sd_x nin_p][mu_y nout_p][sd_y nout_p]. */
+int adg_mlp_get_params(adg_ctx* ctx, float* out, long cap);
+int adg_mlp_set_params(adg_ctx* ctx, const float* in, long n);
+/* Runs the predictor with `iters` Picard iterations on the current state and
+ * appends (x, y) of every valid element to the device dataset; *n_total
+ * (nullable) gets the dataset size.  Leaves Vol/Tr of that predictor. */
+int adg_mlp_collect(adg_ctx* ctx, int iters, long* n_total);
+/* Copies dataset rows [start, start + n) (raw, unstandardised) to the host;
+ * x: n x nin_p, y: n x nout_p (either nullable). */
+int adg_mlp_dataset(adg_ctx* ctx, long start, long n, float* x, float* y);
+int adg_mlp_dataset_clear(adg_ctx* ctx);
+/* Column mean / deviation of the dataset become the normalisation. */
+int adg_mlp_fit_normalization(adg_ctx* ctx);
+/* nsteps Adam steps (lr, beta 0.9 / 0.999) on random minibatches of `batch`
+ * rows (a multiple of 16); *loss_out = mean loss of the last steps. */
+int adg_mlp_train(adg_ctx* ctx, int nsteps, int batch, double lr, unsigned long long seed, double* loss_out);
+/* Loss and gradient (flat weight layout) of n standardised rows x
+ * (n x nin_p), y (n x nout_p), n a multiple of 16. */
+int adg_mlp_loss_grad(adg_ctx* ctx, int n, const float* x, const float* y, double* loss, float* grad);
+/* Warm-start increments Delta [nout][slot] of level lev (all its slots). */
+int adg_mlp_warm_start(adg_ctx* ctx, int lev, double* out);
+/* Predictor mode used by adg_step / adg_ader / adg_step_graph:
+ * on = 1 -> mode 1 with tolerance tol and at most max_iters Picard
+ * iterations from the learned start; on = 2 -> diagnostic: exactly
+ * max_iters iterations from the learned start, no check, no fallback. */
+int adg_set_learned(adg_ctx* ctx, int on, double tol, int max_iters);
+/* out[0..1] = elements run through mode 1, elements that fell back, since
+ * the last reset; reset != 0 clears them. */
+int adg_learned_stats(adg_ctx* ctx, unsigned long long* out2, int reset);
 
 const char* adg_last_error(adg_ctx* ctx);
 void adg_destroy(adg_ctx* ctx);

@@ -992,6 +992,12 @@ static double cell_noise(uint64_t seed, int lev, const int* iv) {
     k = splitmix(k ^ (uint64_t)(uint32_t)iv[2]);
     return (double)(k >> 11) * (1.0 / 9007199254740992.0);
 }
+// uniform [0,1) per (seed, perturbation mode k)
+static double mode_noise(uint64_t seed, int k) {
+    uint64_t h = splitmix(seed ^ 0x6b68ull);
+    h = splitmix(h ^ (uint64_t)(uint32_t)k);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
 
 static void ic_state(int D, int ic_case, const double* prm, double gamma, const double* x,
                      int lev, const int* iv, double* u) {
@@ -1015,6 +1021,16 @@ static void ic_state(int D, int ic_case, const double* prm, double gamma, const 
         bool in = std::fabs(y - 0.5) < 0.25;
         double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
         double u2 = amp * std::sin(4.0 * M_PI * x[0]) + namp * (2.0 * cell_noise(seed, lev, iv) - 1.0);
+        double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+        u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
+    } else if (ic_case == 4) {
+        // KH with v = sum_{k=1..8} a_k sin(2 pi k x), a_k = amax * r_k
+        uint64_t seed = (uint64_t)prm[0];
+        double amax = prm[1];
+        bool in = std::fabs(x[1] - 0.5) < 0.25;
+        double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
+        double u2 = 0.0;
+        for (int k = 1; k <= 8; ++k) u2 += amax * mode_noise(seed, k) * std::sin(2.0 * M_PI * k * x[0]);
         double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
     } else if (ic_case == 3) {
@@ -1570,4 +1586,68 @@ extern "C" void orc_totals(const orc_tables* t, const orc_hier* h, double** Uw, 
             for (int q = 0; q < Q; ++q) out[q] += vol * Uw[l][((long)q * Ns + 0) * lv.nslot + s];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Learned warm start: predictor with a chosen iteration count, and the
+// feature / target definition of the surrogate (no counterpart in the
+// reference; the product's collection and inference kernels compute the same
+// quantities).
+extern "C" void orc_predictor_iters(const orc_tables* t, const orc_hier* h, int lev, const double* Uw,
+                                    double Dt, double gamma, int iters, double* Hw_out) {
+    const long ns = h->L[lev].nslot;
+    const int Q = t->Q, M = t->qMp_st, D = t->D;
+    std::vector<double> H((size_t)Q * M * ns), F((size_t)D * Q * M * ns);
+    set_predictor(t, ns, Uw, Hw_out);
+    for (int it = 0; it < iters; ++it) {
+        get_H(t, ns, Hw_out, H.data());
+        for (int d = 0; d < D; ++d) flux_vol(t, ns, H.data(), F.data() + (size_t)d * Q * M * ns, d, gamma);
+        update_H_w(t, ns, Uw, F.data(), Hw_out, h->L[lev].dx, Dt);
+    }
+}
+
+extern "C" void orc_mlp_dims(const orc_tables* t, int* out) {
+    const int nin = t->Q * t->Np_s + 1 + t->D, nout = t->Q * t->Np_st;
+    out[0] = nin;
+    out[1] = 16 * ((nin + 15) / 16);
+    out[2] = 128;
+    out[3] = nout;
+    out[4] = 16 * ((nout + 15) / 16);
+}
+
+// x = [U[q][k] / rho0 (q-major), nu, tau_0..tau_{D-1}], nu the local CFL
+// number sum_d tau_d (|u_d| + c) of the cell-average state.
+extern "C" void orc_mlp_features(const orc_tables* t, const double* U, const double* tau, double gamma,
+                                 double* x) {
+    const int Q = t->Q, Ns = t->Np_s, D = t->D;
+    const double rho0 = U[0];
+    for (int q = 0; q < Q; ++q)
+        for (int k = 0; k < Ns; ++k) x[q * Ns + k] = U[q * Ns + k] / rho0;
+    double ke = 0.0;
+    for (int d = 0; d < D; ++d) ke += U[(d + 1) * Ns] * U[(d + 1) * Ns];
+    const double prs = (gamma - 1.0) * (U[(D + 1) * Ns] - 0.5 * ke / rho0);
+    const double c = std::sqrt(gamma * prs / rho0);
+    double nu = 0.0;
+    for (int d = 0; d < D; ++d) nu += tau[d] * (std::fabs(U[(d + 1) * Ns] / rho0) + c);
+    x[Q * Ns] = nu;
+    for (int d = 0; d < D; ++d) x[Q * Ns + 1 + d] = tau[d];
+}
+
+// y = (H - A0 U) / (rho0 max_d tau_d), A0 = Mk_h_w^-1 Mk_pred, q-major.
+extern "C" void orc_mlp_target(const orc_tables* t, const double* U, const double* Hst, const double* tau,
+                               double* y) {
+    const int Q = t->Q, Ns = t->Np_s, Nst = t->Np_st, D = t->D;
+    double tmax = 0.0;
+    for (int d = 0; d < D; ++d) tmax = std::max(tmax, tau[d]);
+    const double sc = 1.0 / (U[0] * tmax);
+    for (int q = 0; q < Q; ++q)
+        for (int n = 0; n < Nst; ++n) {
+            double c0 = 0.0;
+            for (int j = 0; j < Nst; ++j) {
+                double r = 0.0;
+                for (int m = 0; m < Ns; ++m) r += t->Mk_pred(j, m) * U[q * Ns + m];
+                c0 += t->Mk_h_w_inv(n, j) * r;
+            }
+            y[q * Nst + n] = (Hst[q * Nst + n] - c0) * sc;
+        }
 }

@@ -69,6 +69,8 @@ void        orc_hier_slot_centre(const orc_hier*, int lev, int slot, double* x);
  * case 1: Kelvin-Helmholtz (params: amp, noise_amp, seed)  [no reference IC]
  * case 2: uniform state (params: rho, u, v, [w,] p)
  * case 3: radial shock tube (params: r0, rho_in, p_in, rho_out, p_out, centre[D])
+ * case 4: Kelvin-Helmholtz with v = sum_{k=1..8} a_k sin(2 pi k x),
+ *         a_k = amax * r_k, r_k seeded uniform (params: seed, amax)  [no reference IC]
  * Fills every slot (ghosts too, as AmrDG::set_initial_condition on growntilebox). */
 void        orc_project_ic(const orc_tables*, const orc_hier*, int lev, int ic_case,
                            const double* params, double gamma, double* Uw);
@@ -111,6 +113,31 @@ double      orc_lp_error(const orc_tables*, const orc_hier*, double** Uw, int q,
                          double t);
 /* Conserved totals: sum over uncovered valid cells of vol * Uw[q][0] */
 void        orc_totals(const orc_tables*, const orc_hier*, double** Uw, double* out);
+
+/* ---- learned warm start (no counterpart in the reference) ---- */
+/* Predictor with `iters` Picard iterations (iters = P is orc_predictor). */
+void        orc_predictor_iters(const orc_tables*, const orc_hier*, int lev, const double* Uw,
+                                double Dt, double gamma, int iters, double* Hw_out);
+/* MLP widths [nin, nin_p, nh, nout, nout_p]: nin = Q Np_s + 1 + D inputs,
+ * nout = Q Np_st outputs, _p padded to a multiple of 16, nh = 128. */
+void        orc_mlp_dims(const orc_tables*, int* out5);
+/* Features of one element, U [Q][Np_s], tau[d] = Dt/dx_d:
+ * x = [U/rho0 (q-major), nu = sum_d tau_d (|u_d| + c) at the cell average,
+ * tau_0 .. tau_{D-1}] */
+void        orc_mlp_features(const orc_tables*, const double* U, const double* tau, double gamma,
+                             double* x_out);
+/* Target of one element: y = (H - A0 U) / (rho0 max_d tau_d), H [Q][Np_st],
+ * A0 = Mk_h_w^-1 Mk_pred. */
+void        orc_mlp_target(const orc_tables*, const double* U, const double* H, const double* tau,
+                           double* y_out);
+/* Loss (mean squared error over n rows and the first nout columns) and its
+ * gradient (nullable) of the MLP with flat float params
+ * [W1 nh x nin_p][b1][W2 nh x nh][b2][W3 nout_p x nh][b3]; x n x nin_p,
+ * y n x nout_p.  bf16 = 1 rounds every matrix-product operand to bfloat16
+ * as the MFMA kernels do; accumulation in double.  ypred (nullable) gets the
+ * n x nout_p outputs. */
+double      orc_mlp_loss_grad(int nin_p, int nh, int nout_p, int nout, int n, const float* params,
+                              const float* x, const float* y, int bf16, double* grad, double* ypred);
 
 #ifdef __cplusplus
 }

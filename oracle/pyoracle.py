@@ -63,6 +63,14 @@ def lib():
         L.orc_lp_error.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), C.c_int, C.c_int, C.c_int,
                                    dptr, C.c_double, C.c_double]
         L.orc_totals.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), dptr]
+        fptr = C.POINTER(C.c_float)
+        L.orc_predictor_iters.argtypes = [C.c_void_p, C.c_void_p, C.c_int, dptr, C.c_double, C.c_double,
+                                          C.c_int, dptr]
+        L.orc_mlp_dims.argtypes = [C.c_void_p, iptr]
+        L.orc_mlp_features.argtypes = [C.c_void_p, dptr, dptr, C.c_double, dptr]
+        L.orc_mlp_target.argtypes = [C.c_void_p, dptr, dptr, dptr, dptr]
+        L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_mlp_loss_grad.argtypes = [C.c_int] * 5 + [fptr, fptr, fptr, C.c_int, dptr, dptr]
         _LIB = L
     return _LIB
 
@@ -238,3 +246,51 @@ def uniform_boxes(n, D, max_grid=None):
         hi = tuple(min(l + mg, n) - 1 for l in lo)
         out.append((lo, hi))
     return out
+
+
+# ---- learned warm start (no counterpart in the reference) ----
+def predictor_iters(T, H, lev, Uw_l, Dt, iters, gamma=1.4):
+    out = np.zeros((T.Q, T.Np_st, H.nslot(lev)))
+    lib().orc_predictor_iters(T.h, H.h, lev, _d(np.ascontiguousarray(Uw_l)), Dt, gamma, iters, _d(out))
+    return out
+
+
+def mlp_dims(T):
+    o = np.zeros(5, dtype=np.int32)
+    lib().orc_mlp_dims(T.h, _i(o))
+    return [int(v) for v in o]  # nin, nin_p, nh, nout, nout_p
+
+
+def mlp_features(T, U, tau, gamma=1.4):
+    """U: [Q][Np_s] of one element, tau: [D] = Dt/dx_d."""
+    nin = mlp_dims(T)[0]
+    x = np.zeros(nin)
+    lib().orc_mlp_features(T.h, _d(np.ascontiguousarray(U, dtype=np.float64)),
+                           _d(np.ascontiguousarray(tau, dtype=np.float64)), gamma, _d(x))
+    return x
+
+
+def mlp_target(T, U, Hst, tau):
+    nout = mlp_dims(T)[3]
+    y = np.zeros(nout)
+    lib().orc_mlp_target(T.h, _d(np.ascontiguousarray(U, dtype=np.float64)),
+                         _d(np.ascontiguousarray(Hst, dtype=np.float64)),
+                         _d(np.ascontiguousarray(tau, dtype=np.float64)), _d(y))
+    return y
+
+
+def mlp_loss_grad(dims, params, x, y, bf16=True, want_grad=True):
+    """dims = (nin_p, nh, nout_p, nout); params flat float32; x [n, nin_p],
+    y [n, nout_p] float32.  Returns (loss, grad float64 or None, ypred)."""
+    nin_p, nh, nout_p, nout = dims
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    n = x.shape[0]
+    grad = np.zeros(params.size) if want_grad else None
+    yp = np.zeros((n, nout_p))
+    fptr = C.POINTER(C.c_float)
+    loss = lib().orc_mlp_loss_grad(nin_p, nh, nout_p, nout, n, params.ctypes.data_as(fptr),
+                                   x.ctypes.data_as(fptr), y.ctypes.data_as(fptr), 1 if bf16 else 0,
+                                   _d(grad) if want_grad else None, _d(yp))
+    return loss, grad, yp
