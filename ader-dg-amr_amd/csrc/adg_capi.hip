@@ -481,7 +481,7 @@ int adg_download_Uw(adg_ctx* c, int lev, double* host) {
 int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
     if (!c) return ADG_E_ARG;
     if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 4)
+    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 5)
         return fail(c, ADG_E_ARG, "bad level, case or params");
     if (ic_case <= 1 && c->D != 2) return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
     std::vector<double> host;
@@ -1478,6 +1478,22 @@ int adg_get_table(adg_ctx* c, const char* name, double* out, int cap) {
     if (out)
         for (size_t i = 0; i < v->size() && (int)i < cap; ++i) out[i] = (*v)[i];
     return (int)v->size();
+}
+
+// Host-only: a named element table of degree P in D dimensions, built on
+// the host exactly as adg_create builds it (no device work).
+int adg_host_table(int P, int D, const char* name, double* out, int cap) {
+    if (!supported(P, D) || !name) return ADG_E_ARG;
+    try {
+        const adg::ElementTables t = adg::build_element_tables(P, D);
+        const std::vector<double>* v = adg::table_by_name(t, name);
+        if (!v) return ADG_E_ARG;
+        if (out)
+            for (size_t i = 0; i < v->size() && (int)i < cap; ++i) out[i] = (*v)[i];
+        return (int)v->size();
+    } catch (...) {
+        return ADG_E_ARG;
+    }
 }
 
 // Host-only inspection of the hierarchy lists built for a box layout (no

@@ -13,6 +13,9 @@
 //  3  radial / spherical shock tube centred in the domain (2D: the
 //     reference's radial_shock_tube, Compressible_Euler.h:336-357, with the
 //     radius and the two states as parameters; 3D has no reference IC);
+//  5  smooth periodic density wave rho = 1 + amp prod_d sin(2 pi x_d / L_d),
+//     velocity (1, 0.5, 0.25), pressure p0 (params: amp, p0; no counterpart
+//     in the reference);
 //  4  Kelvin-Helmholtz shear layer as in 1 with the seeded multi-mode
 //     perturbation v = sum_{k=1..8} a_k sin(2 pi k x), a_k = amax * r_k,
 //     r_k a seeded uniform number (no counterpart in the reference).
@@ -83,6 +86,16 @@ void pointwise_state(int D, int ic_case, const double* prm, double gamma, const 
         for (int k = 1; k <= 8; ++k) u2 += amax * mode_uniform(seed, k) * std::sin(2.0 * M_PI * k * x[0]);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
         u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+    } else if (ic_case == 5) {
+        // smooth periodic density wave advected by a uniform flow
+        double prod = 1.0;
+        for (int d = 0; d < D; ++d) prod *= std::sin(2.0 * M_PI * (x[d] - plo[d]) / (phi[d] - plo[d]));
+        const double rho = 1.0 + prm[0] * prod, prs = prm[1];
+        const double vel[3] = {1.0, 0.5, 0.25};
+        double k = 0.0;
+        u[0] = rho;
+        for (int d = 0; d < D; ++d) { u[d + 1] = rho * vel[d]; k += vel[d] * vel[d]; }
+        u[D + 1] = prs / (gamma - 1.0) + 0.5 * rho * k;
     } else if (ic_case == 2) {
         const double rho = prm[0];
         const double vel[3] = {prm[1], prm[2], D > 2 ? prm[3] : 0.0};

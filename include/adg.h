@@ -80,7 +80,10 @@ int adg_download_Uw(adg_ctx* ctx, int lev, double* host);
  *              rho_out, p_out} centred in the domain
  *   ic_case 4: Kelvin-Helmholtz shear layer with the seeded multi-mode
  *              perturbation v = sum_{k=1..8} a_k sin(2 pi k x),
- *              a_k = amax * U[0,1), params {seed, amax} */
+ *              a_k = amax * U[0,1), params {seed, amax}
+ *   ic_case 5: smooth periodic density wave rho = 1 + amp prod_d
+ *              sin(2 pi x_d / L_d), velocity (1, 0.5, 0.25), pressure p0,
+ *              params {amp, p0} */
 int adg_project_ic(adg_ctx* ctx, int lev, int ic_case, const double* params);
 
 /* Replaces AmrDG::AMR_FillPatch on every level (AmrDG_Level.tpp:172-224)

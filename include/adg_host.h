@@ -20,6 +20,11 @@ extern "C" {
 int adg_host_inspect(int dim, const int* dom_n, const int* periodic, int nlev, const int* npatch,
                      const int* boxes, int lev, int d, int kind, int* out, int cap);
 
+/* A named element table (the names of adg_get_table) of degree P in D
+ * dimensions, built on the host as adg_create builds it.  Returns the number
+ * of doubles or < 0. */
+int adg_host_table(int P, int D, const char* name, double* out, int cap);
+
 /* Runs one 16x16x4 MFMA on the current device and returns the 64x4
  * accumulator (prec 0: FP64, 1: FP32; mode 0 probes A rows, mode 1 B
  * columns).  Needs a GPU. */

@@ -1033,6 +1033,16 @@ static void ic_state(int D, int ic_case, const double* prm, double gamma, const 
         for (int k = 1; k <= 8; ++k) u2 += amax * mode_noise(seed, k) * std::sin(2.0 * M_PI * k * x[0]);
         double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
+    } else if (ic_case == 5) {
+        // smooth periodic density wave (unit-cube domain) advected by (1, 0.5, 0.25)
+        double prod = 1.0;
+        for (int d = 0; d < D; ++d) prod *= std::sin(2.0 * M_PI * x[d]);
+        double rho = 1.0 + prm[0] * prod, prs = prm[1];
+        const double vel[3] = {1.0, 0.5, 0.25};
+        double k = 0.0;
+        u[0] = rho;
+        for (int d = 0; d < D; ++d) { u[d + 1] = rho * vel[d]; k += vel[d] * vel[d]; }
+        u[D + 1] = prs / (gamma - 1.0) + 0.5 * rho * k;
     } else if (ic_case == 3) {
         // radial / spherical shock tube centred in the unit-cell domain
         // (2D form of Compressible_Euler.h:336-357 with parameters)

@@ -69,6 +69,8 @@ void        orc_hier_slot_centre(const orc_hier*, int lev, int slot, double* x);
  * case 1: Kelvin-Helmholtz (params: amp, noise_amp, seed)  [no reference IC]
  * case 2: uniform state (params: rho, u, v, [w,] p)
  * case 3: radial shock tube (params: r0, rho_in, p_in, rho_out, p_out, centre[D])
+ * case 5: smooth density wave rho = 1 + amp prod_d sin(2 pi x_d) on the unit
+ *         cube, velocity (1, 0.5, 0.25), pressure p0 (params: amp, p0)
  * case 4: Kelvin-Helmholtz with v = sum_{k=1..8} a_k sin(2 pi k x),
  *         a_k = amax * r_k, r_k seeded uniform (params: seed, amax)  [no reference IC]
  * Fills every slot (ghosts too, as AmrDG::set_initial_condition on growntilebox). */

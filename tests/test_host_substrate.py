@@ -98,3 +98,27 @@ def test_bad_layouts_rejected(adg):
         inspect(adg, 2, [8, 8], [tiles(8, 8), [((3, 2), (6, 5))]], 0, 0, 3)   # fine box not ratio-aligned
     with pytest.raises(AssertionError):
         inspect(adg, 2, [8, 8], [[((0, 0), (3, 7))], []], 0, 0, 3)            # level 0 incomplete -> ghosts fail
+
+
+TABLES = ["nodes", "weights", "V", "Vinv", "Mk_h_w", "Mk_h_w_inv", "Mk_pred", "Sk_predVinv", "Mk_corr", "Sk_corr",
+          "Mkbdm", "Mkbdp", "quadmat", "Minv", "P_cf", "P_fc", "P_flux_low", "P_flux_high"]
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_product_tables_match_oracle(adg, orc, P, D):
+    """The element tables the library builds (host side of adg_create) equal
+    the oracle's restatement of AmrDG_ElementMatrix.tpp / AmrDG_Interpolater.tpp."""
+    lib = adg.lib()
+    fn = lib.adg_host_table
+    fn.argtypes = [C.c_int, C.c_int, C.c_char_p, C.POINTER(C.c_double), C.c_int]
+    fn.restype = C.c_int
+    T = orc.Tables(P, D)
+    for name in TABLES:
+        ref = T.get(name)
+        n = fn(P, D, name.encode(), None, 0)
+        assert n == ref.size, (name, n, ref.size)
+        got = np.zeros(n)
+        fn(P, D, name.encode(), got.ctypes.data_as(C.POINTER(C.c_double)), n)
+        scale = max(np.abs(ref).max(), 1.0)
+        assert np.abs(got - ref.ravel()).max() <= 1e-12 * scale, name
