@@ -7,6 +7,7 @@
 // whole step can be replayed without host round trips.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -70,6 +71,7 @@ struct adg_ctx {
     uint8_t *mom_d = nullptr, *mom_kind = nullptr, *mom_sel = nullptr, *mom_lev = nullptr, *rc_lev = nullptr;
     double* re_coef = nullptr;
     int* uncovered = nullptr;
+    double* unc_centre = nullptr;   // cell centres of the uncovered cells [n][3]
     adg::TimeState* ts = nullptr;
     bool keep_hw = false;
     int iters_override = 0;    // > 0: Picard iterations of the next plain predictor launches (collection)
@@ -238,6 +240,28 @@ int pack_tables(adg_ctx* c) {
     for (int d = 0; d < t.D; ++d)
         for (int k = 0; k < t.Np_s; ++k) sgn[(size_t)d * t.Np_s + k] = (t.mi_s[k][d] % 2) ? -1.0 : 1.0;
     c->to.sgn = add(sgn);
+    // spatial basis at the spatial nodes, node weights and coordinates (Lp norms)
+    {
+        const int Ms = t.qMp_s, Ns = t.Np_s, D = t.D;
+        std::vector<double> vs((size_t)Ms * Ns), ws((size_t)Ms), xs((size_t)Ms * D);
+        for (int m = 0; m < Ms; ++m) {
+            double w = 1.0;
+            for (int d = 0; d < D; ++d) {
+                const int k = t.node_idx(m, d, D);
+                xs[(size_t)m * D + d] = t.nodes[k];
+                w *= t.weights[k];
+            }
+            ws[m] = w;
+            for (int n = 0; n < Ns; ++n) {
+                double ph = 1.0;
+                for (int d = 0; d < D; ++d) ph *= std::legendre(t.mi_s[n][d], xs[(size_t)m * D + d]);
+                vs[(size_t)m * Ns + n] = ph;
+            }
+        }
+        c->to.Vs = add(vs);
+        c->to.ws = add(ws);
+        c->to.xs = add(xs);
+    }
     c->to.total = (int)buf.size();
     void* p = nullptr;
     HIPCHK(c, hipMalloc(&p, buf.size() * sizeof(double)));
@@ -401,6 +425,21 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
     if ((rc = dev_alloc_bytes(c, (void**)&c->warm, (size_t)Q * t.Np_st * NS * sizeof(float))) != ADG_OK) return rc;
+    {
+        std::vector<double> xc(H.uncovered.size() * 3, 0.0);
+        for (size_t i = 0; i < H.uncovered.size(); ++i) {
+            const long s = H.uncovered[i];
+            for (const adg::PatchLayout& pt : H.patch) {
+                const long loc = s - pt.off;
+                if (loc < 0 || loc >= (long)pt.g[0] * pt.g[1] * pt.g[2]) continue;
+                const long r[3] = {loc % pt.g[0], (loc / pt.g[0]) % pt.g[1], loc / ((long)pt.g[0] * pt.g[1])};
+                for (int d = 0; d < D; ++d)
+                    xc[i * 3 + d] = H.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * H.lev[pt.lev].dx[d];
+                break;
+            }
+        }
+        if ((rc = dev_upload(c, &c->unc_centre, xc)) != ADG_OK) return rc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->levels_set = true;
     return prepare_kernels(c);
@@ -1453,6 +1492,49 @@ int adg_conserved_totals(adg_ctx* c, double* totals) {
         for (unsigned b = 0; b < nb; ++b) s += h[(size_t)q * nb + b];
         totals[q] = s;
     }
+    return ADG_OK;
+}
+
+int adg_lp_error(adg_ctx* c, int q, int p, int ic_case, const double* params, double t, double* out) {
+    if (!c || !c->levels_set || !out || !params) return fail(c, ADG_E_STATE, "levels not set");
+    if (q < 0 || q >= c->Q || p < 1 || p > 2) return fail(c, ADG_E_ARG, "q out of range or p not 1 or 2");
+    if (ic_case != 0 && ic_case != 2 && ic_case != 5) return fail(c, ADG_E_UNSUP, "no exact solution for this case");
+    if (ic_case == 0 && c->D != 2) return fail(c, ADG_E_UNSUP, "the vortex is two-dimensional");
+    const long n = (long)c->H.uncovered.size();
+    const unsigned nb = nblocks(n);
+    adg::LpArgs a{};
+    a.tab = c->d_tab;
+    a.to = c->to;
+    a.Uw = c->Uw;
+    a.list = c->uncovered;
+    a.centre = c->unc_centre;
+    a.slot_lev = c->slot_lev;
+    a.partial = c->partial;
+    a.geo = c->geo;
+    a.NS = c->H.NS;
+    a.n = n;
+    a.q = q;
+    a.p = p;
+    a.ic = ic_case;
+    for (int i = 0; i < 5; ++i) a.prm[i] = params[i];
+    a.gamma = c->gamma;
+    a.t = t;
+    for (int d = 0; d < 3; ++d) { a.plo[d] = c->H.plo[d]; a.phi[d] = c->H.phi[d]; }
+#define L_LP(PP, DD) hipLaunchKernelGGL((adg::k_lp_error<PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_LP);
+#undef L_LP
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h(nb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0, vamr = 0.0;
+    for (unsigned b = 0; b < nb; ++b) tot += h[b];
+    for (int l = 0; l < c->H.nlev; ++l) {
+        long nu = 0;
+        for (int s : c->H.uncovered) nu += c->H.slot_lev[s] == l;
+        vamr += c->geo.vol[l] * (double)nu;
+    }
+    *out = std::pow(tot / vamr, 1.0 / (double)p);
     return ADG_OK;
 }
 

@@ -206,22 +206,32 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                 if (iv[d] < P.lo[d] || iv[d] > P.hi[d]) inside = false;
             if (inside) return;
             long s = h.slot_of(p, iv);
-            long src = find(l, iv);
+            // outside a non-periodic boundary: transmissive (outflow) ghost,
+            // the mirror image of the cell across the boundary face, whose
+            // modes change sign by (-1)^{k_d} in each mirrored direction d
+            int mv[3] = {iv[0], iv[1], D > 2 ? iv[2] : 0}, mask = 0;
+            for (int d = 0; d < D; ++d) {
+                const int n = h.lev[l].dom[d];
+                if (h.per[d]) continue;
+                if (iv[d] < 0) { mv[d] = -1 - iv[d]; mask |= 1 << d; }
+                if (iv[d] >= n) { mv[d] = 2 * n - 1 - iv[d]; mask |= 1 << d; }
+            }
+            long src = find(l, mv);
             if (src >= 0) {
                 h.g_slot.push_back((int)s); h.g_src.push_back((int)src);
-                h.g_kind.push_back(0); h.g_child.push_back(0);
+                h.g_kind.push_back((uint8_t)(0 | (mask << 2))); h.g_child.push_back(0);
                 return;
             }
-            if (l == 0) throw std::runtime_error("non-periodic physical boundaries are not supported");
+            if (l == 0) throw std::runtime_error("level-0 ghost cell without a source");
             int civ[3] = {0, 0, 0}, child = 0;
             for (int d = 0; d < D; ++d) {
-                civ[d] = floordiv2(iv[d]);
-                child += (iv[d] - 2 * civ[d]) << (D - 1 - d);
+                civ[d] = floordiv2(mv[d]);
+                child += (mv[d] - 2 * civ[d]) << (D - 1 - d);
             }
             long cs = find(l - 1, civ);
             if (cs < 0) throw std::runtime_error("coarse parent of a fine ghost cell is missing");
             h.g_slot.push_back((int)s); h.g_src.push_back((int)cs);
-            h.g_kind.push_back(1); h.g_child.push_back((uint8_t)child);
+            h.g_kind.push_back((uint8_t)(1 | (mask << 2))); h.g_child.push_back((uint8_t)child);
         });
     }
     // ---- average down groups

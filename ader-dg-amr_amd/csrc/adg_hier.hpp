@@ -54,7 +54,8 @@ struct Hierarchy {
     std::vector<int> face_lr[3];    // per face: left slot, right slot (2 ints)
     std::vector<uint8_t> face_lev[3];
 
-    // ghost sources: kind 0 copy, kind 1 L2 projection from coarse parent
+    // ghost sources: kind & 3 = 0 copy, 1 L2 projection from coarse parent;
+    // kind >> 2 = mask of mirrored directions (transmissive boundary ghosts)
     std::vector<int> g_slot, g_src;
     std::vector<uint8_t> g_kind, g_child;
 

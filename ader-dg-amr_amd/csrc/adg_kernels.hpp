@@ -74,7 +74,9 @@ __host__ __device__ __forceinline__ long vol_idx(int k, int q, long slot, long N
 
 // Offsets (in doubles) of every table inside the device table buffer.
 struct TabOff {
-    int A0, B, V, Skc, Vbd, Mkbdm, Mkbdp, Mkc, Minv, Pcf, Pfc, Pfl, Pfh, phi0, sgn, total;
+    int A0, B, V, Skc, Vbd, Mkbdm, Mkbdp, Mkc, Minv, Pcf, Pfc, Pfl, Pfh, phi0, sgn;
+    int Vs, ws, xs;   // spatial basis at the spatial nodes [qMp_s][Np_s], node weights, node coordinates [qMp_s][D]
+    int total;
 };
 // Offsets (in 64-entry fragments) of the MFMA A-operand fragment sets.
 struct FragOff {
@@ -1236,15 +1238,22 @@ __device__ __forceinline__ void ghost_item(const double* __restrict__ tab, const
     const long g = t - (long)qn * nghost;
     const int q = qn / Ns, n = qn - q * Ns;
     const long s = g_slot[g], src = g_src[g];
-    if (g_kind[g] == 0) {
-        Uw[((long)q * Ns + n) * NS + s] = Uw[((long)q * Ns + n) * NS + src];
-        return;
-    }
-    const double* __restrict__ Pcf = tab + to.Pcf + (long)g_child[g] * Ns * Ns;
-    double v = 0.0;
+    const int kind = g_kind[g], mask = kind >> 2;
+    double v;
+    if ((kind & 3) == 0) {
+        v = Uw[((long)q * Ns + n) * NS + src];
+    } else {
+        const double* __restrict__ Pcf = tab + to.Pcf + (long)g_child[g] * Ns * Ns;
+        double a = 0.0;
 #pragma unroll
-    for (int k = 0; k < Ns; ++k) v = fma(Pcf[n * Ns + k], Uw[((long)q * Ns + k) * NS + src], v);
-    Uw[((long)q * Ns + n) * NS + s] = tab[to.Minv + n] * v;
+        for (int k = 0; k < Ns; ++k) a = fma(Pcf[n * Ns + k], Uw[((long)q * Ns + k) * NS + src], a);
+        v = tab[to.Minv + n] * a;
+    }
+    // transmissive boundary: mirror image, mode n times (-1)^{k_d} per mirrored d
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (mask & (1 << d)) v *= tab[to.sgn + d * Ns + n];
+    Uw[((long)q * Ns + n) * NS + s] = v;
 }
 
 template <typename T, int P, int D>
@@ -1352,6 +1361,106 @@ __global__ void __launch_bounds__(256) k_totals(TotArgs a) {
 }  // namespace adg
 
 namespace adg {
+// ---------------------------------------------------------------------------
+// Composite Lp error of one conserved variable over the uncovered cells
+// (LpNorm_DG_AMR, AmrDG.h:1399-1578): (P+1)^D Gauss-Legendre points per cell,
+// cell weight vol/2^D, against the exact solution at time t.  Exact
+// solutions: the isentropic vortex (initial condition 0) and the density
+// wave (5) translated by their uniform velocity on the periodic domain, and
+// the uniform state (2).
+struct LpArgs {
+    const double* __restrict__ tab;
+    TabOff to;
+    const double* __restrict__ Uw;
+    const int* __restrict__ list;        // uncovered cells
+    const double* __restrict__ centre;   // their cell centres [n][3]
+    const uint8_t* __restrict__ slot_lev;
+    double* __restrict__ partial;
+    LevelGeo geo;
+    long NS, n;
+    int q, p, ic;
+    double prm[8];
+    double gamma, t;
+    double plo[3], phi[3];
+};
+
+template <int D>
+__device__ void exact_state(const LpArgs& a, const double* xin, double* u) {
+    double x[3] = {xin[0], xin[1], D > 2 ? xin[2] : 0.0};
+    const double vel5[3] = {1.0, 0.5, 0.25};
+    for (int d = 0; d < D; ++d) {
+        const double v = a.ic == 0 ? (d == 0 ? a.prm[3] : (d == 1 ? a.prm[4] : 0.0)) : (a.ic == 5 ? vel5[d] : 0.0);
+        const double L = a.phi[d] - a.plo[d];
+        double sh = x[d] - v * a.t - a.plo[d];
+        sh -= L * floor(sh / L);
+        x[d] = a.plo[d] + sh;
+    }
+    for (int q = 0; q < 5; ++q) u[q] = 0.0;
+    const double g = a.gamma;
+    if (a.ic == 0) {
+        const double beta = a.prm[0], xc = a.prm[1], yc = a.prm[2];
+        const double r2 = (x[0] - xc) * (x[0] - xc) + (x[1] - yc) * (x[1] - yc);
+        const double rho = pow(1.0 - ((g - 1.0) * beta * beta / (8.0 * g * M_PI * M_PI)) * exp(1.0 - r2), 1.0 / (g - 1.0));
+        const double prs = pow(rho, g);
+        const double f = (beta / (2.0 * M_PI)) * exp((1.0 - r2) / 2.0);
+        const double u1 = a.prm[3] - (x[1] - yc) * f, u2 = a.prm[4] + (x[0] - xc) * f;
+        u[0] = rho;
+        u[1] = rho * u1;
+        u[2] = rho * u2;
+        u[D + 1] = prs / (g - 1.0) + 0.5 * rho * (u1 * u1 + u2 * u2);
+    } else if (a.ic == 5) {
+        double prod = 1.0;
+        for (int d = 0; d < D; ++d) prod *= sin(2.0 * M_PI * (x[d] - a.plo[d]) / (a.phi[d] - a.plo[d]));
+        const double rho = 1.0 + a.prm[0] * prod;
+        double k = 0.0;
+        u[0] = rho;
+        for (int d = 0; d < D; ++d) { u[d + 1] = rho * vel5[d]; k += vel5[d] * vel5[d]; }
+        u[D + 1] = a.prm[1] / (g - 1.0) + 0.5 * rho * k;
+    } else {
+        const double rho = a.prm[0];
+        const double vel[3] = {a.prm[1], a.prm[2], D > 2 ? a.prm[3] : 0.0};
+        double k = 0.0;
+        u[0] = rho;
+        for (int d = 0; d < D; ++d) { u[d + 1] = rho * vel[d]; k += vel[d] * vel[d]; }
+        u[D + 1] = a.prm[D == 3 ? 4 : 3] / (g - 1.0) + 0.5 * rho * k;
+    }
+}
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_lp_error(LpArgs a) {
+    using Dm = Dims<P, D>;
+    constexpr int Ns = Dm::Ns, Ms = Dm::Ms;
+    __shared__ double red[256];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double cell = 0.0;
+    if (i < a.n) {
+        const long s = a.list[i];
+        const int lev = a.slot_lev[s];
+        const double* Vs = a.tab + a.to.Vs;
+        const double* ws = a.tab + a.to.ws;
+        const double* xs = a.tab + a.to.xs;
+        double U[Ns];
+        for (int n = 0; n < Ns; ++n) U[n] = a.Uw[((long)a.q * Ns + n) * a.NS + s];
+        for (int m = 0; m < Ms; ++m) {
+            double uh = 0.0;
+            for (int n = 0; n < Ns; ++n) uh += U[n] * Vs[m * Ns + n];
+            double x[3] = {0.0, 0.0, 0.0}, ue[5];
+            for (int d = 0; d < D; ++d) x[d] = 0.5 * a.geo.dx[lev][d] * xs[m * D + d] + a.centre[i * 3 + d];
+            exact_state<D>(a, x, ue);
+            const double e = fabs(ue[a.q] - uh);
+            cell += (a.p == 1 ? e : pow(e, (double)a.p)) * ws[m];
+        }
+        cell *= a.geo.vol[lev] / (double)(1 << D);
+    }
+    red[threadIdx.x] = cell;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = red[0];
+}
+
 // Layout probe of the 16x16x4 MFMA builtins (one wavefront, in-bounds):
 // mode 0: A[lane] = lane+1, B = 1 on lanes 0..15 (kidx 0) else 0
 // mode 1: A = 1 on lanes 0..15 else 0, B[lane] = lane+1

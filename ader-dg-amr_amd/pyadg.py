@@ -28,7 +28,7 @@ EXPORTS = [
     "adg_project_ic", "adg_set_predictor_kernel", "adg_set_predictor_waves", "adg_last_error", "adg_destroy",
     "adg_mlp_sizes", "adg_mlp_init", "adg_mlp_get_params", "adg_mlp_set_params", "adg_mlp_collect",
     "adg_mlp_dataset", "adg_mlp_dataset_clear", "adg_mlp_fit_normalization", "adg_mlp_train",
-    "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats",
+    "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats", "adg_lp_error",
 ]
 
 
@@ -91,6 +91,7 @@ def lib():
     L.adg_mlp_warm_start.argtypes = [C.c_void_p, C.c_int, _dp]
     L.adg_set_learned.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
     L.adg_learned_stats.argtypes = [C.c_void_p, _up, C.c_int]
+    L.adg_lp_error.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, C.c_double, _dp]
     L.adg_last_error.argtypes = [C.c_void_p]
     L.adg_last_error.restype = C.c_char_p
     L.adg_destroy.argtypes = [C.c_void_p]
@@ -240,6 +241,14 @@ class Context:
     def project_ic(self, lev, ic_case, params):
         prm = np.array(params, dtype=np.float64)
         self._chk(self.L.adg_project_ic(self.h, lev, ic_case, _d(prm)), "adg_project_ic")
+
+    def lp_error(self, q, p, ic_case, params, t):
+        """Composite Lp error of variable q against the exact solution at t."""
+        prm = np.zeros(8)
+        prm[:len(params)] = params
+        out = np.zeros(1)
+        self._chk(self.L.adg_lp_error(self.h, q, p, ic_case, _d(prm), t, _d(out)), "adg_lp_error")
+        return float(out[0])
 
     def sync(self):
         self._chk(self.L.adg_sync(self.h), "adg_sync")

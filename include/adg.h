@@ -176,6 +176,15 @@ int adg_set_time(adg_ctx* ctx, double t);
  * (the quantity LpNorm_DG_AMR integrates over, AmrDG.h:1399-1578). */
 int adg_conserved_totals(adg_ctx* ctx, double* totals);
 
+/* Composite Lp error (p = 1 or 2) of conserved variable q over the uncovered
+ * valid cells against the exact solution at time t, as LpNorm_DG_AMR
+ * (AmrDG.h:1399-1578): (P+1)^D Gauss-Legendre points per cell, normalised
+ * by the uncovered volume.  Exact solutions: ic_case 0 (isentropic vortex)
+ * and 5 (density wave) translated periodically by their background
+ * velocity, 2 (uniform state).  params: the initial condition's parameters,
+ * 5 values read (unused ones ignored). */
+int adg_lp_error(adg_ctx* ctx, int q, int p, int ic_case, const double* params, double t, double* out);
+
 /* Blocks until all work on the context stream has finished. */
 int adg_sync(adg_ctx* ctx);
 /* The hipStream_t the context launches on. */

@@ -528,7 +528,7 @@ struct OLevel {
     std::vector<int> valid;         // slot -> 1/0
     // ghost sources: kind 0 copy(src slot) ; kind 1 scatter(coarse slot, child)
     std::vector<long> gslot, gsrc;
-    std::vector<int> gkind, gchild;
+    std::vector<int> gkind, gchild, gmask;   // gmask: mirrored directions of a transmissive boundary ghost
     std::vector<int> bcoarse[3], bfine[3], cidx[3];
 };
 
@@ -647,28 +647,38 @@ static void build_hier(orc_hier* h) {
             if (lv.valid[s]) continue;
             int b, iv[3];
             h->slot_decode(l, s, &b, iv);
-            long src = h->find_valid(l, iv);
+            // outside a non-periodic boundary: transmissive (outflow) ghost as
+            // the mirror image across the boundary face (the reference's own
+            // non-periodic modal BC is a stub, Compressible_Euler.h:58-72)
+            int mv[3] = {iv[0], iv[1], iv[2]}, mask = 0;
+            for (int d = 0; d < D; ++d) {
+                const int n = lv.dom[d];
+                if (h->per[d]) continue;
+                if (iv[d] < 0) { mv[d] = -1 - iv[d]; mask |= 1 << d; }
+                if (iv[d] >= n) { mv[d] = 2 * n - 1 - iv[d]; mask |= 1 << d; }
+            }
+            long src = h->find_valid(l, mv);
             if (src >= 0) {
                 lv.gslot.push_back(s); lv.gsrc.push_back(src);
-                lv.gkind.push_back(0); lv.gchild.push_back(0);
+                lv.gkind.push_back(0); lv.gchild.push_back(0); lv.gmask.push_back(mask);
                 continue;
             }
-            if (l == 0) {  // non-periodic physical boundary: not restated
+            if (l == 0) {  // no source (cannot happen on a level-0 cover)
                 lv.gslot.push_back(s); lv.gsrc.push_back(-1);
-                lv.gkind.push_back(2); lv.gchild.push_back(0);
+                lv.gkind.push_back(2); lv.gchild.push_back(0); lv.gmask.push_back(0);
                 continue;
             }
             int civ[3] = {0, 0, 0};
             int child = 0;
             for (int d = 0; d < D; ++d) {
-                int c = (iv[d] >= 0) ? iv[d] / 2 : -((-iv[d] + 1) / 2);  // coarsen (floor)
+                int c = (mv[d] >= 0) ? mv[d] / 2 : -((-mv[d] + 1) / 2);  // coarsen (floor)
                 civ[d] = c;
-                int bit = iv[d] - c * 2;
+                int bit = mv[d] - c * 2;
                 child += bit * (1 << (D - 1 - d));
             }
             long csrc = h->find_valid(l - 1, civ);
             lv.gslot.push_back(s); lv.gsrc.push_back(csrc);
-            lv.gkind.push_back(csrc >= 0 ? 1 : 2); lv.gchild.push_back(child);
+            lv.gkind.push_back(csrc >= 0 ? 1 : 2); lv.gchild.push_back(child); lv.gmask.push_back(mask);
         }
     }
     // C-F face masks: AmrDG_Level.tpp:240-414
@@ -1464,6 +1474,16 @@ extern "C" void orc_fill_patch(const orc_tables* t, const orc_hier* h, double** 
                     }
                 }
             }
+            // transmissive boundary ghost: mirror image, P_k(-xi) = (-1)^k P_k(xi)
+            const int mask = lv.gmask[g];
+            if (mask && lv.gkind[g] != 2)
+                for (int q = 0; q < Q; ++q)
+                    for (int n = 0; n < Ns; ++n) {
+                        int par = 0;
+                        for (int d = 0; d < t->D; ++d)
+                            if (mask & (1 << d)) par += t->mi_s[n][d];
+                        if (par & 1) Uw[l][((long)q * Ns + n) * ns + s] = -Uw[l][((long)q * Ns + n) * ns + s];
+                    }
         }
     }
 }
@@ -1563,9 +1583,12 @@ extern "C" double orc_lp_error(const orc_tables* t, const orc_hier* h, double** 
                 double x[3], u[5];
                 for (int d = 0; d < D; ++d) {
                     x[d] = (lv.dx[d] / 2.0) * t->xs[m][d] + (h->plo[d] + (iv[d] + 0.5) * lv.dx[d]);
-                    if (ic_case == 0 && d < 2) {
+                    if ((ic_case == 0 && d < 2) || ic_case == 5) {
+                        // translate by the background velocity on the periodic domain
+                        const double vel5[3] = {1.0, 0.5, 0.25};
                         double L = h->phi[d] - h->plo[d];
-                        double sh = x[d] - prm[3 + d] * time - h->plo[d];
+                        double v = ic_case == 0 ? prm[3 + d] : vel5[d];
+                        double sh = x[d] - v * time - h->plo[d];
                         sh = sh - L * std::floor(sh / L);
                         x[d] = h->plo[d] + sh;
                     }

@@ -152,3 +152,33 @@ def test_3d_shock_tube_first_step(adg, orc):
         e = rel_err(ctx.download_Uw(l), Uw[l])
         print(f"  level {l} relative error {e:.2e}", flush=True)
         assert e < TOL_SHOCK, f"level {l}"
+
+
+def test_3d_outflow_shock_tube(adg, orc):
+    """Non-periodic domain with transmissive (mirror-image) ghosts, as the
+    spherical shock-tube workload runs: two steps on two levels."""
+    print("case 3D shock tube outflow two levels P=2 two steps", flush=True)
+    levels = two_level_centre(8, 4, D=3)
+    T = orc.Tables(2, 3)
+    H = orc.Hier(3, levels, [8] * 3, [0] * 3, [1] * 3, periodic=[0, 0, 0])
+    Uw = orc.project_ic(T, H, 3, SHOCK)
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    ctx = adg.Context(3, 2, 1.4, 0)
+    ctx.set_geometry([8] * 3, [0] * 3, [1] * 3, periodic=[0, 0, 0])
+    ctx.set_levels(levels)
+    for l in range(2):
+        ctx.upload_Uw(l, Uw[l])
+    ctx.fill_ghosts()
+    for l in range(2):   # the device ghost fill reproduces the oracle's
+        assert rel_err(ctx.download_Uw(l), Uw[l]) < 1e-15, f"ghosts level {l}"
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    m0 = ctx.totals()
+    for _ in range(2):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=2)
+    for l in range(2):
+        e = rel_err(ctx.download_Uw(l), Uw[l])
+        print(f"  level {l} relative error {e:.2e}", flush=True)
+        assert e < TOL_SHOCK, f"level {l}"

@@ -239,3 +239,26 @@ def test_step_graph_replays(adg, orc, precision):
         orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=4)
         for l in range(3):
             assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL
+
+
+def test_kh_outflow_y_three_levels(adg, orc):
+    """Periodic in x, transmissive (mirror-image) ghosts at y = 0 and 1."""
+    print("case KH P=3 16^2 3 levels, outflow in y", flush=True)
+    levels = kh_bands(16, 3, 16)
+    T = orc.Tables(3, 2)
+    H = orc.Hier(2, levels, [16, 16], [0, 0], [1, 1], periodic=[1, 0])
+    Uw = orc.project_ic(T, H, 1, [0.01, 1e-3, 7])
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    ctx = adg.Context(2, 3, 1.4, 0)
+    ctx.set_geometry([16, 16], [0, 0], [1, 1], periodic=[1, 0])
+    ctx.set_levels(levels)
+    for l in range(3):
+        ctx.upload_Uw(l, Uw[l])
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    for _ in range(3):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=3)
+    for l in range(3):
+        assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL, f"level {l}"

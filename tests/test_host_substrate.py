@@ -57,14 +57,15 @@ LAYOUTS = {
 }
 
 
+@pytest.mark.parametrize("periodic", [[1, 1], [0, 0], [1, 0]])
 @pytest.mark.parametrize("name", sorted(LAYOUTS))
-def test_face_masks_match_oracle(adg, orc, name):
+def test_face_masks_match_oracle(adg, orc, name, periodic):
     levels, dom = LAYOUTS[name]
-    H = orc.Hier(2, levels, dom, [0, 0], [1, 1])
+    H = orc.Hier(2, levels, dom, [0, 0], [1, 1], periodic=periodic)
     for l in range(len(levels)):
         for d in range(2):
             for kind in range(3):
-                got = inspect(adg, 2, dom, levels, l, d, kind)
+                got = inspect(adg, 2, dom, levels, l, d, kind, periodic=periodic)
                 ref = H.face_mask(l, d, kind)
                 assert np.array_equal(got, ref), (name, l, d, kind)
 

@@ -248,3 +248,23 @@ def test_golden_one_step(orc):
         assert np.array_equal(Uw[0], g["step_U0"])
         orc.ader(T, H, Uw, float(g["step_Dt"][0]))
         assert np.array_equal(Uw[0], g["step_U1"])
+
+
+def test_outflow_preserves_free_stream_and_masks(adg):
+    """Transmissive (mirror-image) ghosts on non-periodic boundaries: a
+    uniform flow leaving the domain stays uniform to round-off on one and two
+    levels, and the library's host substrate builds the same coarse-fine
+    masks as the oracle on the non-periodic layout."""
+    import pyoracle as O
+    from layouts import two_level_centre, tiles
+    T = O.Tables(2, 2)
+    for levels in ([tiles(8, 4)], two_level_centre(8, 4)):
+        H = O.Hier(2, levels, [8, 8], [0, 0], [1, 1], periodic=[0, 0])
+        Uw = O.project_ic(T, H, 2, [1.2, 0.7, -0.4, 2.0])
+        ref = [u.copy() for u in Uw]
+        O.fill_patch(T, H, Uw)
+        Dt = O.set_dt(T, H, Uw, 0.5)
+        O.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=4)
+        for l in range(len(levels)):
+            v = H.valid_mask(l)
+            assert np.abs(Uw[l][:, :, v] - ref[l][:, :, v]).max() < 1e-13
