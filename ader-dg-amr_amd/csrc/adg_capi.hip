@@ -25,6 +25,8 @@
 namespace adg {
 void project_ic(const ElementTables& t, const Hierarchy& H, int lev, int ic_case, const double* prm, double gamma,
                 std::vector<double>& out);
+void project_ic_patch(const ElementTables& t, const Hierarchy& H, int p, int ic_case, const double* prm, double gamma,
+                      std::vector<double>& out);
 }
 
 struct adg_ctx {
@@ -75,6 +77,7 @@ struct adg_ctx {
     adg::TimeState* ts = nullptr;
     bool keep_hw = false;
     int iters_override = 0;    // > 0: Picard iterations of the next plain predictor launches (collection)
+    bool independent = false;  // adg_set_levels builds a batch of independent periodic patches
 
     // learned warm start
     struct Mlp {
@@ -359,7 +362,7 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if (nlev < 1 || nlev > adg::kMaxLev || !npatch || !boxes) return fail(c, ADG_E_ARG, "bad level arguments");
     free_levels(c);
     try {
-        c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes);
+        c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes, c->independent ? 1 : 0);
     } catch (const std::exception& e) {
         return fail(c, ADG_E_ARG, e.what());
     }
@@ -526,6 +529,25 @@ int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
     std::vector<double> host;
     adg::project_ic(c->tab, c->H, lev, ic_case, params, c->gamma, host);
     return copy_level(c, lev, c->Uw, c->tab.Np_s, host.data(), true);
+}
+
+int adg_project_ic_patches(adg_ctx* c, int lev, int ic_case, int nprm, const double* params) {
+    if (!c) return ADG_E_ARG;
+    if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (lev < 0 || lev >= c->H.nlev || !params || nprm < 1 || ic_case < 0 || ic_case > 5)
+        return fail(c, ADG_E_ARG, "bad level, case or params");
+    if (ic_case <= 1 && c->D != 2) return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
+    const adg::LevelLayout& L = c->H.lev[lev];
+    std::vector<double> host((size_t)c->Q * c->tab.Np_s * L.nslot, 0.0);
+    for (int p = L.p_begin; p < L.p_end; ++p)
+        adg::project_ic_patch(c->tab, c->H, p, ic_case, params + (size_t)(p - L.p_begin) * nprm, c->gamma, host);
+    return copy_level(c, lev, c->Uw, c->tab.Np_s, host.data(), true);
+}
+
+int adg_set_independent_patches(adg_ctx* c, int on) {
+    if (!c) return ADG_E_ARG;
+    c->independent = on != 0;
+    return ADG_OK;
 }
 
 int adg_keep_Hw(adg_ctx* c, int on) {
@@ -746,6 +768,52 @@ static int configure_mlp_warm(adg_ctx* c) {
     return ADG_OK;
 }
 }  // extern "C++"
+
+int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (reps < 1) return fail(c, ADG_E_ARG, "reps must be >= 1");
+    // candidates: {kind, waves per workgroup} of the predictor launch
+    const int cand[4][2] = {{1, 8}, {2, 8}, {3, 8}, {3, 16}};
+    const int kind0 = c->pred_kind, wpb0 = c->nl_wpb;
+    double best = 1e300;
+    int bi = -1;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    for (int i = 0; i < 4; ++i) {
+        double ms = -1.0;
+        const int kind = cand[i][0], wpb = cand[i][1];
+        const bool ok = (kind == 1 && c->precision == ADG_FP64) || (kind == 2 && quad_available(c)) ||
+                        (kind == 3 && nl_available(c, wpb) && c->pred_per_cu[wpb == 16 ? 2 : 1] > 0);
+        if (ok) {
+            c->pred_kind = kind;
+            c->nl_wpb = wpb;
+            int rc = adg_predictor(c, -1, 0, nullptr);   // warm-up launch
+            if (rc != ADG_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps && rc == ADG_OK; ++r) rc = adg_predictor(c, -1, 0, nullptr);
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float el = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&el, e0, e1));
+            ms = (double)el / reps;
+            if (ms < best) { best = ms; bi = i; }
+        }
+        if (ms_out) ms_out[i] = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bi < 0) {
+        c->pred_kind = kind0;
+        c->nl_wpb = wpb0;
+        return fail(c, ADG_E_UNSUP, "no predictor candidate fits");
+    }
+    c->pred_kind = cand[bi][0];
+    c->nl_wpb = cand[bi][1];
+    drop_graph(c);
+    if (choice_out) { choice_out[0] = c->pred_kind; choice_out[1] = c->nl_wpb; }
+    return ADG_OK;
+}
 
 int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");

@@ -50,11 +50,13 @@ void for_box(int D, const int* lo, const int* hi, F&& f) {
 }  // namespace
 
 Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
-                          const int* per, int nlev, const int* npatch, const int* boxes) {
+                          const int* per, int nlev, const int* npatch, const int* boxes, int independent) {
     if (D < 2 || D > 3 || nlev < 1 || nlev > 8) throw std::runtime_error("bad dim or level count");
+    if (independent && nlev != 1) throw std::runtime_error("independent patches form a single level");
     Hierarchy h;
     h.D = D;
     h.nlev = nlev;
+    h.independent = independent != 0;
     for (int d = 0; d < 3; ++d) {
         h.dom0[d] = d < D ? dom0[d] : 1;
         h.plo[d] = d < D ? plo[d] : 0.0;
@@ -206,6 +208,13 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                 if (iv[d] < P.lo[d] || iv[d] > P.hi[d]) inside = false;
             if (inside) return;
             long s = h.slot_of(p, iv);
+            if (h.independent) {   // periodic inside the patch's own domain
+                int w[3] = {iv[0], iv[1], D > 2 ? iv[2] : 0};
+                for (int d = 0; d < D; ++d) w[d] = P.lo[d] + (((w[d] - P.lo[d]) % P.n[d]) + P.n[d]) % P.n[d];
+                h.g_slot.push_back((int)s); h.g_src.push_back((int)h.slot_of(p, w));
+                h.g_kind.push_back(0); h.g_child.push_back(0);
+                return;
+            }
             // outside a non-periodic boundary: transmissive (outflow) ghost,
             // the mirror image of the cell across the boundary face, whose
             // modes change sign by (-1)^{k_d} in each mirrored direction d

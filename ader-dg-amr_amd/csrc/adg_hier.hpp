@@ -40,6 +40,7 @@ struct PatchLayout {
 
 struct Hierarchy {
     int D = 2, nlev = 0;
+    bool independent = false;   // patches are separate periodic domains
     int dom0[3] = {1, 1, 1};
     double plo[3] = {0, 0, 0}, phi[3] = {1, 1, 1};
     int per[3] = {1, 1, 1};
@@ -82,7 +83,10 @@ struct Hierarchy {
 };
 
 // Throws std::runtime_error on malformed input.
+// independent != 0: one level whose patches are independent periodic
+// domains (a batch of separate problems): ghost cells wrap inside their own
+// patch.
 Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
-                          const int* per, int nlev, const int* npatch, const int* boxes);
+                          const int* per, int nlev, const int* npatch, const int* boxes, int independent = 0);
 
 }  // namespace adg

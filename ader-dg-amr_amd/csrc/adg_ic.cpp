@@ -117,40 +117,53 @@ void pointwise_state(int D, int ic_case, const double* prm, double gamma, const 
     }
 }
 
+// Projects onto the modes of the slots of patch p (ghosts included) of its
+// level, into out ([q][n][slot of the level]).  For independent patches the
+// coordinates are local to the patch, whose domain is [plo, plo + n dx).
+void project_ic_patch(const ElementTables& t, const Hierarchy& H, int p, int ic_case, const double* prm, double gamma,
+                      std::vector<double>& out) {
+    const int D = t.D, Q = t.Q, Ns = t.Np_s, Ms = t.qMp_s;
+    const PatchLayout& P = H.patch[p];
+    const LevelLayout& L = H.lev[P.lev];
+    const long ns = L.nslot;
+    std::vector<double> xi((size_t)Ms * D);
+    for (int m = 0; m < Ms; ++m)
+        for (int d = 0; d < D; ++d) xi[(size_t)m * D + d] = t.nodes[t.node_idx(m, d, D)];
+    double plo[3] = {H.plo[0], H.plo[1], H.plo[2]}, phi[3] = {H.phi[0], H.phi[1], H.phi[2]}, shift[3] = {0, 0, 0};
+    if (H.independent)
+        for (int d = 0; d < D; ++d) {
+            shift[d] = P.lo[d] * L.dx[d];
+            phi[d] = plo[d] + P.n[d] * L.dx[d];
+        }
+    const int glo[3] = {P.lo[0] - 1, P.lo[1] - 1, D > 2 ? P.lo[2] - 1 : 0};
+    const int ghi[3] = {P.hi[0] + 1, P.hi[1] + 1, D > 2 ? P.hi[2] + 1 : 0};
+    int iv[3];
+    std::vector<double> st((size_t)Ms * 5);
+    for (iv[2] = glo[2]; iv[2] <= ghi[2]; ++iv[2])
+        for (iv[1] = glo[1]; iv[1] <= ghi[1]; ++iv[1])
+            for (iv[0] = glo[0]; iv[0] <= ghi[0]; ++iv[0]) {
+                const long s = H.slot_of(p, iv) - L.slot_off;
+                for (int m = 0; m < Ms; ++m) {
+                    double x[3] = {0, 0, 0};
+                    for (int d = 0; d < D; ++d)
+                        x[d] = (L.dx[d] / 2.0) * xi[(size_t)m * D + d] + (H.plo[d] + (iv[d] + 0.5) * L.dx[d]) - shift[d];
+                    pointwise_state(D, ic_case, prm, gamma, x, plo, phi, P.lev, iv, &st[(size_t)m * 5]);
+                }
+                for (int q = 0; q < Q; ++q)
+                    for (int n = 0; n < Ns; ++n) {
+                        double sum = 0.0;
+                        for (int m = 0; m < Ms; ++m) sum += st[(size_t)m * 5 + q] * t.quadmat[(size_t)n * Ms + m];
+                        out[((size_t)q * Ns + n) * ns + s] = sum / t.Mk_corr[(size_t)n * Ns + n];
+                    }
+            }
+}
+
 // Projects onto the modes of every slot of level lev: [q][n][slot].
 void project_ic(const ElementTables& t, const Hierarchy& H, int lev, int ic_case, const double* prm, double gamma,
                 std::vector<double>& out) {
-    const int D = t.D, Q = t.Q, Ns = t.Np_s, Ms = t.qMp_s;
     const LevelLayout& L = H.lev[lev];
-    const long ns = L.nslot;
-    out.assign((size_t)Q * Ns * ns, 0.0);
-    std::vector<double> xi((size_t)Ms * D), phiw((size_t)Ns * Ms);
-    for (int m = 0; m < Ms; ++m)
-        for (int d = 0; d < D; ++d) xi[(size_t)m * D + d] = t.nodes[t.node_idx(m, d, D)];
-    for (int p = L.p_begin; p < L.p_end; ++p) {
-        const PatchLayout& P = H.patch[p];
-        const int glo[3] = {P.lo[0] - 1, P.lo[1] - 1, D > 2 ? P.lo[2] - 1 : 0};
-        const int ghi[3] = {P.hi[0] + 1, P.hi[1] + 1, D > 2 ? P.hi[2] + 1 : 0};
-        int iv[3];
-        for (iv[2] = glo[2]; iv[2] <= ghi[2]; ++iv[2])
-            for (iv[1] = glo[1]; iv[1] <= ghi[1]; ++iv[1])
-                for (iv[0] = glo[0]; iv[0] <= ghi[0]; ++iv[0]) {
-                    const long s = H.slot_of(p, iv) - L.slot_off;
-                    std::vector<double> st((size_t)Ms * 5);
-                    for (int m = 0; m < Ms; ++m) {
-                        double x[3] = {0, 0, 0};
-                        for (int d = 0; d < D; ++d)
-                            x[d] = (L.dx[d] / 2.0) * xi[(size_t)m * D + d] + (H.plo[d] + (iv[d] + 0.5) * L.dx[d]);
-                        pointwise_state(D, ic_case, prm, gamma, x, H.plo, H.phi, lev, iv, &st[(size_t)m * 5]);
-                    }
-                    for (int q = 0; q < Q; ++q)
-                        for (int n = 0; n < Ns; ++n) {
-                            double sum = 0.0;
-                            for (int m = 0; m < Ms; ++m) sum += st[(size_t)m * 5 + q] * t.quadmat[(size_t)n * Ms + m];
-                            out[((size_t)q * Ns + n) * ns + s] = sum / t.Mk_corr[(size_t)n * Ns + n];
-                        }
-                }
-    }
+    out.assign((size_t)t.Q * t.Np_s * L.nslot, 0.0);
+    for (int p = L.p_begin; p < L.p_end; ++p) project_ic_patch(t, H, p, ic_case, prm, gamma, out);
 }
 
 }  // namespace adg

@@ -44,3 +44,22 @@ def kh_bands(n0, nlev=3, mg=32, D=2):
             boxes += tiles(n, mg, D, lo, hi)
         levels.append(boxes)
     return levels
+
+
+def batch_patches(npatch, nmin, nmax, seed, row_width=4096):
+    """Square patches of edge n ~ U{nmin..nmax} (seeded) shelf-packed into
+    rows of a bounding index space, for a batch of independent periodic
+    problems.  Returns (levels, dom, edges)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(nmin, nmax + 1, size=npatch)
+    boxes, x, y, row_h = [], 0, 0, 0
+    for n in edges:
+        n = int(n)
+        if x + n > row_width:
+            x, y, row_h = 0, y + row_h, 0
+        boxes.append(((x, y), (x + n - 1, y + n - 1)))
+        x += n
+        row_h = max(row_h, n)
+    dom = [row_width, y + row_h]
+    return [boxes], dom, [int(n) for n in edges]

@@ -29,6 +29,7 @@ EXPORTS = [
     "adg_mlp_sizes", "adg_mlp_init", "adg_mlp_get_params", "adg_mlp_set_params", "adg_mlp_collect",
     "adg_mlp_dataset", "adg_mlp_dataset_clear", "adg_mlp_fit_normalization", "adg_mlp_train",
     "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats", "adg_lp_error",
+    "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
 ]
 
 
@@ -92,6 +93,9 @@ def lib():
     L.adg_set_learned.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
     L.adg_learned_stats.argtypes = [C.c_void_p, _up, C.c_int]
     L.adg_lp_error.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, C.c_double, _dp]
+    L.adg_autotune.argtypes = [C.c_void_p, C.c_int, _dp, _ip]
+    L.adg_project_ic_patches.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
+    L.adg_set_independent_patches.argtypes = [C.c_void_p, C.c_int]
     L.adg_last_error.argtypes = [C.c_void_p]
     L.adg_last_error.restype = C.c_char_p
     L.adg_destroy.argtypes = [C.c_void_p]
@@ -249,6 +253,23 @@ class Context:
         out = np.zeros(1)
         self._chk(self.L.adg_lp_error(self.h, q, p, ic_case, _d(prm), t, _d(out)), "adg_lp_error")
         return float(out[0])
+
+    def autotune(self, reps=10):
+        """Times the predictor launch shapes and keeps the fastest; returns
+        ({shape: ms}, (kind, waves))."""
+        ms = np.zeros(4)
+        ch = np.zeros(2, dtype=np.int32)
+        self._chk(self.L.adg_autotune(self.h, reps, _d(ms), _i(ch)), "adg_autotune")
+        names = ("valu", "quad8", "nodelane8", "nodelane16")
+        return {n: float(v) for n, v in zip(names, ms) if v >= 0}, (int(ch[0]), int(ch[1]))
+
+    def project_ic_patches(self, lev, ic_case, params):
+        """params: [npatch][nprm] parameter sets, one per patch of the level."""
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(self.L.adg_project_ic_patches(self.h, lev, ic_case, prm.shape[1], _d(prm)), "adg_project_ic_patches")
+
+    def set_independent_patches(self, on=True):
+        self._chk(self.L.adg_set_independent_patches(self.h, 1 if on else 0), "adg_set_independent_patches")
 
     def sync(self):
         self._chk(self.L.adg_sync(self.h), "adg_sync")

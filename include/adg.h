@@ -85,6 +85,15 @@ int adg_download_Uw(adg_ctx* ctx, int lev, double* host);
  *              sin(2 pi x_d / L_d), velocity (1, 0.5, 0.25), pressure p0,
  *              params {amp, p0} */
 int adg_project_ic(adg_ctx* ctx, int lev, int ic_case, const double* params);
+/* As adg_project_ic with one parameter set of nprm values per patch of the
+ * level (params [npatch][nprm]); for independent patches the coordinates are
+ * local to each patch, whose domain is [prob_lo, prob_lo + n dx). */
+int adg_project_ic_patches(adg_ctx* ctx, int lev, int ic_case, int nprm, const double* params);
+/* on != 0 before adg_set_levels: the single level's patches are independent
+ * periodic domains (a batch of separate problems sharing one time step);
+ * ghost cells wrap inside their own patch.  No counterpart in the reference:
+ * the batched form of the hot path (every patch in one launch per stage). */
+int adg_set_independent_patches(adg_ctx* ctx, int on);
 
 /* Replaces AmrDG::AMR_FillPatch on every level (AmrDG_Level.tpp:172-224)
  * and Solver::FillBoundaryCells for periodic domains: same-level and
@@ -124,6 +133,13 @@ int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
  * p = 1 uses VALU.  No counterpart in the reference: a tuning knob of this
  * implementation. */
 int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
+/* Autotuner: times the predictor launch shapes that fit this context and
+ * hierarchy ({VALU} for FP64, {quad-gather, 8 waves}, {node-lane, 8 waves},
+ * {node-lane, 16 waves}) over `reps` launches each on the current state and
+ * keeps the fastest.  ms_out[4] (nullable) gets the average launch time of
+ * each candidate (-1 where it does not apply), choice_out[2] (nullable) the
+ * chosen {kind, waves}.  No counterpart in the reference. */
+int adg_autotune(adg_ctx* ctx, int reps, double* ms_out, int* choice_out);
 /* Wavefronts per workgroup of the node-lane predictor: 8 (default) or 16.
  * No counterpart in the reference: a launch-shape knob of this
  * implementation, set by the autotuner. */
