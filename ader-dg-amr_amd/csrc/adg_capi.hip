@@ -99,7 +99,8 @@ struct adg_ctx {
     unsigned long long* lstats = nullptr;     // [elements, fallbacks] of learned predictor launches
     bool learned = false;
     float learned_tol = 1e-2f;
-    int warm_per_cu = 0;
+    int warm_tiles = 1;           // 16-element column tiles per wavefront of the warm-start kernel (1 or 2)
+    int warm_per_cu[3] = {0, 0, 0};
     int learned_iters = 1;
     std::string err;
     std::vector<void*> level_allocs;
@@ -739,10 +740,12 @@ static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
     a.gamma = c->gamma;
     a.warm = c->warm;
     const size_t lds = (size_t)m.nfrag * 64 * sizeof(uint2);
-    const void* kern = (const void*)adg::k_mlp_warm<P, D, NINP, NOUTP>;
-    if (c->warm_per_cu <= 0) return fail(c, ADG_E_STATE, "warm-start kernel not configured");
-    const long nbatch = (n + 127) / 128;
-    const long grid = std::min<long>(nbatch, (long)c->warm_per_cu * c->ncu);
+    const int nt = c->warm_tiles;
+    const void* kern = nt == 2 ? (const void*)adg::k_mlp_warm2<P, D, NINP, NOUTP>
+                               : (const void*)adg::k_mlp_warm1<P, D, NINP, NOUTP>;
+    if (c->warm_per_cu[nt] <= 0) return fail(c, ADG_E_STATE, "warm-start kernel not configured");
+    const long nbatch = (n + 128L * nt - 1) / (128L * nt);
+    const long grid = std::min<long>(nbatch, (long)c->warm_per_cu[nt] * c->ncu);
     void* kargs[] = {&a};
     HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(512), kargs, lds, c->stream));
     HIPCHK(c, hipGetLastError());
@@ -758,13 +761,16 @@ template <int P, int D>
 static int configure_mlp_warm(adg_ctx* c) {
     constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
     constexpr int NINP = 16 * ((Q * Ns + 1 + D + 15) / 16), NOUTP = 16 * ((Q * Nst + 15) / 16);
-    const void* kern = (const void*)adg::k_mlp_warm<P, D, NINP, NOUTP>;
     const size_t lds = (size_t)c->mlp.nfrag * 64 * sizeof(uint2);
     if (lds > kLdsMax) return fail(c, ADG_E_UNSUP, "network fragments exceed LDS");
-    HIPCHK(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int nb = 0;
-    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 512, lds));
-    c->warm_per_cu = nb > 0 ? nb : 1;
+    for (int nt = 1; nt <= 2; ++nt) {
+        const void* kern = nt == 2 ? (const void*)adg::k_mlp_warm2<P, D, NINP, NOUTP>
+                                   : (const void*)adg::k_mlp_warm1<P, D, NINP, NOUTP>;
+        HIPCHK(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int nb = 0;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 512, lds));
+        c->warm_per_cu[nt] = nb > 0 ? nb : 1;
+    }
     return ADG_OK;
 }
 }  // extern "C++"
@@ -801,17 +807,42 @@ int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
         }
         if (ms_out) ms_out[i] = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (bi < 0) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         c->pred_kind = kind0;
         c->nl_wpb = wpb0;
         return fail(c, ADG_E_UNSUP, "no predictor candidate fits");
     }
     c->pred_kind = cand[bi][0];
     c->nl_wpb = cand[bi][1];
+    // warm-start inference: one or two 16-element tiles per wavefront
+    if (ms_out) ms_out[4] = ms_out[5] = -1.0;
+    if (c->mlp.init) {
+        int rc = mlp_pack(c);
+        double wbest = 1e300;
+        for (int nt = 1; nt <= 2 && rc == ADG_OK; ++nt) {
+            c->warm_tiles = nt;
+#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, 0, c->H.NS)
+            ADG_DISPATCH(c, L_MW);
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps && rc == ADG_OK; ++r) ADG_DISPATCH(c, L_MW);
+#undef L_MW
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float el = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&el, e0, e1));
+            const double ms = (double)el / reps;
+            if (ms_out) ms_out[3 + nt] = ms;
+            if (ms < wbest) { wbest = ms; bi = nt; }
+        }
+        if (rc != ADG_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+        c->warm_tiles = bi;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     drop_graph(c);
-    if (choice_out) { choice_out[0] = c->pred_kind; choice_out[1] = c->nl_wpb; }
+    if (choice_out) { choice_out[0] = c->pred_kind; choice_out[1] = c->nl_wpb; choice_out[2] = c->warm_tiles; }
     return ADG_OK;
 }
 

@@ -111,6 +111,11 @@ __device__ __forceinline__ T pressure(const T* s, T rinv, T gm1) {
     return fmax(p, T(kNumLimit));
 }
 
+// 1/x: IEEE division in FP64; the hardware reciprocal (1 ulp) in the FP32
+// form, whose arithmetic is FP32 round-off throughout.
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Flux component q of direction d, branch-free in q (q varies across lanes).
 // sq is the conserved variable q itself (the lane's own value), which keeps
 // the compiler from turning a select chain into a private-memory lookup.
@@ -317,6 +322,24 @@ __host__ __device__ constexpr int nl_tile_stride() {
     return sizeof(T) == 8 ? 18 : 20;
 }
 
+// Workgroup copy of nbytes (a multiple of 16) from global memory to LDS with
+// eight 16-byte loads in flight per thread before their stores, so that the
+// staging of element matrices costs one memory latency, not one per pass.
+__device__ __forceinline__ void stage_to_lds(void* dst, const void* src, int nbytes) {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    const int n = nbytes >> 4, step = (int)blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 7 * step < n; i += 8 * step) {
+        uint4 r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = s[i + k * step];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d[i + k * step] = r[k];
+    }
+    for (; i < n; i += step) d[i] = s[i];
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -357,7 +380,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* fr = reinterpret_cast<T*>(smem_raw);
     const int nfr = a.fo.total * 64;
-    for (int i = threadIdx.x; i < nfr; i += blockDim.x) fr[i] = a.frags[i];
+    stage_to_lds(fr, a.frags, nfr * (int)sizeof(T));
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -416,7 +439,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                     gather_state<Q>(h[r], src_base, s);
                     const bool ok = col_ok && (16 * nb + acc_row<T>(g, r)) < Mst;
                     if (!ok) s[0] = T(1);
-                    const T rinv = T(1) / s[0];
+                    const T rinv = recip(s[0]);
                     const T prs = pressure<D>(s, rinv, gm1);
 #pragma unroll
                     for (int d = 0; d < D; ++d) gd[d][r] = ok ? cd[d] * flux_q<D>(d, q, s, h[r], rinv, prs) : T(0);
@@ -464,7 +487,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                 gather_state<Q>(h[r], src_base, s);
                 const bool ok = col_ok && (16 * nb + acc_row<T>(g, r)) < Mst;
                 if (!ok) s[0] = T(1);
-                const T rinv = T(1) / s[0];
+                const T rinv = recip(s[0]);
                 const T prs = pressure<D>(s, rinv, gm1);
 #pragma unroll
                 for (int d = 0; d < D; ++d) gd[d][r] = ok ? cd[d] * flux_q<D>(d, q, s, h[r], rinv, prs) : T(0);
@@ -558,7 +581,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* fr = reinterpret_cast<T*>(smem_raw);
     const int nfr = a.fo.total * 64;
-    for (int i = threadIdx.x; i < nfr; i += blockDim.x) fr[i] = a.frags[i];
+    stage_to_lds(fr, a.frags, nfr * (int)sizeof(T));
     __syncthreads();
 
     __shared__ unsigned long long wg_stats[2];
@@ -613,7 +636,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
             T s[Q] = {h[0], h[1], h[2], h[3]};
             const bool ok = FULL || (16 * nb + c < Mst);
             if (!ok) s[0] = T(1);
-            const T rinv = T(1) / s[0];
+            const T rinv = recip(s[0]);
             const T prs = pressure<D>(s, rinv, gm1);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -848,7 +871,7 @@ __global__ void __launch_bounds__(256) k_faceflux(FaceArgs<T> a) {
         uR[q] = a.Tr[tr_idx<P, D>(2 * d + 0, mb, q, sr, NS)];
     }
     const T gm = T(a.gamma), gm1 = T(a.gamma - 1.0);
-    const T riL = T(1) / uL[0], riR = T(1) / uR[0];
+    const T riL = recip(uL[0]), riR = recip(uR[0]);
     const T pL = pressure<D>(uL, riL, gm1), pR = pressure<D>(uR, riR, gm1);
     const T cL = sqrt(gm * (pL / fmax(uL[0], T(kNumLimit))));
     const T cR = sqrt(gm * (pR / fmax(uR[0], T(kNumLimit))));

@@ -10,7 +10,7 @@
 //   x and y are standardised per column (dataset mean / deviation).
 // Network: h1 = gelu(W1 x + b1), h2 = gelu(W2 h1 + b2), y = W3 h2 + b3,
 // widths nin_p -> 128 -> 128 -> nout_p (nin_p, nout_p = widths padded to 16),
-// gelu(a) = a/2 (1 + erf(a/sqrt 2)).  Every matrix product runs on the
+// gelu(a) = a/2 (1 + erf(a/sqrt 2)) (erf to 1.5e-7).  Every matrix product runs on the
 // 16x16x16 BF16 MFMA with FP32 accumulation; operands are rounded to BF16
 // (round to nearest even) as they are loaded, master weights stay FP32.
 #pragma once
@@ -48,9 +48,24 @@ __device__ __forceinline__ f32x4 mma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float gelu_f(float a) { return 0.5f * a * (1.0f + erff(a * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, formula
+// 7.1.26: erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2),
+// t = 1 / (1 + p x) for x >= 0, odd extension; |error| <= 1.5e-7, far below
+// the BF16 resolution of the activations it feeds.  About a third of the
+// instructions of the library erf, which dominates the inference kernel.
+__device__ __forceinline__ float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    float y = fmaf(t, 1.061405429f, -1.453152027f);
+    y = fmaf(t, y, 1.421413741f);
+    y = fmaf(t, y, -0.284496736f);
+    y = fmaf(t, y, 0.254829592f);
+    y = 1.0f - t * y * __expf(-ax * ax);
+    return copysignf(y, x);
+}
+__device__ __forceinline__ float gelu_f(float a) { return 0.5f * a * (1.0f + erf_as(a * 0.70710678118654752f)); }
 __device__ __forceinline__ float dgelu_f(float a) {
-    return 0.5f * (1.0f + erff(a * 0.70710678118654752f)) + a * 0.39894228040143268f * expf(-0.5f * a * a);
+    return 0.5f * (1.0f + erf_as(a * 0.70710678118654752f)) + a * 0.39894228040143268f * __expf(-0.5f * a * a);
 }
 
 // ---------------------------------------------------------------------------
@@ -299,15 +314,16 @@ struct WarmArgs {
     float* __restrict__ warm;          // [nout][NS]
 };
 
-template <int P, int D, int NINP, int NOUTP>
-__global__ void __launch_bounds__(512) k_mlp_warm(WarmArgs a) {
+// NT column tiles of 16 elements per wavefront share every fragment read.
+template <int P, int D, int NINP, int NOUTP, int NT>
+__device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst;
     constexpr int NIN = Q * Ns + 1 + D, NOUT = Q * Nst, NH = kMlpHidden;
     constexpr int KS1 = NINP / 16, KS2 = NH / 16, UT3 = NOUTP / 16;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     uint2* fr = reinterpret_cast<uint2*>(smem_raw);
-    for (int i = threadIdx.x; i < a.nfrag * 64; i += blockDim.x) fr[i] = a.frag[i];
+    stage_to_lds(fr, a.frag, a.nfrag * 64 * (int)sizeof(uint2));
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int g = lane >> 4, c = lane & 15;
@@ -315,7 +331,8 @@ __global__ void __launch_bounds__(512) k_mlp_warm(WarmArgs a) {
     const float* sdx = a.norm + NINP;
     const float* muy = a.norm + 2 * NINP;
     const float* sdy = muy + NOUTP;
-    const long nbatch = (a.nelem + 16L * nwave - 1) / (16L * nwave);
+    const long per_wave = 16L * NT;
+    const long nbatch = (a.nelem + per_wave * nwave - 1) / (per_wave * nwave);
     auto bf = [&](int f) {
         const uint2 v = fr[f * 64 + lane];
         bf16x4 r;
@@ -326,82 +343,126 @@ __global__ void __launch_bounds__(512) k_mlp_warm(WarmArgs a) {
         return r;
     };
     for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
-        const long e = batch * 16L * nwave + 16L * wave + c;
-        const bool active = e < a.nelem;
-        const long slot = a.slot_begin + (active ? e : a.nelem - 1);
-        const int lev = a.slot_lev[slot];
-        const double Dt = a.ts->Dt;
-        double tau[D], tmax = 0.0;
+        long slot[NT];
+        bool active[NT];
+        float sc[NT];
+        bf16x4 xb[NT][KS1];
 #pragma unroll
-        for (int d = 0; d < D; ++d) { tau[d] = Dt / a.geo.dx[lev][d]; tmax = fmax(tmax, tau[d]); }
-        double avg[Q];
+        for (int tt = 0; tt < NT; ++tt) {
+            const long e = (batch * nwave + wave) * per_wave + 16L * tt + c;
+            active[tt] = e < a.nelem;
+            slot[tt] = a.slot_begin + (active[tt] ? e : a.nelem - 1);
+            const int lev = a.slot_lev[slot[tt]];
+            const double Dt = a.ts->Dt;
+            double tau[D], tmax = 0.0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) avg[q] = a.Uw[(long)q * Ns * a.NS + slot];
-        const double rho0 = avg[0];
-        double ke = 0.0;
+            for (int d = 0; d < D; ++d) { tau[d] = Dt / a.geo.dx[lev][d]; tmax = fmax(tmax, tau[d]); }
+            double avg[Q];
 #pragma unroll
-        for (int d = 0; d < D; ++d) ke += avg[d + 1] * avg[d + 1];
-        const double prs = (a.gamma - 1.0) * (avg[D + 1] - 0.5 * ke / rho0);
-        const double cs = sqrt(a.gamma * prs / rho0);
-        double nu = 0.0;
+            for (int q = 0; q < Q; ++q) avg[q] = a.Uw[(long)q * Ns * a.NS + slot[tt]];
+            const double rho0 = avg[0];
+            double ke = 0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) nu += tau[d] * (fabs(avg[d + 1] / rho0) + cs);
-        // layer 1 input: features 16 s + 4 g + t of element c
-        bf16x4 xb[KS1];
+            for (int d = 0; d < D; ++d) ke += avg[d + 1] * avg[d + 1];
+            const double prs = (a.gamma - 1.0) * (avg[D + 1] - 0.5 * ke / rho0);
+            const double cs = sqrt(a.gamma * prs / rho0);
+            double nu = 0.0;
 #pragma unroll
-        for (int s = 0; s < KS1; ++s) {
-            float v[4];
+            for (int d = 0; d < D; ++d) nu += tau[d] * (fabs(avg[d + 1] / rho0) + cs);
+            sc[tt] = (float)(rho0 * tmax);
+            // layer 1 input: features 16 s + 4 g + t of element c
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int f = 16 * s + 4 * g + t;
-                double xv = 0.0;
-                if (f < Q * Ns) xv = a.Uw[(long)f * a.NS + slot] / rho0;
-                else if (f == Q * Ns) xv = nu;
-                else if (f < NIN) xv = tau[f - Q * Ns - 1];
-                v[t] = f < NIN ? ((float)xv - mux[f]) / sdx[f] : 0.f;
+            for (int s = 0; s < KS1; ++s) {
+                float v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int f = 16 * s + 4 * g + t;
+                    double xv = 0.0;
+                    if (f < Q * Ns) xv = a.Uw[(long)f * a.NS + slot[tt]] / rho0;
+                    else if (f == Q * Ns) xv = nu;
+                    else if (f < NIN) xv = tau[f - Q * Ns - 1];
+                    v[t] = f < NIN ? ((float)xv - mux[f]) / sdx[f] : 0.f;
+                }
+                xb[tt][s] = pack_bf16(v[0], v[1], v[2], v[3]);
             }
-            xb[s] = pack_bf16(v[0], v[1], v[2], v[3]);
         }
-        bf16x4 h1[KS2];
+        bf16x4 h1[NT][KS2];
 #pragma unroll
         for (int u = 0; u < KS2; ++u) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            f32x4 acc[NT];
 #pragma unroll
-            for (int s = 0; s < KS1; ++s) acc = mma_bf16(bf(a.f1 + u * KS1 + s), xb[s], acc);
-            float v[4];
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[r] + a.b1[16 * u + 4 * g + r]);
-            h1[u] = pack_bf16(v[0], v[1], v[2], v[3]);
+            for (int s = 0; s < KS1; ++s) {
+                const bf16x4 w = bf(a.f1 + u * KS1 + s);
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) acc[tt] = mma_bf16(w, xb[tt][s], acc[tt]);
+            }
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[tt][r] + a.b1[16 * u + 4 * g + r]);
+                h1[tt][u] = pack_bf16(v[0], v[1], v[2], v[3]);
+            }
             __builtin_amdgcn_sched_barrier(0);   // keeps fragment reads next to their MFMAs
         }
-        bf16x4 h2[KS2];
+        bf16x4 h2[NT][KS2];
 #pragma unroll
         for (int u = 0; u < KS2; ++u) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            f32x4 acc[NT];
 #pragma unroll
-            for (int s = 0; s < KS2; ++s) acc = mma_bf16(bf(a.f2 + u * KS2 + s), h1[s], acc);
-            float v[4];
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[r] + a.b2[16 * u + 4 * g + r]);
-            h2[u] = pack_bf16(v[0], v[1], v[2], v[3]);
+            for (int s = 0; s < KS2; ++s) {
+                const bf16x4 w = bf(a.f2 + u * KS2 + s);
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) acc[tt] = mma_bf16(w, h1[tt][s], acc[tt]);
+            }
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[tt][r] + a.b2[16 * u + 4 * g + r]);
+                h2[tt][u] = pack_bf16(v[0], v[1], v[2], v[3]);
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
-        const float sc = (float)(rho0 * tmax);
 #pragma unroll
         for (int u = 0; u < UT3; ++u) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            f32x4 acc[NT];
 #pragma unroll
-            for (int s = 0; s < KS2; ++s) acc = mma_bf16(bf(a.f3 + u * KS2 + s), h2[s], acc);
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < KS2; ++s) {
+                const bf16x4 w = bf(a.f3 + u * KS2 + s);
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) acc[tt] = mma_bf16(w, h2[tt][s], acc[tt]);
+            }
             __builtin_amdgcn_sched_barrier(0);
-            if (active) {
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt) {
+                if (!active[tt]) continue;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int o = 16 * u + 4 * g + r;
-                    if (o < NOUT) a.warm[(long)o * a.NS + slot] = sc * (muy[o] + sdy[o] * (acc[r] + a.b3[o]));
+                    if (o < NOUT)
+                        a.warm[(long)o * a.NS + slot[tt]] = sc[tt] * (muy[o] + sdy[o] * (acc[tt][r] + a.b3[o]));
                 }
             }
         }
     }
+}
+
+// One tile per wavefront capped at 128 VGPRs (two 512-thread workgroups
+// per CU), or two tiles per wavefront sharing each fragment read.
+template <int P, int D, int NINP, int NOUTP>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_mlp_warm1(WarmArgs a) {
+    mlp_warm_body<P, D, NINP, NOUTP, 1>(a);
+}
+template <int P, int D, int NINP, int NOUTP>
+__global__ void __launch_bounds__(512) k_mlp_warm2(WarmArgs a) {
+    mlp_warm_body<P, D, NINP, NOUTP, 2>(a);
 }
 
 // Packs layer weights W [rows][cols] (FP32, row-major) into A-operand

@@ -257,10 +257,10 @@ class Context:
     def autotune(self, reps=10):
         """Times the predictor launch shapes and keeps the fastest; returns
         ({shape: ms}, (kind, waves))."""
-        ms = np.zeros(4)
-        ch = np.zeros(2, dtype=np.int32)
+        ms = np.zeros(6)
+        ch = np.zeros(3, dtype=np.int32)
         self._chk(self.L.adg_autotune(self.h, reps, _d(ms), _i(ch)), "adg_autotune")
-        names = ("valu", "quad8", "nodelane8", "nodelane16")
+        names = ("valu", "quad8", "nodelane8", "nodelane16", "warm1", "warm2")
         return {n: float(v) for n, v in zip(names, ms) if v >= 0}, (int(ch[0]), int(ch[1]))
 
     def project_ic_patches(self, lev, ic_case, params):

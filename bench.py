@@ -113,9 +113,10 @@ def main():
                     help="form reported as value (all three are measured)")
     ap.add_argument("--tol", type=float, default=1e-5,
                     help="learned form: accept the warm start when max|H_k - H_k-1| <= tol * rho0")
-    ap.add_argument("--max-iters", type=int, default=1, help="learned form: Picard iterations from the warm start")
+    ap.add_argument("--max-iters", type=int, default=2, help="learned form: Picard iterations from the warm start")
     ap.add_argument("--weights", default=os.path.join(PKG, "weights", "kh_p3_2d.f32"))
     ap.add_argument("--acc-steps", type=int, default=200, help="steps of the accuracy comparison (0 = skip)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch measurement")
     ap.add_argument("--kernel", type=int, default=0, help="predictor kernel: 0 auto, 1 VALU, 2 MFMA")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
@@ -161,6 +162,7 @@ def main():
 
     def measure(form):
         ctx = make(form)
+        tune = ctx.autotune(5)[1] if not args.kernel else None   # launch shapes for this hierarchy
         stream = ctx.stream()
         ctx.step_graph(c_dt, Tend, args.warmup)   # captures the step graph, then warms up
         ctx.sync()
@@ -187,11 +189,46 @@ def main():
         pred_ms = p0.elapsed_ms(p1) / kp
         dof = n_valid * ctx.Q * ctx.Np_st
         out = {"ms_per_step": ms_step, "value": dof / (ms_step * 1e-3), "pred_ms": pred_ms, "dof": dof,
-               "sizes": (ctx.Q, ctx.Np_s, ctx.Np_st, ctx.qMp_st, ctx.qMp_st_bd), "fallback_fraction": fell}
+               "sizes": (ctx.Q, ctx.Np_s, ctx.Np_st, ctx.qMp_st, ctx.qMp_st_bd), "fallback_fraction": fell,
+               "autotune": tune}
         ctx.close()
         return out
 
+    def measure_batch():
+        """512 independent periodic vortices, edges ~ U{16..128} (seeded),
+        beta ~ U[1,5], centres ~ U(patch), background ~ U[-1,1]^2, p=3,
+        FP32-MFMA, one launch per stage, autotuned predictor shape."""
+        from layouts import batch_patches
+        blev, bdom, edges = batch_patches(512, 16, 128, seed=2024)
+        dx = 10.0 / 64
+        rng = np.random.default_rng(2024)
+        prm = [[rng.uniform(1, 5), rng.uniform(0, n * dx), rng.uniform(0, n * dx), rng.uniform(-1, 1),
+                rng.uniform(-1, 1)] for n in edges]
+        ctx = pyadg.Context(D, P, 1.4, pyadg.ADG_FP32_MFMA)
+        ctx.set_independent_patches(True)
+        ctx.set_geometry(bdom, [0, 0], [bdom[0] * dx, bdom[1] * dx])
+        ctx.set_levels(blev)
+        ctx.project_ic_patches(0, 0, prm)
+        ctx.fill_ghosts()
+        ctx.set_dt(c_dt, Tend, 0.0)
+        tune, choice = ctx.autotune(5)
+        nb = 20
+        ctx.step_graph(c_dt, Tend, 3)
+        ctx.sync()
+        e0, e1 = Event(), Event()
+        e0.record(ctx.stream())
+        ctx.step_graph(c_dt, Tend, nb)
+        e1.record(ctx.stream())
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / nb
+        nel = int(sum(n * n for n in edges))
+        ctx.close()
+        return {"workload": "512 independent periodic vortices, edges U{16..128}, p=3, FP32-MFMA",
+                "elements": nel, "ms_per_step": ms, "value": nel * ctx.Q * ctx.Np_st / (ms * 1e-3),
+                "autotune_ms": tune, "autotune_choice": {"kind": choice[0], "waves": choice[1]}}
+
     res = {f: measure(f) for f in ("fp64", "fp32", "learned")}
+    batch = measure_batch() if not args.no_batch else None
     head = res[args.form]
     accuracy = accuracy_vs_fp64(make, levels, args.acc_steps, c_dt, Tend) if args.acc_steps > 0 else None
     Q, Ns, Nst, Mst, Mbd = head["sizes"]
@@ -227,7 +264,9 @@ def main():
                    "learned_tol": args.tol, "learned_max_iters": args.max_iters,
                    "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
                    "learned_over_fp64": res["learned"]["value"] / res["fp64"]["value"],
+                   "autotune": {f: res[f]["autotune"] for f in res},
                    "accuracy": accuracy,
+                   "batch": batch,
                    "step": "hipGraph replay of predictor, face flux, corrector, reflux+average down, "
                            "ghost fill+CFL+advance"},
         "roofline": roofline, "cpu_baseline": cpu,

@@ -136,9 +136,11 @@ int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
 /* Autotuner: times the predictor launch shapes that fit this context and
  * hierarchy ({VALU} for FP64, {quad-gather, 8 waves}, {node-lane, 8 waves},
  * {node-lane, 16 waves}) over `reps` launches each on the current state and
- * keeps the fastest.  ms_out[4] (nullable) gets the average launch time of
- * each candidate (-1 where it does not apply), choice_out[2] (nullable) the
- * chosen {kind, waves}.  No counterpart in the reference. */
+ * keeps the fastest; with a network loaded it also times the warm-start
+ * kernel with one and two 16-element tiles per wavefront.  ms_out[6]
+ * (nullable) gets the average launch time of each candidate (-1 where it
+ * does not apply), choice_out[3] (nullable) the chosen {kind, waves,
+ * warm-start tiles}.  No counterpart in the reference. */
 int adg_autotune(adg_ctx* ctx, int reps, double* ms_out, int* choice_out);
 /* Wavefronts per workgroup of the node-lane predictor: 8 (default) or 16.
  * No counterpart in the reference: a launch-shape knob of this

@@ -77,8 +77,9 @@ def test_autotuner_picks_a_valid_shape(adg):
         times, choice = ctx.autotune(5)
         print(f"  precision {prec}: {times} -> {choice}", flush=True)
         assert choice[0] in (1, 2, 3) and choice[1] in (8, 16)
-        assert min(times.values()) == pytest.approx(times[{(1, 8): "valu", (2, 8): "quad8", (3, 8): "nodelane8",
-                                                          (3, 16): "nodelane16"}[choice]])
+        pred = {k: v for k, v in times.items() if not k.startswith("warm")}
+        assert min(pred.values()) == pytest.approx(pred[{(1, 8): "valu", (2, 8): "quad8", (3, 8): "nodelane8",
+                                                        (3, 16): "nodelane16"}[choice]])
         t0 = ctx.totals()
         ctx.step_graph(0.5, 1.0, 3)
         t1 = ctx.totals()
