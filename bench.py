@@ -30,6 +30,10 @@ PEAK_FP64 = 78.6e12   # MI355X dense FP64 (vector and matrix)
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix
 PEAK_HBM = 8.0e12
 KH_PARAMS = [0.01, 1e-3, 20240611]
+# HBM bytes per launch of the (autotuned) node-lane predictor on this workload,
+# measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE and scaled by
+# the counters' reading of a plain 256 MiB device copy (x2.0 reads, x1.0 writes)
+PRED_TRAFFIC = {"fp64": 143.7e6, "fp32": 54.6e6, "learned": 54.6e6}
 
 
 def algorithmic_flops(P, D, Q, Ns, Nst, Mst, Mbd):
@@ -236,9 +240,10 @@ def main():
     achieved = f_pred * n_valid / (head["pred_ms"] * 1e-3)
     peak = PEAK_FP64 if args.form == "fp64" else PEAK_FP32
     roofline = {"bound": "mfma",
-                "kernel": f"k_predictor_nl<{'double' if args.form == 'fp64' else 'float'},3,2,8> (P Picard iterations)",
+                "kernel": f"k_predictor_nl<{'double' if args.form == 'fp64' else 'float'},3,2,16> (P Picard iterations)",
                 "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": None, "avg_launch_ms": head["pred_ms"], "flops_per_launch": f_pred * n_valid}
+                "traffic": PRED_TRAFFIC[args.form], "avg_launch_ms": head["pred_ms"],
+                "flops_per_launch": f_pred * n_valid}
 
     cpu = None
     if not args.no_cpu:
