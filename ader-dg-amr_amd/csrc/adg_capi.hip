@@ -524,9 +524,10 @@ int adg_download_Uw(adg_ctx* c, int lev, double* host) {
 int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
     if (!c) return ADG_E_ARG;
     if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 5)
+    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 6)
         return fail(c, ADG_E_ARG, "bad level, case or params");
-    if (ic_case <= 1 && c->D != 2) return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
+    if ((ic_case <= 1 || ic_case == 4 || ic_case == 6) && c->D != 2)
+        return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
     std::vector<double> host;
     adg::project_ic(c->tab, c->H, lev, ic_case, params, c->gamma, host);
     return copy_level(c, lev, c->Uw, c->tab.Np_s, host.data(), true);
@@ -535,9 +536,10 @@ int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
 int adg_project_ic_patches(adg_ctx* c, int lev, int ic_case, int nprm, const double* params) {
     if (!c) return ADG_E_ARG;
     if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    if (lev < 0 || lev >= c->H.nlev || !params || nprm < 1 || ic_case < 0 || ic_case > 5)
+    if (lev < 0 || lev >= c->H.nlev || !params || nprm < 1 || ic_case < 0 || ic_case > 6)
         return fail(c, ADG_E_ARG, "bad level, case or params");
-    if (ic_case <= 1 && c->D != 2) return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
+    if ((ic_case <= 1 || ic_case == 4 || ic_case == 6) && c->D != 2)
+        return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
     const adg::LevelLayout& L = c->H.lev[lev];
     std::vector<double> host((size_t)c->Q * c->tab.Np_s * L.nslot, 0.0);
     for (int p = L.p_begin; p < L.p_end; ++p)
@@ -601,7 +603,7 @@ int adg_average_down(adg_ctx* c) {
         const long ng = (long)c->H.ad_coarse[l].size();
         if (ng == 0) continue;
         adg::AvgArgs a{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
-#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(ng * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, a)
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(adg::avg_groups_padded(ng) * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, a)
         ADG_DISPATCH(c, L_AD);
 #undef L_AD
         HIPCHK(c, hipGetLastError());
@@ -626,6 +628,16 @@ int adg_set_time(adg_ctx* c, double t) {
     if (!c) return ADG_E_ARG;
     HIPCHK(c, hipMemcpyAsync(&c->ts->t, &t, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+int adg_get_time(adg_ctx* c, double* t_out, double* dt_out) {
+    if (!c) return ADG_E_ARG;
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, &c->ts->t, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t_out) *t_out = h[0];
+    if (dt_out) *dt_out = h[1];
     return ADG_OK;
 }
 
@@ -1478,7 +1490,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
         a.ad_fine = c->ad_fine[L - 1];
         a.ngroup = (long)c->H.ad_coarse[L - 1].size();
         a.nblk_a = nblocks(a.nrc * c->Q * c->tab.Np_s);
-        const unsigned nb = a.nblk_a + nblocks(a.ngroup * c->Q * c->tab.Np_s);
+        const unsigned nb = a.nblk_a + nblocks(adg::avg_groups_padded(a.ngroup) * c->Q * c->tab.Np_s);
         if (nb > 0) {
 #define L_RA(PP, DD) hipLaunchKernelGGL((adg::k_reflux_avg<T, PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)
             ADG_DISPATCH(c, L_RA);
@@ -1489,7 +1501,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
             const long ng = (long)c->H.ad_coarse[l].size();
             if (ng == 0) continue;
             adg::AvgArgs g{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
-#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(ng * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, g)
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(adg::avg_groups_padded(ng) * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, g)
             ADG_DISPATCH(c, L_AD);
 #undef L_AD
             HIPCHK(c, hipGetLastError());

@@ -18,7 +18,11 @@
 //     in the reference);
 //  4  Kelvin-Helmholtz shear layer as in 1 with the seeded multi-mode
 //     perturbation v = sum_{k=1..8} a_k sin(2 pi k x), a_k = amax * r_k,
-//     r_k a seeded uniform number (no counterpart in the reference).
+//     r_k a seeded uniform number (no counterpart in the reference);
+//  6  the reference's kelvin_helmolz_instability state
+//     (Compressible_Euler.h:237-256): v = w0 sin(4 pi x) times Gaussians of
+//     width sigma about y = 0.25 and 0.75 (params: w0, sigma; the reference
+//     uses w0 = 0.2, sigma = 0.05/sqrt 2).
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -84,6 +88,16 @@ void pointwise_state(int D, int ic_case, const double* prm, double gamma, const 
         const double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
         double u2 = 0.0;
         for (int k = 1; k <= 8; ++k) u2 += amax * mode_uniform(seed, k) * std::sin(2.0 * M_PI * k * x[0]);
+        u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
+        u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+    } else if (ic_case == 6) {
+        const double w0 = prm[0], sigma = prm[1];
+        const double y = x[1];
+        const bool in = (y > 0.25) && (y < 0.75);
+        const double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
+        const double two_s2 = 2.0 * sigma * sigma;
+        const double env = std::exp(-(y - 0.25) * (y - 0.25) / two_s2) + std::exp(-(y - 0.75) * (y - 0.75) / two_s2);
+        const double u2 = w0 * std::sin(4.0 * M_PI * x[0]) * env;
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
         u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
     } else if (ic_case == 5) {

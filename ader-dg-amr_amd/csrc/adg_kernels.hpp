@@ -913,41 +913,62 @@ template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
+    // directions whose face values are held in registers at once (all, but
+    // one at a time for the 3D FP64 form: 2 x 27 doubles per direction)
+    constexpr int DPRE = (D == 3 && sizeof(T) == 8) ? 1 : D;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nvalid * Q) return;
     const int q = (int)(t / a.nvalid);
     const long i = t - (long)q * a.nvalid;
-    const long slot = a.valid_list[i];
-    const int lev = a.slot_lev[slot];
-    const double Dt = a.ts->Dt;
     const long NS = a.NS;
     const double* __restrict__ Mkc = a.tab + a.to.Mkc;
     const double* __restrict__ sgn = a.tab + a.to.sgn;
+    // every load of the thread is issued before its arithmetic (two memory
+    // round trips: the lists, then the modes and face values)
+    long flo[D], fhi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) { flo[d] = a.cell_face[d][0][i]; fhi[d] = a.cell_face[d][1][i]; }
+    const long slot = a.valid_list[i];
+    const int lev = a.slot_lev[slot];
+    double u[Ns], mk[Ns];
+    T vol[Ns];
+#pragma unroll
+    for (int k = 0; k < Ns; ++k) {
+        u[k] = a.Uw[((long)q * Ns + k) * NS + slot];
+        vol[k] = a.Vol[vol_idx<P, D>(k, q, slot, NS)];
+        mk[k] = Mkc[k];   // held: the stores to Uw below may alias the table for the compiler
+    }
+    T Fh[DPRE][Mbd], Fl[DPRE][Mbd];
+    auto load_dir = [&](int d, T* fh, T* fl) {
+        const long nf = a.NF[d];
+        const T* __restrict__ ph = a.Fnum[d] + (long)q * Mbd * nf + fhi[d];
+        const T* __restrict__ pl = a.Fnum[d] + (long)q * Mbd * nf + flo[d];
+#pragma unroll
+        for (int mb = 0; mb < Mbd; ++mb) { fh[mb] = ph[mb * nf]; fl[mb] = pl[mb * nf]; }
+    };
+#pragma unroll
+    for (int d = 0; d < DPRE; ++d) load_dir(d, Fh[d], Fl[d]);
+    __builtin_amdgcn_sched_barrier(0);
+    const double Dt = a.ts->Dt;
     double r[Ns];
 #pragma unroll
-    for (int k = 0; k < Ns; ++k)
-        r[k] = Mkc[k] * a.Uw[((long)q * Ns + k) * NS + slot] + double(a.Vol[vol_idx<P, D>(k, q, slot, NS)]);
+    for (int k = 0; k < Ns; ++k) r[k] = mk[k] * u[k] + double(vol[k]);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
+        const int s = DPRE == D ? d : 0;
+        if (DPRE < D && d > 0) load_dir(d, Fh[0], Fl[0]);
         const double cdd = Dt / a.geo.dx[lev][d];
-        const long flo = a.cell_face[d][0][i], fhi = a.cell_face[d][1][i];
         const double* __restrict__ Mbp = a.tab + a.to.Mkbdp + (long)d * Ns * Mbd;
-        T Fh[Mbd], Fl[Mbd];
-        const long nf = a.NF[d];
-        const T* __restrict__ ph = a.Fnum[d] + (long)q * Mbd * nf + fhi;
-        const T* __restrict__ pl = a.Fnum[d] + (long)q * Mbd * nf + flo;
-#pragma unroll
-        for (int mb = 0; mb < Mbd; ++mb) { Fh[mb] = ph[mb * nf]; Fl[mb] = pl[mb * nf]; }
 #pragma unroll
         for (int k = 0; k < Ns; ++k) {
-            const T sh = face_moment<T, Mbd>(Mbp + k * Mbd, Fh);
-            const T sl = face_moment<T, Mbd>(Mbp + k * Mbd, Fl);
+            const T sh = face_moment<T, Mbd>(Mbp + k * Mbd, Fh[s]);
+            const T sl = face_moment<T, Mbd>(Mbp + k * Mbd, Fl[s]);
             r[k] -= cdd * double(sh);
             r[k] += cdd * (sgn[d * Ns + k] * double(sl));
         }
     }
 #pragma unroll
-    for (int k = 0; k < Ns; ++k) a.Uw[((long)q * Ns + k) * NS + slot] = r[k] / Mkc[k];
+    for (int k = 0; k < Ns; ++k) a.Uw[((long)q * Ns + k) * NS + slot] = r[k] / mk[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -960,7 +981,7 @@ __device__ __forceinline__ double register_moment(const double* __restrict__ tab
                                                   const T* __restrict__ Fnum_d, long nf, int d, int kind, int sel,
                                                   long f, int q, int n, double jac) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
+    constexpr int Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
     T F[Mbd];
     const T* __restrict__ pf = Fnum_d + (long)q * Mbd * nf + f;
 #pragma unroll
@@ -997,7 +1018,7 @@ struct MomArgs {
 template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.NM * Q * Ns) return;
     const int qn = (int)(t / a.NM);
@@ -1203,7 +1224,7 @@ struct EpiArgs {
 template <typename T, int P, int D>
 __device__ __forceinline__ void reflux_item(const EpiArgs<T>& a, long t) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
     if (t >= a.nrc * Q * Ns) return;
     const int qn = (int)(t / a.nrc);
     const long r = t - (long)qn * a.nrc;
@@ -1224,28 +1245,45 @@ __device__ __forceinline__ void reflux_item(const EpiArgs<T>& a, long t) {
     a.Uw[((long)q * Ns + n) * a.NS + slot] += inv_jac * (a.tab[a.to.Minv + n] * corr);
 }
 
-// u_c[n] = 2^-D Minv[n] sum_c sum_k P_fc[c](n,k) u_f[c][k], one thread per (group, q, n)
+// Threads of the average down: groups padded to whole wavefronts, so that the
+// (q, n) of a wavefront is uniform.
+__host__ __device__ __forceinline__ long avg_groups_padded(long ngroup) { return (ngroup + 63) & ~63L; }
+
+// u_c[n] = 2^-D Minv[n] sum_c sum_k P_fc[c](n,k) u_f[c][k], one thread per
+// (group, q, n), group fastest and padded to whole wavefronts: (q, n) is
+// wavefront-uniform, so the P_fc row is read with scalar loads, and the
+// children's modes are loaded in one memory round trip.
 template <int P, int D>
 __device__ __forceinline__ void avgdown_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
                                              const int* __restrict__ coarse, const int* __restrict__ fine,
                                              long NS, long ngroup, long t) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, NCH = Dm::NCH;
-    if (t >= ngroup * Q * Ns) return;
-    const int qn = (int)(t / ngroup);
-    const long g = t - (long)qn * ngroup;
+    const long gp = avg_groups_padded(ngroup);
+    if (t >= gp * Q * Ns) return;
+    const int qn = __builtin_amdgcn_readfirstlane((int)(t / gp));
+    const long g = t - (long)qn * gp;
+    if (g >= ngroup) return;
     const int q = qn / Ns, n = qn - q * Ns;
     const double* __restrict__ Pfc = tab + to.Pfc;
+    long fs[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) fs[c] = fine[g * NCH + c];
+    const long cs = coarse[g];
+    double u[NCH][Ns];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int k = 0; k < Ns; ++k) u[c][k] = Uw[((long)q * Ns + k) * NS + fs[c]];
     double s = 0.0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        const long fs = fine[g * NCH + c];
         double v = 0.0;
 #pragma unroll
-        for (int k = 0; k < Ns; ++k) v = fma(Pfc[(c * Ns + n) * Ns + k], Uw[((long)q * Ns + k) * NS + fs], v);
+        for (int k = 0; k < Ns; ++k) v = fma(Pfc[(c * Ns + n) * Ns + k], u[c][k], v);
         s += v;
     }
-    Uw[((long)q * Ns + n) * NS + coarse[g]] = (1.0 / NCH) * (tab[to.Minv + n] * s);
+    Uw[((long)q * Ns + n) * NS + cs] = (1.0 / NCH) * (tab[to.Minv + n] * s);
 }
 
 // ghost slot mode n: copy, or Minv[n] sum_k P_cf[child](n,k) u_c[k]
@@ -1306,12 +1344,18 @@ __global__ void __launch_bounds__(256) k_ghost_cfl(EpiArgs<T> a) {
         const long slot = a.valid_list[i];
         const int lev = a.slot_lev[slot];
         const double* __restrict__ phi0 = a.tab + a.to.phi0;
+        double u[Q][Ns];   // all modes in one memory round trip
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int n = 0; n < Ns; ++n) u[q][n] = a.Uw[((long)q * Ns + n) * a.NS + slot];
+        __builtin_amdgcn_sched_barrier(0);
         double s[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             double v = 0.0;
 #pragma unroll
-            for (int n = 0; n < Ns; ++n) v += a.Uw[((long)q * Ns + n) * a.NS + slot] * phi0[n];
+            for (int n = 0; n < Ns; ++n) v += u[q][n] * phi0[n];
             s[q] = v;
         }
         const double rinv = 1.0 / s[0];
@@ -1324,16 +1368,19 @@ __global__ void __launch_bounds__(256) k_ghost_cfl(EpiArgs<T> a) {
     }
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
     __shared__ double wmin[4];
+    __shared__ unsigned last;
     if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
     __syncthreads();
     if (threadIdx.x == 0) {
+        // one thread per block publishes the block minimum, then counts the
+        // block in: the device-scope fence (an L2 write-back on a multi-XCD
+        // part) is paid once per block, not once per wavefront
         double m = wmin[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin(m, wmin[w]);
         if (m < 1.0e300) atomicMin(&a.ts->dtmin_bits, (unsigned long long)__double_as_longlong(m));
+        __threadfence();
+        last = (atomicAdd(&a.ts->done, 1u) == a.ncfl_blocks - 1) ? 1u : 0u;
     }
-    __threadfence();
-    __shared__ unsigned last;
-    if (threadIdx.x == 0) last = (atomicAdd(&a.ts->done, 1u) == a.ncfl_blocks - 1) ? 1u : 0u;
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();

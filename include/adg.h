@@ -83,7 +83,11 @@ int adg_download_Uw(adg_ctx* ctx, int lev, double* host);
  *              a_k = amax * U[0,1), params {seed, amax}
  *   ic_case 5: smooth periodic density wave rho = 1 + amp prod_d
  *              sin(2 pi x_d / L_d), velocity (1, 0.5, 0.25), pressure p0,
- *              params {amp, p0} */
+ *              params {amp, p0}
+ *   ic_case 6: the reference's kelvin_helmolz_instability state
+ *              (Compressible_Euler.h:237-256), v = w0 sin(4 pi x) times
+ *              Gaussians of width sigma about y = 0.25, 0.75, params
+ *              {w0, sigma} (reference: 0.2, 0.05/sqrt(2)) */
 int adg_project_ic(adg_ctx* ctx, int lev, int ic_case, const double* params);
 /* As adg_project_ic with one parameter set of nprm values per patch of the
  * level (params [npatch][nprm]); for independent patches the coordinates are
@@ -189,6 +193,9 @@ int adg_step(adg_ctx* ctx, double c_dt, double T, double* t_out);
 int adg_step_graph(adg_ctx* ctx, double c_dt, double T, int nsteps, double* t_out);
 /* Sets the device-resident time. */
 int adg_set_time(adg_ctx* ctx, double t);
+/* Blocking copy of the device-resident time t and step size Dt (the members
+ * t, Dt of Solver, Core/Solver.h; either pointer nullable). */
+int adg_get_time(adg_ctx* ctx, double* t_out, double* dt_out);
 
 /* Conserved totals sum(vol * U_w[q][0]) over uncovered valid cells, FP64
  * (the quantity LpNorm_DG_AMR integrates over, AmrDG.h:1399-1578). */

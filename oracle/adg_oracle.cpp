@@ -1043,6 +1043,17 @@ static void ic_state(int D, int ic_case, const double* prm, double gamma, const 
         for (int k = 1; k <= 8; ++k) u2 += amax * mode_noise(seed, k) * std::sin(2.0 * M_PI * k * x[0]);
         double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
+    } else if (ic_case == 6) {
+        // Compressible_Euler.h:237-256: shear layer with a Gaussian-modulated
+        // v perturbation of amplitude w0 and width sigma about both layers
+        double w0 = prm[0], sig = prm[1], y = x[1];
+        bool in = (y > 0.25) && (y < 0.75);
+        double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
+        double g = std::exp(-(y - 0.25) * (y - 0.25) / (2.0 * sig * sig)) +
+                   std::exp(-(y - 0.75) * (y - 0.75) / (2.0 * sig * sig));
+        double u2 = w0 * std::sin(4.0 * M_PI * x[0]) * g;
+        double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+        u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
     } else if (ic_case == 5) {
         // smooth periodic density wave (unit-cube domain) advected by (1, 0.5, 0.25)
         double prod = 1.0;
