@@ -277,7 +277,7 @@ int pack_tables(adg_ctx* c) {
     for (int v = 0; v < 2; ++v) {
         adg::Fragments& f = c->frags[v];
         f = adg::build_fragments(t, fp64, v == 1);
-        c->fo[v] = adg::FragOff{f.A0, f.V, f.B, f.S, f.T, f.total};
+        c->fo[v] = adg::FragOff{f.A0, f.V, f.B, f.S, f.T, f.BT, f.total, f.tail ? 1 : 0};
         HIPCHK(c, hipMalloc(&c->d_frag[v], f.data.size() * c->esz));
         if (fp64) {
             HIPCHK(c, hipMemcpy(c->d_frag[v], f.data.data(), f.data.size() * sizeof(double), hipMemcpyHostToDevice));

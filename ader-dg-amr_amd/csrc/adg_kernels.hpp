@@ -80,7 +80,8 @@ struct TabOff {
 };
 // Offsets (in 64-entry fragments) of the MFMA A-operand fragment sets.
 struct FragOff {
-    int A0, V, B, S, T, total;
+    int A0, V, B, S, T, BT, total;
+    int tail;   // the B rows past the last full 16-row block use the BT fragments
 };
 
 struct LevelGeo {
@@ -312,6 +313,11 @@ struct MF<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // 16 blocks of 4x4x1: block b = lane/4 takes A row lane%4 and B column
+    // lane%4 of its lanes; lane 4b + j, register i holds D_b(i, j)
+    static __device__ __forceinline__ v4 mma4(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
     }
 };
 
@@ -577,6 +583,14 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     constexpr int MB2 = Dm::MB2, KSU = Dm::KSU, NB = Dm::NB, KSH = Dm::KSH, MBS = Dm::MBS, MBD = Dm::MBD;
     constexpr int TST = nl_tile_stride<T>();
     constexpr bool FULL = Mst % 16 == 0;
+    // FP32 with 4 modes past the last full 16-row block (p = 3: 20 = 16 + 4):
+    // those rows use the 4x4x1 16-block MFMA (a quarter of the cycles of a
+    // 16x16x4 step whose block would be 12/16 padding); block b = 4g + c/4 of
+    // K step (d, nb, s) contracts node 16 nb + 4 g + s, so its B operand is the
+    // same gv[d][s] register and the 4 lane groups hold partial sums over
+    // disjoint node sets, added at the end of the iteration
+    constexpr bool TAIL = sizeof(T) == 4 && MB2 >= 2 && Nst - 16 * (MB2 - 1) == 4;
+    constexpr int MBF = TAIL ? MB2 - 1 : MB2;
     static_assert(Q == 4, "the node-lane layout keeps the 4 variables of a node in one lane");
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* fr = reinterpret_cast<T*>(smem_raw);
@@ -662,6 +676,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
             v4 acc[MB2];
 #pragma unroll
             for (int mb = 0; mb < MB2; ++mb) acc[mb] = c0[mb];
+            v4 tacc = v4{0, 0, 0, 0};
 #pragma unroll 1
             for (int nb = 0; nb < NB; ++nb) {
                 T gv[D][4];
@@ -669,11 +684,25 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
 #pragma unroll
                 for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
-                    for (int d = 0; d < D; ++d)
+                    for (int d = 0; d < D; ++d) {
 #pragma unroll
-                        for (int mb = 0; mb < MB2; ++mb)
+                        for (int mb = 0; mb < MBF; ++mb)
                             acc[mb] = M::mma(fr[(a.fo.B + ((mb * D + d) * NB + nb) * 4 + s4) * 64 + lane], gv[d][s4],
                                              acc[mb]);
+                        if constexpr (TAIL)
+                            tacc = M::mma4(fr[(a.fo.BT + (d * NB + nb) * 4 + s4) * 64 + lane], gv[d][s4], tacc);
+                    }
+            }
+            if constexpr (TAIL) {
+                // rows 16(MB2-1) + i summed over the 4 lane groups; register 0
+                // of lane group g of the last block holds row 16(MB2-1) + g
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    tacc[i] += __shfl_xor(tacc[i], 16, 64);
+                    tacc[i] += __shfl_xor(tacc[i], 32, 64);
+                }
+                const T tv = g == 0 ? tacc[0] : g == 1 ? tacc[1] : g == 2 ? tacc[2] : tacc[3];
+                acc[MB2 - 1][0] = c0[MB2 - 1][0] + tv;
             }
 #pragma unroll
             for (int ks = 0; ks < KSH; ++ks) hb[ks] = acc[ks >> 2][ks & 3];

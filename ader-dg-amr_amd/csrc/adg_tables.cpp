@@ -379,9 +379,15 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows, bool node_lane
                 int node = 16 * nb + i, mode = 4 * ks + k;
                 return (node < Mst && mode < Nst) ? t.V[(size_t)node * Nst + mode] : 0.0;
             });
+    // FP32 node-lane form with 4 modes past the last full 16-row block (p = 3
+    // in 2D: 20 = 16 + 4): those rows go through the 4x4x1 16-block MFMA,
+    // whose block b = lane/4 contracts one node, instead of a 16-row block
+    // of which 12 rows would be padding
+    f.tail = node_lane && !fp64_rows && f.MB2 >= 2 && Nst - 16 * (f.MB2 - 1) == 4;
+    const int MBfull = f.tail ? f.MB2 - 1 : f.MB2;
     // H_w update: rows modes (blocks mb), K step (nb, r) takes knode(nb, r, k)
     f.B = (int)(v.size() / 64);
-    for (int mb = 0; mb < f.MB2; ++mb)
+    for (int mb = 0; mb < MBfull; ++mb)
         for (int d = 0; d < D; ++d)
             for (int nb = 0; nb < f.NB; ++nb)
                 for (int r = 0; r < 4; ++r)
@@ -408,6 +414,17 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows, bool node_lane
                     int row = 16 * mbd + lrow(i), mode = 4 * ks + k;
                     return (row < Mbd && mode < Nst) ? t.Vbd[((size_t)ds * Mbd + row) * Nst + mode] : 0.0;
                 });
+    // tail rows: lane (k, i) of K step (d, nb, r) holds row 16 (MB2-1) + (i & 3)
+    // of B_d at node 16 nb + 4 k + r, the node its 4x4 block contracts
+    f.BT = (int)(v.size() / 64);
+    if (f.tail)
+        for (int d = 0; d < D; ++d)
+            for (int nb = 0; nb < f.NB; ++nb)
+                for (int r = 0; r < 4; ++r)
+                    frag([&](int i, int k) {
+                        int mode = 16 * (f.MB2 - 1) + (i & 3), node = 16 * nb + 4 * k + r;
+                        return (mode < Nst && node < Mst) ? t.B[((size_t)d * Nst + mode) * Mst + node] : 0.0;
+                    });
     f.total = (int)(v.size() / 64);
     return f;
 }

@@ -47,7 +47,8 @@ ElementTables build_element_tables(int P, int D);
 // order: fragment f holds 64 values, lane l = row (l%16) + 16 * kidx (l/16).
 // Rows/columns beyond the matrix are zero.  Offsets count fragments.
 struct Fragments {
-    int A0 = 0, V = 0, B = 0, S = 0, T = 0, total = 0;
+    int A0 = 0, V = 0, B = 0, S = 0, T = 0, BT = 0, total = 0;
+    bool tail = false;   // B rows past the last full 16-row block as 4x4x1 16-block fragments
     int MB2 = 0, KSU = 0, NB = 0, KSH = 0, MBS = 0, MBD = 0;
     std::vector<double> data;  // [total][64]
 };
