@@ -321,6 +321,43 @@ struct MF<float> {
     }
 };
 
+// Work dealing of the persistent element kernels (predictors, warm start):
+// whole rounds give each workgroup nwave * unit consecutive elements (one
+// unit per wavefront, full cache lines); the remainder, less than a round, is
+// cut into units dealt kq per workgroup, so that the last partial round runs
+// on every workgroup (a few wavefronts each, on distinct SIMDs) instead of on
+// a few whole workgroups while the rest of the GPU idles.
+struct Deal {
+    long nfull;   // whole rounds
+    long rest0;   // first element of the remainder
+    long nrwb;    // units in the remainder
+    long kq;      // remainder units per workgroup
+};
+__device__ __forceinline__ Deal make_deal(long nelem, long unit, int nwave) {
+    Deal d;
+    const long round = (long)gridDim.x * nwave * unit;
+    d.nfull = nelem / round;
+    d.rest0 = d.nfull * round;
+    d.nrwb = (nelem - d.rest0 + unit - 1) / unit;
+    d.kq = (d.nrwb + gridDim.x - 1) / gridDim.x;
+    return d;
+}
+// First element of iteration it of this wavefront; false when it has none.
+__device__ __forceinline__ bool deal_next(const Deal& d, long it, int wave, long unit, int nwave, long* e0) {
+    if (it < d.nfull) {
+        *e0 = ((it * gridDim.x + blockIdx.x) * nwave + wave) * unit;
+        return true;
+    }
+    if (it == d.nfull && wave < d.kq) {
+        const long rb = (long)blockIdx.x * d.kq + wave;
+        if (rb < d.nrwb) {
+            *e0 = d.rest0 + rb * unit;
+            return true;
+        }
+    }
+    return false;
+}
+
 // Row stride (elements) of the node-lane predictor's per-wave LDS tile:
 // 16 nodes plus padding that keeps 16-byte alignment of each row.
 template <typename T>
@@ -397,11 +434,10 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
     const int q = col_ok ? c - e_loc * Q : 0;
     const int src_base = col_ok ? (lane - q) : lane;
     const T gm1 = T(a.gamma - 1.0);
-    const long per_batch = (long)nwave * EPM;
-    const long nbatch = (a.nelem + per_batch - 1) / per_batch;
-
-    for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
-        const long e = batch * per_batch + (long)wave * EPM + e_loc;
+    const Deal deal = make_deal(a.nelem, EPM, nwave);
+    long eb;
+    for (long it = 0; deal_next(deal, it, wave, EPM, nwave, &eb); ++it) {
+        const long e = eb + e_loc;
         const bool active = col_ok && e < a.nelem;
         const long slot = a.slot_begin + (e < a.nelem ? e : a.nelem - 1);
         const int lev = a.slot_lev[slot];
@@ -608,11 +644,9 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     const int ce = sizeof(T) == 8 ? (c & 3) : (c >> 2);
     const int cq = sizeof(T) == 8 ? (c >> 2) : (c & 3);
     const T gm1 = T(a.gamma - 1.0);
-    const long per_batch = (long)nwave * 4;
-    const long nbatch = (a.nelem + per_batch - 1) / per_batch;
-
-    for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
-        const long e0 = batch * per_batch + (long)wave * 4;
+    const Deal deal = make_deal(a.nelem, 4, nwave);
+    long e0;
+    for (long it = 0; deal_next(deal, it, wave, 4, nwave, &e0); ++it) {
         const bool active = e0 + ce < a.nelem;
         const long slot = a.slot_begin + (active ? e0 + ce : a.nelem - 1);
         const long slot_f = a.slot_begin + (e0 + g < a.nelem ? e0 + g : a.nelem - 1);

@@ -332,7 +332,7 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     const float* muy = a.norm + 2 * NINP;
     const float* sdy = muy + NOUTP;
     const long per_wave = 16L * NT;
-    const long nbatch = (a.nelem + per_wave * nwave - 1) / (per_wave * nwave);
+    const Deal deal = make_deal(a.nelem, per_wave, nwave);
     auto bf = [&](int f) {
         const uint2 v = fr[f * 64 + lane];
         bf16x4 r;
@@ -342,14 +342,15 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
         r[3] = (short)(v.y >> 16);
         return r;
     };
-    for (long batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
+    long eb;
+    for (long it = 0; deal_next(deal, it, wave, per_wave, nwave, &eb); ++it) {
         long slot[NT];
         bool active[NT];
         float sc[NT];
         bf16x4 xb[NT][KS1];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
-            const long e = (batch * nwave + wave) * per_wave + 16L * tt + c;
+            const long e = eb + 16L * tt + c;
             active[tt] = e < a.nelem;
             slot[tt] = a.slot_begin + (active[tt] ? e : a.nelem - 1);
             const int lev = a.slot_lev[slot[tt]];
