@@ -99,7 +99,7 @@ struct adg_ctx {
     unsigned long long* lstats = nullptr;     // [elements, fallbacks] of learned predictor launches
     bool learned = false;
     float learned_tol = 1e-2f;
-    int warm_tiles = 1;           // 16-element column tiles per wavefront of the warm-start kernel (1 or 2)
+    int warm_tiles = 1;           // warm-start tiles per wavefront: 1 or 2 of 16 elements, 0 = one of 8
     int warm_per_cu[3] = {0, 0, 0};
     int learned_iters = 1;
     std::string err;
@@ -724,8 +724,23 @@ int adg_set_predictor_waves(adg_ctx* c, int wpb) {
     return ADG_OK;
 }
 
+int adg_set_warm_tiles(adg_ctx* c, int tiles) {
+    if (!c) return ADG_E_ARG;
+    if (tiles < 0 || tiles > 2) return fail(c, ADG_E_ARG, "warm-start tiles must be 0, 1 or 2");
+    c->warm_tiles = tiles;
+    drop_graph(c);
+    return ADG_OK;
+}
+
 extern "C++" {
 // Warm-start increments of the slots [begin, begin + n) from the packed MLP.
+template <int P, int D, int NINP, int NOUTP>
+static const void* warm_kernel(int nt) {
+    if (nt == 0) return (const void*)adg::k_mlp_warm_half<P, D, NINP, NOUTP>;
+    if (nt == 2) return (const void*)adg::k_mlp_warm2<P, D, NINP, NOUTP>;
+    return (const void*)adg::k_mlp_warm1<P, D, NINP, NOUTP>;
+}
+
 template <int P, int D>
 static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
     constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
@@ -751,12 +766,12 @@ static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
     a.nelem = n;
     a.gamma = c->gamma;
     a.warm = c->warm;
-    const size_t lds = (size_t)m.nfrag * 64 * sizeof(uint2);
+    const size_t lds = (size_t)m.nfrag * 64 * sizeof(uint2) + adg::warm_const_bytes<NINP, NOUTP>();
     const int nt = c->warm_tiles;
-    const void* kern = nt == 2 ? (const void*)adg::k_mlp_warm2<P, D, NINP, NOUTP>
-                               : (const void*)adg::k_mlp_warm1<P, D, NINP, NOUTP>;
+    const void* kern = warm_kernel<P, D, NINP, NOUTP>(nt);
     if (c->warm_per_cu[nt] <= 0) return fail(c, ADG_E_STATE, "warm-start kernel not configured");
-    const long nbatch = (n + 128L * nt - 1) / (128L * nt);
+    const long per_wg = 8L * (nt == 0 ? 8 : 16 * nt);   // 8 wavefronts per workgroup
+    const long nbatch = (n + per_wg - 1) / per_wg;
     const long grid = std::min<long>(nbatch, (long)c->warm_per_cu[nt] * c->ncu);
     void* kargs[] = {&a};
     HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(512), kargs, lds, c->stream));
@@ -773,11 +788,10 @@ template <int P, int D>
 static int configure_mlp_warm(adg_ctx* c) {
     constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
     constexpr int NINP = 16 * ((Q * Ns + 1 + D + 15) / 16), NOUTP = 16 * ((Q * Nst + 15) / 16);
-    const size_t lds = (size_t)c->mlp.nfrag * 64 * sizeof(uint2);
+    const size_t lds = (size_t)c->mlp.nfrag * 64 * sizeof(uint2) + adg::warm_const_bytes<NINP, NOUTP>();
     if (lds > kLdsMax) return fail(c, ADG_E_UNSUP, "network fragments exceed LDS");
-    for (int nt = 1; nt <= 2; ++nt) {
-        const void* kern = nt == 2 ? (const void*)adg::k_mlp_warm2<P, D, NINP, NOUTP>
-                                   : (const void*)adg::k_mlp_warm1<P, D, NINP, NOUTP>;
+    for (int nt = 0; nt <= 2; ++nt) {
+        const void* kern = warm_kernel<P, D, NINP, NOUTP>(nt);
         HIPCHK(c, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         int nb = 0;
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 512, lds));
@@ -828,12 +842,13 @@ int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
     }
     c->pred_kind = cand[bi][0];
     c->nl_wpb = cand[bi][1];
-    // warm-start inference: one or two 16-element tiles per wavefront
-    if (ms_out) ms_out[4] = ms_out[5] = -1.0;
+    // warm-start inference: one or two 16-element tiles, or one 8-element
+    // tile, per wavefront
+    if (ms_out) ms_out[4] = ms_out[5] = ms_out[6] = -1.0;
     if (c->mlp.init) {
         int rc = mlp_pack(c);
         double wbest = 1e300;
-        for (int nt = 1; nt <= 2 && rc == ADG_OK; ++nt) {
+        for (int nt = 0; nt <= 2 && rc == ADG_OK; ++nt) {
             c->warm_tiles = nt;
 #define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, 0, c->H.NS)
             ADG_DISPATCH(c, L_MW);
@@ -845,7 +860,7 @@ int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
             float el = 0.f;
             HIPCHK(c, hipEventElapsedTime(&el, e0, e1));
             const double ms = (double)el / reps;
-            if (ms_out) ms_out[3 + nt] = ms;
+            if (ms_out) ms_out[nt == 0 ? 6 : 3 + nt] = ms;
             if (ms < wbest) { wbest = ms; bi = nt; }
         }
         if (rc != ADG_OK) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }

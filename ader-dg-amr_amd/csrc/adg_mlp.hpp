@@ -42,6 +42,21 @@ __device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c, float d) 
     r[3] = (short)bf16_bits(d);
     return r;
 }
+// The same rounding (to nearest, ties to even) by the gfx950 conversion
+// instruction v_cvt_pk_bf16_f32, two values per instruction.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x4 pack_bf16_hw(float a, float b, float c, float d) {
+    const bf16x2_t lo = __builtin_convertvector(f32x2_t{a, b}, bf16x2_t);
+    const bf16x2_t hi = __builtin_convertvector(f32x2_t{c, d}, bf16x2_t);
+    const unsigned ul = __builtin_bit_cast(unsigned, lo), uh = __builtin_bit_cast(unsigned, hi);
+    bf16x4 r;
+    r[0] = (short)(ul & 0xffffu);
+    r[1] = (short)(ul >> 16);
+    r[2] = (short)(uh & 0xffffu);
+    r[3] = (short)(uh >> 16);
+    return r;
+}
 // C += A B, 16x16x16: A[i][4k'+t] on lane i+16k' (t = 0..3 packed),
 // B[4k'+t][j] on lane j+16k', C row 4g+r of lane group g in register r.
 __device__ __forceinline__ f32x4 mma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -64,6 +79,18 @@ __device__ __forceinline__ float erf_as(float x) {
     return copysignf(y, x);
 }
 __device__ __forceinline__ float gelu_f(float a) { return 0.5f * a * (1.0f + erf_as(a * 0.70710678118654752f)); }
+// The same GELU written as (a + |a| erf(|a|/sqrt 2)) / 2 for the inference
+// kernel: no sign transfer, one FMA for the final combination.
+__device__ __forceinline__ float gelu_w(float a) {
+    const float z = fabsf(a) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+    float y = fmaf(t, 1.061405429f, -1.453152027f);
+    y = fmaf(t, y, 1.421413741f);
+    y = fmaf(t, y, -0.284496736f);
+    y = fmaf(t, y, 0.254829592f);
+    const float e = fmaf(-t * y, __expf(-z * z), 1.0f);   // erf(z), z >= 0
+    return 0.5f * fmaf(fabsf(a), e, a);
+}
 __device__ __forceinline__ float dgelu_f(float a) {
     return 0.5f * (1.0f + erf_as(a * 0.70710678118654752f)) + a * 0.39894228040143268f * __expf(-0.5f * a * a);
 }
@@ -314,9 +341,25 @@ struct WarmArgs {
     float* __restrict__ warm;          // [nout][NS]
 };
 
-// NT column tiles of 16 elements per wavefront share every fragment read.
-template <int P, int D, int NINP, int NOUTP, int NT>
+// Floats of the warm-start constants held in LDS after the fragments: mu_x,
+// sd_x [nin_p], mu_y, sd_y [nout_p], b1, b2 [hidden], b3 [nout_p]; then the
+// cell sizes of every level (kMaxLev x 3 doubles).
+template <int NINP, int NOUTP>
+__host__ __device__ constexpr int warm_const_floats() {
+    return 2 * NINP + 3 * NOUTP + 2 * kMlpHidden;
+}
+template <int NINP, int NOUTP>
+__host__ __device__ constexpr int warm_const_bytes() {
+    return (warm_const_floats<NINP, NOUTP>() * 4 + kMaxLev * 3 * 8 + 15) / 16 * 16;
+}
+
+// NT column tiles of 16 columns per wavefront share every fragment read.  A
+// tile holds ELT = 16 elements, or ELT = 8 with columns 8-15 repeating 0-7
+// (twice the wavefronts, each with half the activation work: the kernel is
+// latency-bound at the sizes of a step, not MFMA-bound).
+template <int P, int D, int NINP, int NOUTP, int NT, int ELT>
 __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
+    static_assert(ELT == 16 || ELT == 8, "8 or 16 elements per tile");
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst;
     constexpr int NIN = Q * Ns + 1 + D, NOUT = Q * Nst, NH = kMlpHidden;
@@ -324,14 +367,32 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     uint2* fr = reinterpret_cast<uint2*>(smem_raw);
     stage_to_lds(fr, a.frag, a.nfrag * 64 * (int)sizeof(uint2));
+    // normalisation, biases and cell sizes in LDS as well: read from global
+    // memory inside the layer loops they cost a memory round trip per block
+    float* cst = reinterpret_cast<float*>(fr + a.nfrag * 64);
+    double* ldx = reinterpret_cast<double*>(cst + warm_const_floats<NINP, NOUTP>());
+    constexpr int NNORM = 2 * NINP + 2 * NOUTP;
+    for (int i = threadIdx.x; i < warm_const_floats<NINP, NOUTP>(); i += blockDim.x) {
+        float v;
+        if (i < NNORM) v = (i >= NINP && i < 2 * NINP) ? 1.f / a.norm[i] : a.norm[i];   // 1 / sd_x
+        else if (i < NNORM + NH) v = a.b1[i - NNORM];
+        else if (i < NNORM + 2 * NH) v = a.b2[i - NNORM - NH];
+        else v = (i - NNORM - 2 * NH < NOUT) ? a.b3[i - NNORM - 2 * NH] : 0.f;
+        cst[i] = v;
+    }
+    for (int i = threadIdx.x; i < kMaxLev * 3; i += blockDim.x) ldx[i] = a.geo.dx[i / 3][i % 3];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int g = lane >> 4, c = lane & 15;
-    const float* mux = a.norm;
-    const float* sdx = a.norm + NINP;
-    const float* muy = a.norm + 2 * NINP;
+    const float* mux = cst;
+    const float* rsdx = cst + NINP;
+    const float* muy = cst + 2 * NINP;
     const float* sdy = muy + NOUTP;
-    const long per_wave = 16L * NT;
+    const float* b1 = cst + NNORM;
+    const float* b2 = b1 + NH;
+    const float* b3 = b2 + NH;
+    const long per_wave = (long)ELT * NT;
+    const int ce = c & (ELT - 1);   // element of column c
     const Deal deal = make_deal(a.nelem, per_wave, nwave);
     auto bf = [&](int f) {
         const uint2 v = fr[f * 64 + lane];
@@ -348,28 +409,44 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
         bool active[NT];
         float sc[NT];
         bf16x4 xb[NT][KS1];
+        // every global load of the features first, in one memory round trip
+        int levs[NT];
+        double avgs[NT][Q], uf[NT][KS1][4];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
-            const long e = eb + 16L * tt + c;
-            active[tt] = e < a.nelem;
-            slot[tt] = a.slot_begin + (active[tt] ? e : a.nelem - 1);
-            const int lev = a.slot_lev[slot[tt]];
-            const double Dt = a.ts->Dt;
+            const long e = eb + (long)ELT * tt + ce;
+            active[tt] = e < a.nelem && c < ELT;   // columns past ELT repeat, not stored
+            slot[tt] = a.slot_begin + (e < a.nelem ? e : a.nelem - 1);
+            levs[tt] = a.slot_lev[slot[tt]];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) avgs[tt][q] = a.Uw[(long)q * Ns * a.NS + slot[tt]];
+#pragma unroll
+            for (int s = 0; s < KS1; ++s)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int f = 16 * s + 4 * g + t;
+                    uf[tt][s][t] = f < Q * Ns ? a.Uw[(long)f * a.NS + slot[tt]] : 0.0;
+                }
+        }
+        const double Dt = a.ts->Dt;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt) {
+            const int lev = levs[tt];
             double tau[D], tmax = 0.0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) { tau[d] = Dt / a.geo.dx[lev][d]; tmax = fmax(tmax, tau[d]); }
-            double avg[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) avg[q] = a.Uw[(long)q * Ns * a.NS + slot[tt]];
+            for (int d = 0; d < D; ++d) { tau[d] = Dt / ldx[lev * 3 + d]; tmax = fmax(tmax, tau[d]); }
+            const double* avg = avgs[tt];
             const double rho0 = avg[0];
+            const double rrho = 1.0 / rho0;
             double ke = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) ke += avg[d + 1] * avg[d + 1];
-            const double prs = (a.gamma - 1.0) * (avg[D + 1] - 0.5 * ke / rho0);
-            const double cs = sqrt(a.gamma * prs / rho0);
+            const double prs = (a.gamma - 1.0) * (avg[D + 1] - 0.5 * ke * rrho);
+            const double cs = sqrt(a.gamma * prs * rrho);
             double nu = 0.0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) nu += tau[d] * (fabs(avg[d + 1] / rho0) + cs);
+            for (int d = 0; d < D; ++d) nu += tau[d] * (fabs(avg[d + 1] * rrho) + cs);
             sc[tt] = (float)(rho0 * tmax);
             // layer 1 input: features 16 s + 4 g + t of element c
 #pragma unroll
@@ -379,12 +456,12 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
                 for (int t = 0; t < 4; ++t) {
                     const int f = 16 * s + 4 * g + t;
                     double xv = 0.0;
-                    if (f < Q * Ns) xv = a.Uw[(long)f * a.NS + slot[tt]] / rho0;
+                    if (f < Q * Ns) xv = uf[tt][s][t] * rrho;
                     else if (f == Q * Ns) xv = nu;
                     else if (f < NIN) xv = tau[f - Q * Ns - 1];
-                    v[t] = f < NIN ? ((float)xv - mux[f]) / sdx[f] : 0.f;
+                    v[t] = f < NIN ? ((float)xv - mux[f]) * rsdx[f] : 0.f;
                 }
-                xb[tt][s] = pack_bf16(v[0], v[1], v[2], v[3]);
+                xb[tt][s] = pack_bf16_hw(v[0], v[1], v[2], v[3]);
             }
         }
         bf16x4 h1[NT][KS2];
@@ -403,8 +480,8 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
             for (int tt = 0; tt < NT; ++tt) {
                 float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[tt][r] + a.b1[16 * u + 4 * g + r]);
-                h1[tt][u] = pack_bf16(v[0], v[1], v[2], v[3]);
+                for (int r = 0; r < 4; ++r) v[r] = gelu_w(acc[tt][r] + b1[16 * u + 4 * g + r]);
+                h1[tt][u] = pack_bf16_hw(v[0], v[1], v[2], v[3]);
             }
             __builtin_amdgcn_sched_barrier(0);   // keeps fragment reads next to their MFMAs
         }
@@ -424,8 +501,8 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
             for (int tt = 0; tt < NT; ++tt) {
                 float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = gelu_f(acc[tt][r] + a.b2[16 * u + 4 * g + r]);
-                h2[tt][u] = pack_bf16(v[0], v[1], v[2], v[3]);
+                for (int r = 0; r < 4; ++r) v[r] = gelu_w(acc[tt][r] + b2[16 * u + 4 * g + r]);
+                h2[tt][u] = pack_bf16_hw(v[0], v[1], v[2], v[3]);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -448,7 +525,7 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
                 for (int r = 0; r < 4; ++r) {
                     const int o = 16 * u + 4 * g + r;
                     if (o < NOUT)
-                        a.warm[(long)o * a.NS + slot[tt]] = sc[tt] * (muy[o] + sdy[o] * (acc[tt][r] + a.b3[o]));
+                        a.warm[(long)o * a.NS + slot[tt]] = sc[tt] * (muy[o] + sdy[o] * (acc[tt][r] + b3[o]));
                 }
             }
         }
@@ -459,11 +536,15 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
 // per CU), or two tiles per wavefront sharing each fragment read.
 template <int P, int D, int NINP, int NOUTP>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_mlp_warm1(WarmArgs a) {
-    mlp_warm_body<P, D, NINP, NOUTP, 1>(a);
+    mlp_warm_body<P, D, NINP, NOUTP, 1, 16>(a);
 }
 template <int P, int D, int NINP, int NOUTP>
 __global__ void __launch_bounds__(512) k_mlp_warm2(WarmArgs a) {
-    mlp_warm_body<P, D, NINP, NOUTP, 2>(a);
+    mlp_warm_body<P, D, NINP, NOUTP, 2, 16>(a);
+}
+template <int P, int D, int NINP, int NOUTP>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_mlp_warm_half(WarmArgs a) {
+    mlp_warm_body<P, D, NINP, NOUTP, 1, 8>(a);
 }
 
 // Packs layer weights W [rows][cols] (FP32, row-major) into A-operand

@@ -562,7 +562,7 @@ class AmrDG : public Solver<AmrDG<P>> {
     int Q = SPACEDIM + 2;
     int precision_ = ADG_FP64;
     std::string weights_;
-    double learned_tol_ = 1e-5;
+    double learned_tol_ = 1e-4;
     int learned_iters_ = 2;
     int graph_batch_ = 64;
     int autotune_reps_ = 0;

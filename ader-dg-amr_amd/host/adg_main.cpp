@@ -22,7 +22,7 @@ namespace {
 struct Options {
     std::string simulation_case = "isentropic_vortex";
     int p = 2, n = -1, max_level = -1, max_steps = -1, graph_steps = 64, max_iters = 2;
-    double T = -1.0, c_dt = -1.0, tol = 1e-5;
+    double T = -1.0, c_dt = -1.0, tol = 1e-4;
     std::string precision = "fp64", weights, dump;
     bool quiet = false;
 };
@@ -129,7 +129,6 @@ int run(const Options& opt) {
     sim.solver->setVerbose(!opt.quiet);
 
     // ------------------------------ RUN (main.cpp:156-158)
-    auto m0 = std::vector<double>();
     sim.run();
     auto& s = *sim.solver;
     const std::vector<double> l1 = s.L1Norm_DG_AMR(std::static_pointer_cast<ModelEquation<Compressible_Euler>>(sim.model));

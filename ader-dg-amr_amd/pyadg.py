@@ -25,7 +25,7 @@ EXPORTS = [
     "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
     "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_set_time", "adg_get_time",
     "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
-    "adg_project_ic", "adg_set_predictor_kernel", "adg_set_predictor_waves", "adg_last_error", "adg_destroy",
+    "adg_project_ic", "adg_set_predictor_kernel", "adg_set_predictor_waves", "adg_set_warm_tiles", "adg_last_error", "adg_destroy",
     "adg_mlp_sizes", "adg_mlp_init", "adg_mlp_get_params", "adg_mlp_set_params", "adg_mlp_collect",
     "adg_mlp_dataset", "adg_mlp_dataset_clear", "adg_mlp_fit_normalization", "adg_mlp_train",
     "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats", "adg_lp_error",
@@ -77,6 +77,7 @@ def lib():
     L.adg_project_ic.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
     L.adg_set_predictor_kernel.argtypes = [C.c_void_p, C.c_int]
     L.adg_set_predictor_waves.argtypes = [C.c_void_p, C.c_int]
+    L.adg_set_warm_tiles.argtypes = [C.c_void_p, C.c_int]
     _fp = C.POINTER(C.c_float)
     _lp = C.POINTER(C.c_long)
     _up = C.POINTER(C.c_ulonglong)
@@ -177,6 +178,10 @@ class Context:
         """Wavefronts per workgroup of the node-lane predictor (8 or 16)."""
         self._chk(self.L.adg_set_predictor_waves(self.h, waves), "adg_set_predictor_waves")
 
+    def set_warm_tiles(self, tiles):
+        """Warm-start inference tiles per wavefront: 1, 2 (16 elements each) or 0 (8 elements)."""
+        self._chk(self.L.adg_set_warm_tiles(self.h, tiles), "adg_set_warm_tiles")
+
     def keep_Hw(self, on=True):
         self._chk(self.L.adg_keep_Hw(self.h, 1 if on else 0), "adg_keep_Hw")
 
@@ -264,10 +269,10 @@ class Context:
     def autotune(self, reps=10):
         """Times the predictor launch shapes and keeps the fastest; returns
         ({shape: ms}, (kind, waves))."""
-        ms = np.zeros(6)
+        ms = np.zeros(7)
         ch = np.zeros(3, dtype=np.int32)
         self._chk(self.L.adg_autotune(self.h, reps, _d(ms), _i(ch)), "adg_autotune")
-        names = ("valu", "quad8", "nodelane8", "nodelane16", "warm1", "warm2")
+        names = ("valu", "quad8", "nodelane8", "nodelane16", "warm1", "warm2", "warm_half")
         return {n: float(v) for n, v in zip(names, ms) if v >= 0}, (int(ch[0]), int(ch[1]))
 
     def project_ic_patches(self, lev, ic_case, params):
@@ -366,7 +371,7 @@ class Context:
         self._chk(self.L.adg_mlp_warm_start(self.h, lev, _d(out)), "adg_mlp_warm_start")
         return out
 
-    def set_learned(self, on=True, tol=1e-2, max_iters=1):
+    def set_learned(self, on=True, tol=1e-4, max_iters=2):
         """on: False/0 plain, True/1 learned start with early exit and
         fallback, 2 diagnostic (exactly max_iters iterations, no check)."""
         self._chk(self.L.adg_set_learned(self.h, int(on), tol, max_iters), "adg_set_learned")

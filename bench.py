@@ -115,7 +115,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["fp64", "fp32", "learned"], default="learned",
                     help="form reported as value (all three are measured)")
-    ap.add_argument("--tol", type=float, default=1e-5,
+    ap.add_argument("--tol", type=float, default=1e-4,
                     help="learned form: accept the warm start when max|H_k - H_k-1| <= tol * rho0")
     ap.add_argument("--max-iters", type=int, default=2, help="learned form: Picard iterations from the warm start")
     ap.add_argument("--weights", default=os.path.join(PKG, "weights", "kh_p3_2d.f32"))

@@ -141,15 +141,20 @@ int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
  * hierarchy ({VALU} for FP64, {quad-gather, 8 waves}, {node-lane, 8 waves},
  * {node-lane, 16 waves}) over `reps` launches each on the current state and
  * keeps the fastest; with a network loaded it also times the warm-start
- * kernel with one and two 16-element tiles per wavefront.  ms_out[6]
- * (nullable) gets the average launch time of each candidate (-1 where it
- * does not apply), choice_out[3] (nullable) the chosen {kind, waves,
- * warm-start tiles}.  No counterpart in the reference. */
+ * kernel with one and two 16-element tiles and one 8-element tile per
+ * wavefront.  ms_out[7] (nullable) gets the average launch time of each
+ * candidate (-1 where it does not apply), choice_out[3] (nullable) the
+ * chosen {kind, waves, warm-start tiles (1, 2, or 0 for the 8-element
+ * tile)}.  No counterpart in the reference. */
 int adg_autotune(adg_ctx* ctx, int reps, double* ms_out, int* choice_out);
 /* Wavefronts per workgroup of the node-lane predictor: 8 (default) or 16.
  * No counterpart in the reference: a launch-shape knob of this
  * implementation, set by the autotuner. */
 int adg_set_predictor_waves(adg_ctx* ctx, int waves_per_block);
+/* Tiles per wavefront of the warm-start inference: 1 (16 elements, default),
+ * 2 (two 16-element tiles) or 0 (one 8-element tile).  No counterpart in the
+ * reference: a launch-shape knob, set by the autotuner. */
+int adg_set_warm_tiles(adg_ctx* ctx, int tiles);
 
 /* Debug readback of the last predictor H_w of a level, [q][Np_st][slot]
  * (only after adg_keep_Hw(ctx,1)). */

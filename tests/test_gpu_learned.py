@@ -156,6 +156,13 @@ def test_training_reduces_loss_and_inference_matches_oracle(adg, orc):
     e = np.linalg.norm(delta - ref) / np.linalg.norm(ref)
     print(f"  warm start normwise relative difference {e:.2e}", flush=True)
     assert e < 1e-2
+    # the other launch shapes of the inference kernel give the same increments
+    for tiles in (2, 0, 1):
+        ctx.set_warm_tiles(tiles)
+        d2 = ctx.mlp_warm_start(0)
+        e2 = np.abs(d2 - delta).max() / np.abs(delta).max()
+        print(f"  warm tiles {tiles}: max difference to the default shape {e2:.2e}", flush=True)
+        assert e2 <= 1e-6
 
 
 def test_learned_predictor_accuracy_and_gain(adg, orc):
