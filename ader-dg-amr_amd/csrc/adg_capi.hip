@@ -885,7 +885,7 @@ int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     const bool learned = mode == 1;
     int rc = ADG_OK;
     if (learned) {
-        if (variant != PV_NL) return fail(c, ADG_E_UNSUP, "the learned warm start runs in the node-lane predictor");
+        if (variant == PV_VALU) return fail(c, ADG_E_UNSUP, "the learned warm start runs in the MFMA predictors");
         if (!c->mlp.init) return fail(c, ADG_E_STATE, "no network: adg_mlp_init / adg_mlp_set_params first");
         if ((rc = mlp_pack(c)) != ADG_OK) return rc;
         if (n_fallback) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));

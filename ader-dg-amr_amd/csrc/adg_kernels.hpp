@@ -383,6 +383,13 @@ __device__ __forceinline__ void stage_to_lds(void* dst, const void* src, int nby
     for (; i < n; i += step) d[i] = s[i];
 }
 
+// max that propagates NaN from either operand (fmax drops a NaN operand,
+// which would let a residual of NaN pass an acceptance test)
+template <typename T>
+__device__ __forceinline__ T nanmax(T a, T b) {
+    return (a > b || a != a) ? a : b;
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -424,6 +431,8 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
     T* fr = reinterpret_cast<T*>(smem_raw);
     const int nfr = a.fo.total * 64;
     stage_to_lds(fr, a.frags, nfr * (int)sizeof(T));
+    __shared__ unsigned long long wg_stats[2];
+    if (threadIdx.x == 0) { wg_stats[0] = 0ull; wg_stats[1] = 0ull; }
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -460,8 +469,8 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
 #pragma unroll
         for (int ks = 0; ks < KSU; ++ks) hu[ks] = hb[ks];
 
-        for (int it = 0; it < a.iters; ++it) {
-            v4 acc[MB2];
+        // C0 = A0 U_w into acc
+        auto c0_into = [&](v4 (&acc)[MB2]) {
 #pragma unroll
             for (int mb = 0; mb < MB2; ++mb) acc[mb] = v4{0, 0, 0, 0};
 #pragma unroll
@@ -469,6 +478,29 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
 #pragma unroll
                 for (int mb = 0; mb < MB2; ++mb)
                     acc[mb] = M::mma(fr[(a.fo.A0 + mb * KSU + ks) * 64 + lane], hu[ks], acc[mb]);
+        };
+        // accumulator layout -> B operand layout (mode 4ks+g).  FP64: register
+        // r of block mb holds mode 16mb+4r+g = 4(4mb+r)+g already; FP32
+        // (mode 16mb+4g+r) goes through the per-wave LDS buffer.
+        auto relay = [&](const v4 (&acc)[MB2]) {
+            if constexpr (sizeof(T) == 8) {
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) hb[ks] = acc[ks >> 2][ks & 3];
+            } else {
+#pragma unroll
+                for (int mb = 0; mb < MB2; ++mb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) xch[(16 * mb + acc_row<T>(g, r)) * 16 + c] = acc[mb][r];
+                wave_lds_sync();
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) hb[ks] = xch[(4 * ks + g) * 16 + c];
+                wave_lds_sync();
+            }
+        };
+        // one Picard iteration H_w <- C0 + sum_d B_d g_d(H_w)
+        auto picard = [&]() {
+            v4 acc[MB2];
+            c0_into(acc);
 #pragma unroll 1
             for (int nb = 0; nb < NB; ++nb) {
                 v4 h = v4{0, 0, 0, 0};
@@ -495,22 +527,80 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                             acc[mb] = M::mma(fr[(a.fo.B + ((mb * D + d) * NB + nb) * 4 + r) * 64 + lane], gd[d][r],
                                              acc[mb]);
             }
-            // accumulator layout -> B operand layout (mode 4ks+g).  FP64: register
-            // r of block mb holds mode 16mb+4r+g = 4(4mb+r)+g already; FP32
-            // (mode 16mb+4g+r) goes through the per-wave LDS buffer.
-            if constexpr (sizeof(T) == 8) {
-#pragma unroll
-                for (int ks = 0; ks < KSH; ++ks) hb[ks] = acc[ks >> 2][ks & 3];
-            } else {
-#pragma unroll
-                for (int mb = 0; mb < MB2; ++mb)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) xch[(16 * mb + acc_row<T>(g, r)) * 16 + c] = acc[mb][r];
-                wave_lds_sync();
-#pragma unroll
-                for (int ks = 0; ks < KSH; ++ks) hb[ks] = xch[(4 * ks + g) * 16 + c];
-                wave_lds_sync();
+            relay(acc);
+        };
+
+        int nplain = a.iters;
+        bool keep_learned = false;   // this column's element passed the check
+        T hl[KSH];
+        if (a.warm) {
+            // learned start H_0 = A0 U_w + Delta and Picard iterations until
+            // the update of an element passes the tolerance; the decision is
+            // per element (its data only), elements that do not pass are
+            // recomputed by the plain predictor from its own start
+            const T inv_sc = T(1.0 / fabs(a.Uw[slot]));   // 1 / rho0 of the column's element
+            {
+                v4 acc[MB2];
+                c0_into(acc);
+                relay(acc);
             }
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) {
+                const int mode = 4 * ks + g;
+                if (col_ok && mode < Nst) hb[ks] += T(a.warm[((long)q * Nst + mode) * a.NS + slot]);
+            }
+            bool pass_e = false;
+            if (a.tol < 0.f) {
+                for (int k = 0; k < a.learned_iters; ++k) picard();   // diagnostic: no check
+                pass_e = true;
+            } else {
+                for (int k = 0; k < a.learned_iters; ++k) {
+                    T prev[KSH];
+#pragma unroll
+                    for (int ks = 0; ks < KSH; ++ks) prev[ks] = hb[ks];
+                    picard();
+                    T r = T(0);
+#pragma unroll
+                    for (int ks = 0; ks < KSH; ++ks) r = nanmax(r, T(fabs(hb[ks] - prev[ks])));
+                    r = col_ok ? r * inv_sc : T(0);
+                    // max over the element's lanes: 4 lane groups x its Q columns
+                    r = nanmax(r, __shfl_xor(r, 16, 64));
+                    r = nanmax(r, __shfl_xor(r, 32, 64));
+                    T re = r;
+#pragma unroll
+                    for (int j = 0; j < Q; ++j) re = nanmax(re, __shfl(r, src_base + j, 64));
+                    if (pass_e) {
+#pragma unroll
+                        for (int ks = 0; ks < KSH; ++ks) hb[ks] = prev[ks];
+                    } else {
+                        pass_e = re <= T(a.tol);   // NaN fails
+                    }
+                    if (__all(pass_e)) break;
+                }
+            }
+            // one representative lane per element (lane group 0, variable 0)
+            const bool rep = g == 0 && q == 0 && active;
+            const unsigned long long n_el = __popcll(__ballot(rep));
+            const unsigned long long n_fb = __popcll(__ballot(rep && !pass_e));
+            if (lane == 0 && n_el) {   // workgroup totals in LDS, one global update per workgroup
+                atomicAdd(&wg_stats[0], n_el);
+                if (n_fb) atomicAdd(&wg_stats[1], n_fb);
+            }
+            if (__all(pass_e)) {
+                nplain = 0;
+            } else {
+                keep_learned = pass_e;
+#pragma unroll
+                for (int ks = 0; ks < KSH; ++ks) {
+                    hl[ks] = hb[ks];
+                    hb[ks] = ks < KSU ? hu[ks] : T(0);
+                }
+            }
+        }
+        for (int it = 0; it < nplain; ++it) picard();
+        if (keep_learned) {
+#pragma unroll
+            for (int ks = 0; ks < KSH; ++ks) hb[ks] = hl[ks];
         }
 
         // final evaluation: corrector volume term
@@ -576,6 +666,13 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                     }
                 }
             }
+    }
+    if (a.warm) {
+        __syncthreads();
+        if (threadIdx.x == 0 && wg_stats[0]) {
+            atomicAdd(a.stats, wg_stats[0]);
+            if (wg_stats[1]) atomicAdd(a.stats + 1, wg_stats[1]);
+        }
     }
 }
 
@@ -774,14 +871,14 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
                     picard();
                     T r = T(0);
 #pragma unroll
-                    for (int ks = 0; ks < KSH; ++ks) r = fmax(r, fabs(hb[ks] - prev[ks]));
+                    for (int ks = 0; ks < KSH; ++ks) r = nanmax(r, T(fabs(hb[ks] - prev[ks])));
                     r *= inv_sc;
                     // max over the 16 lanes of the element: its 4 columns x 4 lane groups
                     constexpr int QX0 = sizeof(T) == 8 ? 4 : 1, QX1 = sizeof(T) == 8 ? 8 : 2;
-                    r = fmax(r, __shfl_xor(r, QX0, 64));
-                    r = fmax(r, __shfl_xor(r, QX1, 64));
-                    r = fmax(r, __shfl_xor(r, 16, 64));
-                    r = fmax(r, __shfl_xor(r, 32, 64));
+                    r = nanmax(r, __shfl_xor(r, QX0, 64));
+                    r = nanmax(r, __shfl_xor(r, QX1, 64));
+                    r = nanmax(r, __shfl_xor(r, 16, 64));
+                    r = nanmax(r, __shfl_xor(r, 32, 64));
                     if (pass_e) {
 #pragma unroll
                         for (int ks = 0; ks < KSH; ++ks) hb[ks] = prev[ks];

@@ -13,7 +13,12 @@ Evaluation: held-out seeds, distance of the learned start after 1 and 2
 Picard iterations to the converged predictor, next to the plain predictor
 after 1, 2 and 3 iterations.
 
+With --case wave3d: the 3D p=2 network, from runs of the smooth density wave
+(initial condition 5) with seeded amplitude and pressure on the periodic
+[0,1]^3 with two levels (32^3, the centre half refined).
+
     python train_warm_start.py --out weights/kh_p3_2d
+    python train_warm_start.py --case wave3d --out weights/wave_p2_3d
 """
 import argparse
 import json
@@ -27,9 +32,33 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 import pyadg  # noqa: E402
-from layouts import kh_bands  # noqa: E402
+from layouts import kh_bands, two_level_centre  # noqa: E402
 
 C_DT, T_END = 0.5, 10.0
+
+
+def wave3d_params(seed):
+    """Smooth density wave (IC 5) with seeded amplitude and pressure."""
+    rng = np.random.default_rng(9500 + seed)
+    return [rng.uniform(0.05, 0.4), rng.uniform(0.5, 2.0)]
+
+
+def wave3d_project(ctx, levels, seed):
+    for l in range(len(levels)):
+        ctx.project_ic(l, 5, wave3d_params(seed))
+    ctx.average_down()
+    ctx.fill_ghosts()
+    ctx.set_dt(C_DT, T_END, 0.0)
+
+
+def wave3d_context(seed, precision=pyadg.ADG_FP32_MFMA, n0=32):
+    """3D p=2 on the periodic [0,1]^3, two levels."""
+    ctx = pyadg.Context(3, 2, 1.4, precision)
+    ctx.set_geometry([n0] * 3, [0] * 3, [1] * 3)
+    levels = two_level_centre(n0, 32, D=3)
+    ctx.set_levels(levels)
+    wave3d_project(ctx, levels, seed)
+    return ctx, levels
 
 
 def kh_context(seed, P=3, n0=64, nlev=3, precision=pyadg.ADG_FP32_MFMA, ic=4):
@@ -49,15 +78,19 @@ def valid_slots(levels, lev):
     """Slot indices of the valid cells of a level (grown boxes, x fastest)."""
     out, off = [], 0
     for lo, hi in levels[lev]:
-        g0, g1 = hi[0] - lo[0] + 3, hi[1] - lo[1] + 3
-        for j in range(1, g1 - 1):
-            out.extend(off + j * g0 + i for i in range(1, g0 - 1))
-        off += g0 * g1
+        g = [hi[d] - lo[d] + 3 for d in range(len(lo))]
+        inner = np.ix_(*[np.arange(1, n - 1) for n in reversed(g)])
+        idx = np.arange(int(np.prod(g))).reshape(tuple(reversed(g)))[inner]
+        out.extend((off + idx).ravel().tolist())
+        off += int(np.prod(g))
     return np.array(out)
 
 
-def evaluate(par, seed, ic, P=3):
-    ctx, levels = kh_context(seed, precision=pyadg.ADG_FP64, ic=ic)
+def evaluate(par, seed, ic, P=3, case="kh"):
+    if case == "kh":
+        ctx, levels = kh_context(seed, precision=pyadg.ADG_FP64, ic=ic)
+    else:
+        ctx, levels = wave3d_context(seed, precision=pyadg.ADG_FP64)
     ctx.mlp_set_params(par)
     ctx.keep_Hw(True)
     res = {}
@@ -82,7 +115,8 @@ def evaluate(par, seed, ic, P=3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=os.path.join(HERE, "weights", "kh_p3_2d"))
+    ap.add_argument("--case", choices=["kh", "wave3d"], default="kh")
+    ap.add_argument("--out", default=None)
     ap.add_argument("--pairs", type=int, default=2_000_000)
     ap.add_argument("--every", type=int, default=20, help="steps between collections")
     ap.add_argument("--per-seed", type=int, default=9, help="collections per seed")
@@ -92,14 +126,22 @@ def main():
     ap.add_argument("--heldout", default="4:1001,4:1002,1:20240611",
                     help="comma list of ic:seed evaluation runs (not used for training)")
     args = ap.parse_args()
+    kh = args.case == "kh"
+    P = 3 if kh else 2
+    if args.out is None:
+        args.out = os.path.join(HERE, "weights", "kh_p3_2d" if kh else "wave_p2_3d")
+    if not kh and args.heldout == ap.get_default("heldout"):
+        args.heldout = "5:1001,5:1002"
 
     t0 = time.time()
-    ctx, levels = kh_context(1)
+    ctx, levels = kh_context(1) if kh else wave3d_context(1)
     ctx.mlp_init(2024)
     seed, n = 1, 0
     seeds = []
     while n < args.pairs:
-        if seed > 1:   # same hierarchy, next seed (the dataset stays on the device)
+        if not kh and seed > 1:
+            wave3d_project(ctx, levels, seed)
+        elif seed > 1:   # same hierarchy, next seed (the dataset stays on the device)
             for l in range(len(levels)):
                 # odd seeds: multi-mode perturbation; even seeds: single mode
                 # plus seeded cell noise (initial condition 1, other seeds
@@ -112,7 +154,7 @@ def main():
             ctx.fill_ghosts()
             ctx.set_dt(C_DT, T_END, 0.0)
         for _ in range(args.per_seed):
-            n = ctx.mlp_collect(2 * 3 + 2)
+            n = ctx.mlp_collect(2 * P + 2)
             ctx.step_graph(C_DT, T_END, args.every)
             if n >= args.pairs:
                 break
@@ -131,15 +173,17 @@ def main():
     evals = {}
     for hs in args.heldout.split(","):
         ic, sd = hs.split(":")
-        evals[hs] = evaluate(par, int(sd), int(ic))
+        evals[hs] = evaluate(par, int(sd), int(ic), P=P, case=args.case)
         print(f"held-out seed {hs}: " + ", ".join(f"{k} {v:.2e}" for k, v in evals[hs].items()), flush=True)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     par.astype(np.float32).tofile(args.out + ".f32")
     sz = ctx.mlp_sizes()
     meta = {"widths": sz, "pairs": n, "train_seeds": seeds, "collect_every_steps": args.every,
-            "collect_iterations": 8, "batch": args.batch, "loss_history": hist, "heldout_distance": evals,
-            "workload": "KH, odd seeds IC 4 (multi-mode, amax 0.02), even seeds IC 1 (single mode + noise), "
-                        "p=3, 3 levels 64^2/128^2/256^2 bands, FP32 form",
+            "collect_iterations": 2 * P + 2, "batch": args.batch, "loss_history": hist, "heldout_distance": evals,
+            "workload": ("KH, odd seeds IC 4 (multi-mode, amax 0.02), even seeds IC 1 (single mode + noise), "
+                         "p=3, 3 levels 64^2/128^2/256^2 bands, FP32 form") if kh else
+                        ("3D p=2, periodic [0,1]^3, 32^3 + centre half refined, density wave (IC 5) with "
+                         "seeded amplitude and pressure, FP32 form"),
             "seconds": time.time() - t0}
     with open(args.out + ".json", "w") as f:
         json.dump(meta, f, indent=1)

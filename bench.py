@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--weights", default=os.path.join(PKG, "weights", "kh_p3_2d.f32"))
     ap.add_argument("--acc-steps", type=int, default=200, help="steps of the accuracy comparison (0 = skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch measurement")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D density-wave measurement")
     ap.add_argument("--kernel", type=int, default=0, help="predictor kernel: 0 auto, 1 VALU, 2 MFMA")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
@@ -231,8 +232,56 @@ def main():
                 "elements": nel, "ms_per_step": ms, "value": nel * ctx.Q * ctx.Np_st / (ms * 1e-3),
                 "autotune_ms": tune, "autotune_choice": {"kind": choice[0], "waves": choice[1]}}
 
+    def measure_wave3d():
+        """3D p=2 density wave (IC 5, amplitude 0.2, pressure 1) on the
+        periodic [0,1]^3, 64^3 and the centre half refined (524 288 valid
+        elements): FP64, FP32-MFMA and learned + FP32 (quad-gather MFMA
+        predictor, network weights/wave_p2_3d trained on other amplitudes
+        and pressures)."""
+        from layouts import two_level_centre
+        lev3 = two_level_centre(64, 32, D=3)
+        w3 = np.fromfile(os.path.join(PKG, "weights", "wave_p2_3d.f32"), dtype=np.float32)
+        out = {"workload": "3D density wave p=2, periodic [0,1]^3, 64^3 + centre half refined"}
+        nel = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lv in lev3 for lo, hi in lv)
+        out["elements"] = nel
+        for form in ("fp64", "fp32", "learned"):
+            prec = pyadg.ADG_FP64 if form == "fp64" else pyadg.ADG_FP32_MFMA
+            ctx = pyadg.Context(3, 2, 1.4, prec)
+            ctx.set_geometry([64] * 3, [0] * 3, [1] * 3)
+            ctx.set_levels(lev3)
+            for l in range(len(lev3)):
+                ctx.project_ic(l, 5, [0.2, 1.0])
+            ctx.average_down()
+            ctx.fill_ghosts()
+            ctx.set_dt(c_dt, Tend, 0.0)
+            if form == "learned":
+                ctx.mlp_set_params(w3)
+                ctx.set_learned(True, tol=args.tol, max_iters=args.max_iters)
+            ctx.autotune(3)
+            ctx.step_graph(c_dt, Tend, 3)
+            ctx.sync()
+            if form == "learned":
+                ctx.learned_stats(reset=True)
+            nb = 10
+            e0, e1 = Event(), Event()
+            e0.record(ctx.stream())
+            ctx.step_graph(c_dt, Tend, nb)
+            e1.record(ctx.stream())
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / nb
+            out[f"{form}_ms_per_step"] = ms
+            out[f"{form}_value"] = nel * ctx.Q * ctx.Np_st / (ms * 1e-3)
+            if form == "learned":
+                ne, nf = ctx.learned_stats()
+                out["learned_fallback_fraction"] = nf / max(ne, 1)
+            ctx.close()
+        out["fp32_over_fp64"] = out["fp64_ms_per_step"] / out["fp32_ms_per_step"]
+        out["learned_over_fp64"] = out["fp64_ms_per_step"] / out["learned_ms_per_step"]
+        return out
+
     res = {f: measure(f) for f in ("fp64", "fp32", "learned")}
     batch = measure_batch() if not args.no_batch else None
+    wave3d = measure_wave3d() if not args.no_3d else None
     head = res[args.form]
     accuracy = accuracy_vs_fp64(make, levels, args.acc_steps, c_dt, Tend) if args.acc_steps > 0 else None
     Q, Ns, Nst, Mst, Mbd = head["sizes"]
@@ -272,6 +321,7 @@ def main():
                    "autotune": {f: res[f]["autotune"] for f in res},
                    "accuracy": accuracy,
                    "batch": batch,
+                   "wave3d": wave3d,
                    "step": "hipGraph replay of predictor, face flux, corrector, reflux+average down, "
                            "ghost fill+CFL+advance"},
         "roofline": roofline, "cpu_baseline": cpu,

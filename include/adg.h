@@ -125,9 +125,10 @@ int adg_set_dt_value(adg_ctx* ctx, double Dt);
  * mode 1: learned warm start (no counterpart in the reference): the network
  *   of adg_mlp_* gives the start H_0 = A0 U_w + Delta, Picard iterations run
  *   until max|H_k - H_{k-1}| <= tol rho0 (adg_set_learned, rho0 the element's
- *   mean density, so tol is relative to the state) and
- *   a wavefront of 4 elements that does not pass falls back to mode 0.
- *   Node-lane predictor (2D) only.
+ *   mean density, so tol is relative to the state); an element that does
+ *   not pass is recomputed by mode 0 (the fallback), one that passes keeps
+ *   the iterate it passed with.  MFMA predictors (node-lane in 2D,
+ *   quad-gather in 2D and 3D).
  * n_fallback (nullable) receives the number of elements that fell back
  * (0 in mode 0). */
 int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);

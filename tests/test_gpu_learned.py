@@ -238,3 +238,28 @@ def test_learned_predictor_accuracy_and_gain(adg, orc):
     # the error grows monotonically as the tolerance is relaxed
     errs = [rows[(t, 2)][2] for t in (1e-6, 1e-5, 1e-4, 1e-3)]
     assert all(x <= 1.5 * y for x, y in zip(errs, errs[1:]))
+
+
+def test_nan_start_falls_back(adg):
+    """A network whose output is NaN (NaN output biases): every element
+    fails the acceptance test (a NaN residual must not pass through a max
+    that drops NaN) and the result is the plain predictor's, bit for bit."""
+    import os
+    print("case learned start with NaN network output, KH 32^2 FP32 node-lane", flush=True)
+    w = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ader-dg-amr_amd", "weights",
+                     "kh_p3_2d.f32")
+    g = _kh_ctx(adg, 5)
+    g.mlp_init(1)
+    par = np.fromfile(w, dtype=np.float32).copy()
+    sz = g.mlp_sizes()
+    par[sz["npar"] - sz["nout_p"]:sz["npar"]] = np.nan   # b3
+    g.mlp_set_params(par)
+    g.keep_Hw(True)
+    g.set_learned(True, tol=1e-4, max_iters=2)
+    nf = g.predictor(-1, 1)
+    learned = g.download_Hw(0)
+    g.predictor(-1, 0)
+    plain = g.download_Hw(0)
+    print(f"  {nf} of {g.nslot(0)} elements fell back", flush=True)
+    assert nf == g.nslot(0)
+    assert np.array_equal(learned, plain)
