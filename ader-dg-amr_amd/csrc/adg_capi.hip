@@ -741,22 +741,26 @@ static const void* warm_kernel(int nt) {
     return (const void*)adg::k_mlp_warm1<P, D, NINP, NOUTP>;
 }
 
+// bytes of the warm-start constant block staged after the fragments
+static int mlp_warm_const_pad(const adg_ctx* c) {
+    return adg::warm_const_pad(2 * c->mlp.nin_p + 3 * c->mlp.nout_p + 2 * c->mlp.nh);
+}
+
 template <int P, int D>
 static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
     constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
     constexpr int NINP = 16 * ((Q * Ns + 1 + D + 15) / 16), NOUTP = 16 * ((Q * Nst + 15) / 16);
     auto& m = c->mlp;
     if (m.nin_p != NINP || m.nout_p != NOUTP) return fail(c, ADG_E_STATE, "MLP widths do not match the tables");
+    if (mlp_warm_const_pad(c) != adg::warm_const_pad(adg::warm_const_floats<NINP, NOUTP>()))
+        return fail(c, ADG_E_STATE, "warm-start constant block does not match the kernel");
     adg::WarmArgs a{};
-    a.frag = m.frag;
+    a.stage = reinterpret_cast<const uint4*>(m.frag);
     a.f1 = m.f1;
     a.f2 = m.f2;
     a.f3 = m.f3;
     a.nfrag = m.nfrag;
-    a.b1 = m.par + m.ob1;
-    a.b2 = m.par + m.ob2;
-    a.b3 = m.par + m.ob3;
-    a.norm = m.norm;
+    a.stage16 = (int)(((size_t)m.nfrag * 64 * sizeof(uint2) + mlp_warm_const_pad(c)) / 16);
     a.Uw = c->Uw;
     a.slot_lev = c->slot_lev;
     a.ts = c->ts;
@@ -961,7 +965,7 @@ static int mlp_alloc(adg_ctx* c) {
     for (float** p : {&m.par, &m.grad, &m.m, &m.v})
         HIPCHK(c, hipMalloc((void**)p, (size_t)m.npar * sizeof(float)));
     HIPCHK(c, hipMalloc((void**)&m.norm, (size_t)mlp_norm_size(c) * sizeof(float)));
-    HIPCHK(c, hipMalloc((void**)&m.frag, (size_t)m.nfrag * 64 * sizeof(uint2)));
+    HIPCHK(c, hipMalloc((void**)&m.frag, (size_t)m.nfrag * 64 * sizeof(uint2) + mlp_warm_const_pad(c)));
     HIPCHK(c, hipMalloc((void**)&m.dacc, (size_t)(1 + 2 * (m.nin_p + m.nout_p)) * sizeof(double)));
     return ADG_OK;
 }
@@ -977,12 +981,17 @@ static int mlp_pack(adg_ctx* c) {
     auto& m = c->mlp;
     if (m.frag_ok) return ADG_OK;
     const int nh = m.nh;
+    // layers 2 and 3 at half scale: the inference kernel feeds them twice the GELU
     hipLaunchKernelGGL(adg::k_mlp_pack, dim3((nh / 16) * (m.nin_p / 16)), dim3(64), 0, c->stream, m.par + m.oW1, nh,
-                       m.nin_p, m.frag + (long)m.f1 * 64);
+                       m.nin_p, 1.0f, m.frag + (long)m.f1 * 64);
     hipLaunchKernelGGL(adg::k_mlp_pack, dim3((nh / 16) * (nh / 16)), dim3(64), 0, c->stream, m.par + m.oW2, nh, nh,
-                       m.frag + (long)m.f2 * 64);
+                       0.5f, m.frag + (long)m.f2 * 64);
     hipLaunchKernelGGL(adg::k_mlp_pack, dim3((m.nout_p / 16) * (nh / 16)), dim3(64), 0, c->stream, m.par + m.oW3,
-                       m.nout_p, nh, m.frag + (long)m.f3 * 64);
+                       m.nout_p, nh, 0.5f, m.frag + (long)m.f3 * 64);
+    const int npad = mlp_warm_const_pad(c) / 4;
+    hipLaunchKernelGGL(adg::k_mlp_warm_consts, dim3((npad + 255) / 256), dim3(256), 0, c->stream, m.norm, m.par + m.ob1,
+                       m.par + m.ob2, m.par + m.ob3, m.nin_p, m.nout_p, m.nout, nh, npad,
+                       reinterpret_cast<float*>(m.frag + (long)m.nfrag * 64));
     HIPCHK(c, hipGetLastError());
     int rc = ADG_OK;
 #define L_CW(PP, DD) rc = configure_mlp_warm<PP, DD>(c)

@@ -49,13 +49,8 @@ typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bf16x4 pack_bf16_hw(float a, float b, float c, float d) {
     const bf16x2_t lo = __builtin_convertvector(f32x2_t{a, b}, bf16x2_t);
     const bf16x2_t hi = __builtin_convertvector(f32x2_t{c, d}, bf16x2_t);
-    const unsigned ul = __builtin_bit_cast(unsigned, lo), uh = __builtin_bit_cast(unsigned, hi);
-    bf16x4 r;
-    r[0] = (short)(ul & 0xffffu);
-    r[1] = (short)(ul >> 16);
-    r[2] = (short)(uh & 0xffffu);
-    r[3] = (short)(uh >> 16);
-    return r;
+    typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(bf16x4, u32x2_t{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
 }
 // C += A B, 16x16x16: A[i][4k'+t] on lane i+16k' (t = 0..3 packed),
 // B[4k'+t][j] on lane j+16k', C row 4g+r of lane group g in register r.
@@ -79,17 +74,19 @@ __device__ __forceinline__ float erf_as(float x) {
     return copysignf(y, x);
 }
 __device__ __forceinline__ float gelu_f(float a) { return 0.5f * a * (1.0f + erf_as(a * 0.70710678118654752f)); }
-// The same GELU written as (a + |a| erf(|a|/sqrt 2)) / 2 for the inference
-// kernel: no sign transfer, one FMA for the final combination.
+// Twice the same GELU, a + |a| erf(|a|/sqrt 2), for the inference kernel: no
+// sign transfer, one FMA for the final combination; the factor 1/2 is folded
+// into the packed weights of the layer that consumes it (an exact power-of-two
+// scaling of BF16 values).
 __device__ __forceinline__ float gelu_w(float a) {
-    const float z = fabsf(a) * 0.70710678118654752f;
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(a), 1.0f));
     float y = fmaf(t, 1.061405429f, -1.453152027f);
     y = fmaf(t, y, 1.421413741f);
     y = fmaf(t, y, -0.284496736f);
     y = fmaf(t, y, 0.254829592f);
-    const float e = fmaf(-t * y, __expf(-z * z), 1.0f);   // erf(z), z >= 0
-    return 0.5f * fmaf(fabsf(a), e, a);
+    const float r = a * 0.84932180028801904f;   // exp(-a^2/2) = 2^(-r^2), r = a sqrt(log2(e)/2)
+    const float e = fmaf(-t * y, __builtin_amdgcn_exp2f(-r * r), 1.0f);   // erf(|a|/sqrt 2)
+    return fmaf(fabsf(a), e, a);
 }
 __device__ __forceinline__ float dgelu_f(float a) {
     return 0.5f * (1.0f + erf_as(a * 0.70710678118654752f)) + a * 0.39894228040143268f * __expf(-0.5f * a * a);
@@ -326,12 +323,9 @@ __global__ void __launch_bounds__(64) k_mlp_collect(CollectArgs a) {
 // next layer, so activations never leave registers.  Writes the warm-start
 // increment Delta = rho0 tau_max (mu_y + sd_y y) per output, [o][slot].
 struct WarmArgs {
-    const uint2* __restrict__ frag;    // [nfrag][64] packed BF16 fragments
+    const uint4* __restrict__ stage;   // packed BF16 fragments [nfrag][64] (8 B each), then the constants
     int f1, f2, f3, nfrag;             // first fragment of each layer
-    const float* __restrict__ b1;
-    const float* __restrict__ b2;
-    const float* __restrict__ b3;
-    const float* __restrict__ norm;    // mu_x, sd_x [nin_p], mu_y, sd_y [nout_p]
+    int stage16;                       // 16-byte units of the staged block
     const double* __restrict__ Uw;
     const uint8_t* __restrict__ slot_lev;
     const TimeState* __restrict__ ts;
@@ -341,49 +335,71 @@ struct WarmArgs {
     float* __restrict__ warm;          // [nout][NS]
 };
 
-// Floats of the warm-start constants held in LDS after the fragments: mu_x,
-// sd_x [nin_p], mu_y, sd_y [nout_p], b1, b2 [hidden], b3 [nout_p]; then the
-// cell sizes of every level (kMaxLev x 3 doubles).
+// Warm-start constants, staged after the fragments: mu_x, 1/sd_x [nin_p],
+// mu_y, sd_y [nout_p], b1, b2 [hidden], b3 [nout_p] (zero past nout), padded
+// to 16 bytes; in LDS they are followed by the cell sizes of every level
+// (kMaxLev x 3 doubles).
 template <int NINP, int NOUTP>
 __host__ __device__ constexpr int warm_const_floats() {
     return 2 * NINP + 3 * NOUTP + 2 * kMlpHidden;
 }
+__host__ __device__ constexpr int warm_const_pad(int nfloats) { return (nfloats * 4 + 15) / 16 * 16; }
 template <int NINP, int NOUTP>
 __host__ __device__ constexpr int warm_const_bytes() {
-    return (warm_const_floats<NINP, NOUTP>() * 4 + kMaxLev * 3 * 8 + 15) / 16 * 16;
+    return warm_const_pad(warm_const_floats<NINP, NOUTP>()) + (kMaxLev * 3 * 8 + 15) / 16 * 16;
+}
+
+// Builds the constant block (layout above) from the normalisation vector
+// [mu_x, sd_x, mu_y, sd_y] and the biases; one thread per float.
+__global__ void __launch_bounds__(256) k_mlp_warm_consts(const float* __restrict__ norm, const float* __restrict__ b1,
+                                                         const float* __restrict__ b2, const float* __restrict__ b3,
+                                                         int nin_p, int nout_p, int nout, int nh, int npad,
+                                                         float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nnorm = 2 * nin_p + 2 * nout_p, nconst = nnorm + 2 * nh + nout_p;
+    if (i >= npad) return;
+    float v = 0.f;
+    if (i < nnorm) v = (i >= nin_p && i < 2 * nin_p) ? 1.f / norm[i] : norm[i];
+    else if (i < nnorm + nh) v = b1[i - nnorm];
+    else if (i < nnorm + 2 * nh) v = b2[i - nnorm - nh];
+    else if (i < nconst) v = (i - nnorm - 2 * nh < nout) ? b3[i - nnorm - 2 * nh] : 0.f;
+    out[i] = v;
+}
+
+// Copies n16 16-byte units from global memory to LDS with the global-to-LDS
+// loads of gfx950 (global_load_lds_dwordx4): lane l of a wavefront writes
+// bytes [16 l, 16 l + 16) past the wavefront's (uniform) LDS address, so no
+// registers hold the data in flight.  Visible after s_waitcnt vmcnt(0) and a
+// workgroup barrier.
+__device__ __forceinline__ void stage_to_lds_dma(unsigned char* dst, const uint4* src, int n16) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int b = wave * 64; b < n16; b += nwave * 64)
+        if (b + lane < n16)
+            __builtin_amdgcn_global_load_lds(src + b + lane, (__attribute__((address_space(3))) void*)(dst + (long)b * 16),
+                                             16, 0, 0);
 }
 
 // NT column tiles of 16 columns per wavefront share every fragment read.  A
 // tile holds ELT = 16 elements, or ELT = 8 with columns 8-15 repeating 0-7
 // (twice the wavefronts, each with half the activation work: the kernel is
-// latency-bound at the sizes of a step, not MFMA-bound).
+// latency-bound at the sizes of a step, not MFMA-bound).  Layers 1 and 2 keep
+// twice the GELU (their consumers' weights are packed at half scale), and
+// every accumulator starts at its bias.
 template <int P, int D, int NINP, int NOUTP, int NT, int ELT>
 __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     static_assert(ELT == 16 || ELT == 8, "8 or 16 elements per tile");
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns, Nst = Dm::Nst;
-    constexpr int NIN = Q * Ns + 1 + D, NOUT = Q * Nst, NH = kMlpHidden;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns;
+    constexpr int NIN = Q * Ns + 1 + D, NOUT = Q * Dm::Nst, NH = kMlpHidden;
     constexpr int KS1 = NINP / 16, KS2 = NH / 16, UT3 = NOUTP / 16;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    uint2* fr = reinterpret_cast<uint2*>(smem_raw);
-    stage_to_lds(fr, a.frag, a.nfrag * 64 * (int)sizeof(uint2));
-    // normalisation, biases and cell sizes in LDS as well: read from global
-    // memory inside the layer loops they cost a memory round trip per block
-    float* cst = reinterpret_cast<float*>(fr + a.nfrag * 64);
-    double* ldx = reinterpret_cast<double*>(cst + warm_const_floats<NINP, NOUTP>());
-    constexpr int NNORM = 2 * NINP + 2 * NOUTP;
-    for (int i = threadIdx.x; i < warm_const_floats<NINP, NOUTP>(); i += blockDim.x) {
-        float v;
-        if (i < NNORM) v = (i >= NINP && i < 2 * NINP) ? 1.f / a.norm[i] : a.norm[i];   // 1 / sd_x
-        else if (i < NNORM + NH) v = a.b1[i - NNORM];
-        else if (i < NNORM + 2 * NH) v = a.b2[i - NNORM - NH];
-        else v = (i - NNORM - 2 * NH < NOUT) ? a.b3[i - NNORM - 2 * NH] : 0.f;
-        cst[i] = v;
-    }
-    for (int i = threadIdx.x; i < kMaxLev * 3; i += blockDim.x) ldx[i] = a.geo.dx[i / 3][i % 3];
-    __syncthreads();
+    const bf16x4* fr = reinterpret_cast<const bf16x4*>(smem_raw);
+    const float* cst = reinterpret_cast<const float*>(smem_raw + (long)a.nfrag * 64 * 8);
+    double* ldx = reinterpret_cast<double*>(smem_raw + (long)a.nfrag * 64 * 8 +
+                                            warm_const_pad(warm_const_floats<NINP, NOUTP>()));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int g = lane >> 4, c = lane & 15;
+    constexpr int NNORM = 2 * NINP + 2 * NOUTP;
     const float* mux = cst;
     const float* rsdx = cst + NINP;
     const float* muy = cst + 2 * NINP;
@@ -394,42 +410,43 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     const long per_wave = (long)ELT * NT;
     const int ce = c & (ELT - 1);   // element of column c
     const Deal deal = make_deal(a.nelem, per_wave, nwave);
-    auto bf = [&](int f) {
-        const uint2 v = fr[f * 64 + lane];
-        bf16x4 r;
-        r[0] = (short)(v.x & 0xffff);
-        r[1] = (short)(v.x >> 16);
-        r[2] = (short)(v.y & 0xffff);
-        r[3] = (short)(v.y >> 16);
-        return r;
-    };
-    long eb;
-    for (long it = 0; deal_next(deal, it, wave, per_wave, nwave, &eb); ++it) {
-        long slot[NT];
-        bool active[NT];
-        float sc[NT];
-        bf16x4 xb[NT][KS1];
-        // every global load of the features first, in one memory round trip
-        int levs[NT];
-        double avgs[NT][Q], uf[NT][KS1][4];
+
+    long slot[NT];
+    bool active[NT];
+    int levs[NT];
+    double avgs[NT][Q], uf[NT][KS1][4];
+    auto load = [&](long eb) {   // every global load of a tile's features, one memory round trip
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
             const long e = eb + (long)ELT * tt + ce;
             active[tt] = e < a.nelem && c < ELT;   // columns past ELT repeat, not stored
             slot[tt] = a.slot_begin + (e < a.nelem ? e : a.nelem - 1);
             levs[tt] = a.slot_lev[slot[tt]];
+            const double* us = a.Uw + slot[tt];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) avgs[tt][q] = a.Uw[(long)q * Ns * a.NS + slot[tt]];
+            for (int q = 0; q < Q; ++q) avgs[tt][q] = us[(long)q * Ns * a.NS];
+            // row 16 s + 4 g + t: the lane part 4 g apart, the rest uniform
+            const double* ug = us + (long)(4 * g) * a.NS;
 #pragma unroll
             for (int s = 0; s < KS1; ++s)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int f = 16 * s + 4 * g + t;
-                    uf[tt][s][t] = f < Q * Ns ? a.Uw[(long)f * a.NS + slot[tt]] : 0.0;
-                }
+                for (int t = 0; t < 4; ++t)
+                    uf[tt][s][t] = 16 * s + 4 * g + t < Q * Ns ? ug[(long)(16 * s + t) * a.NS] : 0.0;
         }
-        const double Dt = a.ts->Dt;
-        __builtin_amdgcn_sched_barrier(0);
+    };
+    // fragments and constants by DMA, then the first tile's features: one
+    // round trip covers both
+    stage_to_lds_dma(smem_raw, a.stage, a.stage16);
+    long eb;
+    bool have = deal_next(deal, 0, wave, per_wave, nwave, &eb);
+    if (have) load(eb);
+    for (int i = threadIdx.x; i < kMaxLev * 3; i += blockDim.x) ldx[i] = a.geo.dx[i / 3][i % 3];
+    __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): the DMA has landed
+    __syncthreads();
+    const double Dt = a.ts->Dt;
+    for (long it = 0; have; ++it) {
+        float sc[NT];
+        bf16x4 xb[NT][KS1];
 #pragma unroll
         for (int tt = 0; tt < NT; ++tt) {
             const int lev = levs[tt];
@@ -468,67 +485,66 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
 #pragma unroll
         for (int u = 0; u < KS2; ++u) {
             f32x4 acc[NT];
+            const f32x4 bias = *reinterpret_cast<const f32x4*>(b1 + 16 * u + 4 * g);
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = bias;
 #pragma unroll
             for (int s = 0; s < KS1; ++s) {
-                const bf16x4 w = bf(a.f1 + u * KS1 + s);
+                const bf16x4 w = fr[(a.f1 + u * KS1 + s) * 64 + lane];
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) acc[tt] = mma_bf16(w, xb[tt][s], acc[tt]);
             }
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) {
-                float v[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = gelu_w(acc[tt][r] + b1[16 * u + 4 * g + r]);
-                h1[tt][u] = pack_bf16_hw(v[0], v[1], v[2], v[3]);
-            }
+            for (int tt = 0; tt < NT; ++tt)
+                h1[tt][u] = pack_bf16_hw(gelu_w(acc[tt][0]), gelu_w(acc[tt][1]), gelu_w(acc[tt][2]), gelu_w(acc[tt][3]));
             __builtin_amdgcn_sched_barrier(0);   // keeps fragment reads next to their MFMAs
         }
         bf16x4 h2[NT][KS2];
 #pragma unroll
         for (int u = 0; u < KS2; ++u) {
             f32x4 acc[NT];
+            const f32x4 bias = *reinterpret_cast<const f32x4*>(b2 + 16 * u + 4 * g);
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = bias;
 #pragma unroll
             for (int s = 0; s < KS2; ++s) {
-                const bf16x4 w = bf(a.f2 + u * KS2 + s);
+                const bf16x4 w = fr[(a.f2 + u * KS2 + s) * 64 + lane];
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) acc[tt] = mma_bf16(w, h1[tt][s], acc[tt]);
             }
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) {
-                float v[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = gelu_w(acc[tt][r] + b2[16 * u + 4 * g + r]);
-                h2[tt][u] = pack_bf16_hw(v[0], v[1], v[2], v[3]);
-            }
+            for (int tt = 0; tt < NT; ++tt)
+                h2[tt][u] = pack_bf16_hw(gelu_w(acc[tt][0]), gelu_w(acc[tt][1]), gelu_w(acc[tt][2]), gelu_w(acc[tt][3]));
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int u = 0; u < UT3; ++u) {
             f32x4 acc[NT];
+            const f32x4 bias = *reinterpret_cast<const f32x4*>(b3 + 16 * u + 4 * g);
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = bias;
 #pragma unroll
             for (int s = 0; s < KS2; ++s) {
-                const bf16x4 w = bf(a.f3 + u * KS2 + s);
+                const bf16x4 w = fr[(a.f3 + u * KS2 + s) * 64 + lane];
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) acc[tt] = mma_bf16(w, h2[tt][s], acc[tt]);
             }
             __builtin_amdgcn_sched_barrier(0);
+            const f32x4 my = *reinterpret_cast<const f32x4*>(muy + 16 * u + 4 * g);
+            const f32x4 sy = *reinterpret_cast<const f32x4*>(sdy + 16 * u + 4 * g);
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt) {
                 if (!active[tt]) continue;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int o = 16 * u + 4 * g + r;
-                    if (o < NOUT)
-                        a.warm[(long)o * a.NS + slot[tt]] = sc[tt] * (muy[o] + sdy[o] * (acc[tt][r] + b3[o]));
+                    if (o < NOUT) a.warm[(long)o * a.NS + slot[tt]] = sc[tt] * fmaf(sy[r], acc[tt][r], my[r]);
                 }
             }
         }
+        __builtin_amdgcn_sched_barrier(0);   // the next tile's loads stay behind this tile's stores
+        have = deal_next(deal, it + 1, wave, per_wave, nwave, &eb);
+        if (have) load(eb);
     }
 }
 
@@ -547,17 +563,19 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     mlp_warm_body<P, D, NINP, NOUTP, 1, 8>(a);
 }
 
-// Packs layer weights W [rows][cols] (FP32, row-major) into A-operand
-// fragments: fragment (u, s), lane i + 16k holds W[16u + i][16s + 4k + t].
-__global__ void __launch_bounds__(64) k_mlp_pack(const float* __restrict__ W, int rows, int cols, uint2* __restrict__ out) {
+// Packs layer weights W [rows][cols] (FP32, row-major), times scale (a power
+// of two, so exact), into A-operand fragments: fragment (u, s), lane i + 16k
+// holds W[16u + i][16s + 4k + t].
+__global__ void __launch_bounds__(64) k_mlp_pack(const float* __restrict__ W, int rows, int cols, float scale,
+                                                  uint2* __restrict__ out) {
     const int f = blockIdx.x, lane = threadIdx.x;
     const int ks = cols / 16;
     const int u = f / ks, s = f - u * ks;
     const int i = lane & 15, k = lane >> 4;
     const float* w = W + (long)(16 * u + i) * cols + 16 * s + 4 * k;
     uint2 v;
-    v.x = (unsigned)bf16_bits(w[0]) | ((unsigned)bf16_bits(w[1]) << 16);
-    v.y = (unsigned)bf16_bits(w[2]) | ((unsigned)bf16_bits(w[3]) << 16);
+    v.x = (unsigned)bf16_bits(scale * w[0]) | ((unsigned)bf16_bits(scale * w[1]) << 16);
+    v.y = (unsigned)bf16_bits(scale * w[2]) | ((unsigned)bf16_bits(scale * w[3]) << 16);
     out[(long)f * 64 + lane] = v;
     (void)rows;
 }
