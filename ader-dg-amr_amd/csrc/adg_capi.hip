@@ -233,6 +233,12 @@ int pack_tables(adg_ctx* c) {
     c->to.Vbd = add(t.Vbd);
     c->to.Mkbdm = add(t.Mkbdm);
     c->to.Mkbdp = add(t.Mkbdp);
+    {
+        std::vector<float> f32(t.Mkbdp.begin(), t.Mkbdp.end());
+        std::vector<double> packed((f32.size() + 1) / 2, 0.0);
+        std::memcpy(packed.data(), f32.data(), f32.size() * sizeof(float));
+        c->to.Mkbdpf = add(packed);
+    }
     c->to.Mkc = add(mkc);
     c->to.Minv = add(minv);
     c->to.Pcf = add(t.P_cf);
@@ -1410,7 +1416,11 @@ static int launch_corrector(adg_ctx* c, long begin, long n) {
     a.geo = c->geo;
     a.NS = c->H.NS;
     a.nvalid = n;
-#define L_CO(PP, DD) hipLaunchKernelGGL((adg::k_corrector<T, PP, DD>), dim3(nblocks(n * c->Q)), dim3(256), 0, c->stream, a)
+    if (c->H.NS >= (1L << 29)) return fail(c, ADG_E_UNSUP, "the corrector addresses at most 2^29 slots");
+    for (int d = 0; d < c->D; ++d)
+        if (c->H.NF[d] >= (1L << 29)) return fail(c, ADG_E_UNSUP, "the corrector addresses at most 2^29 faces");
+#define L_CO(PP, DD) \
+    hipLaunchKernelGGL((adg::k_corrector<T, PP, DD>), dim3(nblocks(adg::pad_wave(n) * c->Q)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_CO);
 #undef L_CO
     HIPCHK(c, hipGetLastError());

@@ -75,6 +75,7 @@ __host__ __device__ __forceinline__ long vol_idx(int k, int q, long slot, long N
 // Offsets (in doubles) of every table inside the device table buffer.
 struct TabOff {
     int A0, B, V, Skc, Vbd, Mkbdm, Mkbdp, Mkc, Minv, Pcf, Pfc, Pfl, Pfh, phi0, sgn;
+    int Mkbdpf;       // Mkbdp rounded to FP32 (two per double entry), the face moments of the FP32 form
     int Vs, ws, xs;   // spatial basis at the spatial nodes [qMp_s][Np_s], node weights, node coordinates [qMp_s][D]
     int total;
 };
@@ -986,11 +987,18 @@ __global__ void __launch_bounds__(64 * WPB) k_predictor_nl(PredMArgs<T> a) {
 // cell (its - face) adds (-1)^{k_d} S since Mkbdm[d](k,m) = (-1)^{k_d}
 // Mkbdp[d](k,m) exactly, and the mode-0 rows telescope.
 template <typename T, int Mbd>
-__device__ __forceinline__ T face_moment(const double* __restrict__ Mrow, const T* F) {
+__device__ __forceinline__ T face_moment(const T* __restrict__ Mrow, const T* F) {
     T s = T(0);
 #pragma unroll
-    for (int mb = 0; mb < Mbd; ++mb) s = fma(T(Mrow[mb]), F[mb], s);
+    for (int mb = 0; mb < Mbd; ++mb) s = fma(Mrow[mb], F[mb], s);
     return s;
+}
+// Mkbdp in the arithmetic type of the face moments (the FP32 copy is the
+// FP64 table rounded to nearest once, on the host)
+template <typename T>
+__device__ __forceinline__ const T* moment_rows(const double* __restrict__ tab, const TabOff& to) {
+    if constexpr (sizeof(T) == 8) return tab + to.Mkbdp;
+    else return reinterpret_cast<const float*>(tab + to.Mkbdpf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1069,6 +1077,22 @@ struct CorrArgs {
     long nvalid;
 };
 
+// Element of a row whose base is wavefront-uniform, at a 32-bit per-lane
+// byte offset: the global access takes the scalar-base + vector-offset form,
+// with no 64-bit address arithmetic per access.
+template <typename T>
+__device__ __forceinline__ T& row_at(T* base, unsigned byte_off) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ const T& row_at(const T* base, unsigned byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__host__ __device__ __forceinline__ long pad_wave(long n) { return (n + 63) & ~63L; }
+
+// One thread per (q, valid cell), the cells padded to whole wavefronts so
+// that q is wavefront-uniform: every array offset is a uniform row plus the
+// 32-bit byte offset of the slot or face (slots and faces below 2^29).
 template <typename T, int P, int D>
 __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     using Dm = Dims<P, D>;
@@ -1076,35 +1100,42 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     // directions whose face values are held in registers at once (all, but
     // one at a time for the 3D FP64 form: 2 x 27 doubles per direction)
     constexpr int DPRE = (D == 3 && sizeof(T) == 8) ? 1 : D;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nvalid * Q) return;
-    const int q = (int)(t / a.nvalid);
-    const long i = t - (long)q * a.nvalid;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nvp = pad_wave(a.nvalid);
+    if (t >= nvp * Q) return;
+    const int q = __builtin_amdgcn_readfirstlane((int)(t / nvp));
+    const long i = t - (long)q * nvp;
+    if (i >= a.nvalid) return;
     const long NS = a.NS;
     const double* __restrict__ Mkc = a.tab + a.to.Mkc;
     const double* __restrict__ sgn = a.tab + a.to.sgn;
     // every load of the thread is issued before its arithmetic (two memory
     // round trips: the lists, then the modes and face values)
-    long flo[D], fhi[D];
+    unsigned flo[D], fhi[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) { flo[d] = a.cell_face[d][0][i]; fhi[d] = a.cell_face[d][1][i]; }
+    for (int d = 0; d < D; ++d) {
+        flo[d] = (unsigned)a.cell_face[d][0][i] * (unsigned)sizeof(T);
+        fhi[d] = (unsigned)a.cell_face[d][1][i] * (unsigned)sizeof(T);
+    }
     const long slot = a.valid_list[i];
     const int lev = a.slot_lev[slot];
+    const unsigned sb = (unsigned)slot * 8u, sbt = (unsigned)slot * (unsigned)sizeof(T);
+    double* __restrict__ Uq = a.Uw + (long)q * Ns * NS;
+    const T* __restrict__ Vq = a.Vol + (long)q * Ns * NS;
     double u[Ns], mk[Ns];
     T vol[Ns];
 #pragma unroll
     for (int k = 0; k < Ns; ++k) {
-        u[k] = a.Uw[((long)q * Ns + k) * NS + slot];
-        vol[k] = a.Vol[vol_idx<P, D>(k, q, slot, NS)];
+        u[k] = row_at(Uq + k * NS, sb);
+        vol[k] = row_at(Vq + k * NS, sbt);
         mk[k] = Mkc[k];   // held: the stores to Uw below may alias the table for the compiler
     }
     T Fh[DPRE][Mbd], Fl[DPRE][Mbd];
     auto load_dir = [&](int d, T* fh, T* fl) {
         const long nf = a.NF[d];
-        const T* __restrict__ ph = a.Fnum[d] + (long)q * Mbd * nf + fhi[d];
-        const T* __restrict__ pl = a.Fnum[d] + (long)q * Mbd * nf + flo[d];
+        const T* __restrict__ Fq = a.Fnum[d] + (long)q * Mbd * nf;
 #pragma unroll
-        for (int mb = 0; mb < Mbd; ++mb) { fh[mb] = ph[mb * nf]; fl[mb] = pl[mb * nf]; }
+        for (int mb = 0; mb < Mbd; ++mb) { fh[mb] = row_at(Fq + mb * nf, fhi[d]); fl[mb] = row_at(Fq + mb * nf, flo[d]); }
     };
 #pragma unroll
     for (int d = 0; d < DPRE; ++d) load_dir(d, Fh[d], Fl[d]);
@@ -1118,7 +1149,7 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
         const int s = DPRE == D ? d : 0;
         if (DPRE < D && d > 0) load_dir(d, Fh[0], Fl[0]);
         const double cdd = Dt / a.geo.dx[lev][d];
-        const double* __restrict__ Mbp = a.tab + a.to.Mkbdp + (long)d * Ns * Mbd;
+        const T* __restrict__ Mbp = moment_rows<T>(a.tab, a.to) + (long)d * Ns * Mbd;
 #pragma unroll
         for (int k = 0; k < Ns; ++k) {
             const T sh = face_moment<T, Mbd>(Mbp + k * Mbd, Fh[s]);
@@ -1128,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
         }
     }
 #pragma unroll
-    for (int k = 0; k < Ns; ++k) a.Uw[((long)q * Ns + k) * NS + slot] = r[k] / mk[k];
+    for (int k = 0; k < Ns; ++k) row_at(Uq + k * NS, sb) = r[k] / mk[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1146,7 +1177,7 @@ __device__ __forceinline__ double register_moment(const double* __restrict__ tab
     const T* __restrict__ pf = Fnum_d + (long)q * Mbd * nf + f;
 #pragma unroll
     for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb * nf];
-    const double* __restrict__ Mbp = tab + to.Mkbdp + (long)d * Ns * Mbd;
+    const T* __restrict__ Mbp = moment_rows<T>(tab, to) + (long)d * Ns * Mbd;
     if (kind == 0 || n == 0) {
         const double sv = double(face_moment<T, Mbd>(Mbp + n * Mbd, F));
         return jac * ((kind == 1 || (sel & 1)) ? sv : tab[to.sgn + d * Ns + n] * sv);
