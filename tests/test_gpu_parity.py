@@ -241,6 +241,40 @@ def test_step_graph_replays(adg, orc, precision):
             assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL
 
 
+@pytest.mark.parametrize("nlev,periodic", [(3, [1, 1]), (4, [1, 1]), (3, [1, 0]), (2, [1, 1])])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_step_equals_unfused(adg, nlev, periodic, precision):
+    """The step's fused launches (reflux beside the finest average down, the
+    ghost fill beside the CFL reduction and the time advance) give bitwise
+    the U_w, t and Dt of the separate calls in the reference's order
+    (AmrDG.h:1000-1029): predictor, face flux, corrector, reflux, average
+    down from the finest level, ghost fill, CFL and advance."""
+    print(f"case fused vs unfused step KH P=3 16^2 {nlev} levels periodic={periodic} precision={precision}",
+          flush=True)
+    levels = kh_bands(16, nlev, 16)
+    ctxs = []
+    for _ in range(2):
+        ctx = adg.Context(2, 3, 1.4, precision)
+        ctx.set_geometry([16, 16], [0, 0], [1, 1], periodic=periodic)
+        ctx.set_levels(levels)
+        for l in range(nlev):
+            ctx.project_ic(l, 1, [0.01, 1e-3, 7])
+        ctx.average_down()
+        ctx.fill_ghosts()
+        ctx.set_dt(0.5, 1.0, 0.0)
+        ctxs.append(ctx)
+    fused, split = ctxs
+    for step in range(3):
+        fused.step(0.5, 1.0)
+        split.ader()
+        split.average_down()
+        split.fill_ghosts()
+        split.advance(0.5, 1.0)
+        assert fused.get_time() == split.get_time(), f"step {step}"
+        for l in range(nlev):
+            assert np.array_equal(fused.download_Uw(l), split.download_Uw(l)), f"step {step} level {l}"
+
+
 def test_kh_outflow_y_three_levels(adg, orc):
     """Periodic in x, transmissive (mirror-image) ghosts at y = 0 and 1."""
     print("case KH P=3 16^2 3 levels, outflow in y", flush=True)
