@@ -58,14 +58,18 @@ __host__ __device__ constexpr int acc_row(int g, int r) {
     return sizeof(T) == 8 ? 4 * r + g : 4 * g + r;
 }
 
-// Structure-of-arrays layouts, slot (or face) index fastest:
-//   Tr  [ds][mb][q][slot]   face traces, ds = 2*d + side (0 = lo, 1 = hi)
+// Layouts, slot (or face) index fastest except for the traces:
+//   Tr  [ds][mb][slot][q]   face traces, ds = 2*d + side (0 = lo, 1 = hi);
+//                           the Q variables of a node innermost: a predictor
+//                           tile's store covers its elements' variables as
+//                           one contiguous run, and the face flux reads a
+//                           node's state as one run
 //   Vol [q][k][slot]        corrector volume term, like U_w
 //   Fnum[d][q][mb][face]    numerical flux at the face quadrature nodes
 template <int P, int D>
 __host__ __device__ __forceinline__ long tr_idx(int ds, int mb, int q, long slot, long NS) {
     using Dm = Dims<P, D>;
-    return ((long)(ds * Dm::Mbd + mb) * Dm::Q + q) * NS + slot;
+    return ((long)(ds * Dm::Mbd + mb) * NS + slot) * Dm::Q + q;
 }
 template <int P, int D>
 __host__ __device__ __forceinline__ long vol_idx(int k, int q, long slot, long NS) {
