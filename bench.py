@@ -33,7 +33,7 @@ KH_PARAMS = [0.01, 1e-3, 20240611]
 # HBM bytes per launch of the (autotuned) node-lane predictor on this workload,
 # measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE and scaled by
 # the counters' reading of a plain 256 MiB device copy (x2.0 reads, x1.0 writes)
-PRED_TRAFFIC = {"fp64": 152.1e6, "fp32": 56.4e6, "learned": 56.4e6}
+PRED_TRAFFIC = {"fp64": 147.2e6, "fp32": 53.1e6, "learned": 53.1e6}
 
 
 def algorithmic_flops(P, D, Q, Ns, Nst, Mst, Mbd):
