@@ -609,7 +609,7 @@ int adg_average_down(adg_ctx* c) {
         const long ng = (long)c->H.ad_coarse[l].size();
         if (ng == 0) continue;
         adg::AvgArgs a{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
-#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(adg::avg_groups_padded(ng) * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, a)
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(adg::avg_threads(ng, c->Q, c->tab.Np_s, c->D))), dim3(256), 0, c->stream, a)
         ADG_DISPATCH(c, L_AD);
 #undef L_AD
         HIPCHK(c, hipGetLastError());
@@ -1524,7 +1524,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
         a.ad_fine = c->ad_fine[L - 1];
         a.ngroup = (long)c->H.ad_coarse[L - 1].size();
         a.nblk_a = nblocks(a.nrc * c->Q * c->tab.Np_s);
-        const unsigned nb = a.nblk_a + nblocks(adg::avg_groups_padded(a.ngroup) * c->Q * c->tab.Np_s);
+        const unsigned nb = a.nblk_a + nblocks(adg::avg_threads(a.ngroup, c->Q, c->tab.Np_s, c->D));
         if (nb > 0) {
 #define L_RA(PP, DD) hipLaunchKernelGGL((adg::k_reflux_avg<T, PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)
             ADG_DISPATCH(c, L_RA);
@@ -1535,7 +1535,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
             const long ng = (long)c->H.ad_coarse[l].size();
             if (ng == 0) continue;
             adg::AvgArgs g{c->d_tab, c->to, c->Uw, c->ad_coarse[l], c->ad_fine[l], c->H.NS, ng};
-#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(adg::avg_groups_padded(ng) * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, g)
+#define L_AD(PP, DD) hipLaunchKernelGGL((adg::k_avgdown<PP, DD>), dim3(nblocks(adg::avg_threads(ng, c->Q, c->tab.Np_s, c->D))), dim3(256), 0, c->stream, g)
             ADG_DISPATCH(c, L_AD);
 #undef L_AD
             HIPCHK(c, hipGetLastError());

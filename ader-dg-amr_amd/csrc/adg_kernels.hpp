@@ -1441,44 +1441,71 @@ __device__ __forceinline__ void reflux_item(const EpiArgs<T>& a, long t) {
 }
 
 // Threads of the average down: groups padded to whole wavefronts, so that the
-// (q, n) of a wavefront is uniform.
+// variable of a wavefront is uniform.
 __host__ __device__ __forceinline__ long avg_groups_padded(long ngroup) { return (ngroup + 63) & ~63L; }
 
-// u_c[n] = 2^-D Minv[n] sum_c sum_k P_fc[c](n,k) u_f[c][k], one thread per
-// (group, q, n), group fastest and padded to whole wavefronts: (q, n) is
-// wavefront-uniform, so the P_fc row is read with scalar loads, and the
-// children's modes are loaded in one memory round trip.
+// Children of an average-down group held in registers at once: the largest
+// divisor of 2^D whose modes fit 48 doubles.
+template <int NCH, int Ns>
+__host__ __device__ constexpr int avg_child_block() {
+    return NCH * Ns <= 48 ? NCH : (NCH / 2) * Ns <= 48 ? NCH / 2 : (NCH / 4) * Ns <= 48 ? NCH / 4 : 1;
+}
+
+// Threads of an average down over ngroup groups.  2D: one per (group, q, n),
+// a short latency-bound item (the launch is small); 3D: one per (group, q)
+// computing every mode, so that each child mode is read once per thread
+// (8 children x Np_s modes would otherwise be re-read by Np_s threads).
+__host__ __device__ __forceinline__ long avg_threads(long ngroup, int Q, int Ns, int D) {
+    return avg_groups_padded(ngroup) * Q * (D == 3 ? 1 : Ns);
+}
+
+// u_c[n] = 2^-D Minv[n] sum_c sum_k P_fc[c](n,k) u_f[c][k], group fastest and
+// padded to whole wavefronts: (q, n) or q is wavefront-uniform and P_fc is
+// read with scalar loads; every mode n sums its children in order, one FMA
+// chain per child, in both mappings.
 template <int P, int D>
 __device__ __forceinline__ void avgdown_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
                                              const int* __restrict__ coarse, const int* __restrict__ fine,
                                              long NS, long ngroup, long t) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns, NCH = Dm::NCH;
+    constexpr int NPT = D == 3 ? Ns : 1;   // modes per thread
     const long gp = avg_groups_padded(ngroup);
-    if (t >= gp * Q * Ns) return;
+    if (t >= gp * Q * (Ns / NPT)) return;
     const int qn = __builtin_amdgcn_readfirstlane((int)(t / gp));
     const long g = t - (long)qn * gp;
     if (g >= ngroup) return;
-    const int q = qn / Ns, n = qn - q * Ns;
+    const int q = NPT == Ns ? qn : qn / Ns, n0 = NPT == Ns ? 0 : qn - q * Ns;
     const double* __restrict__ Pfc = tab + to.Pfc;
     long fs[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) fs[c] = fine[g * NCH + c];
     const long cs = coarse[g];
-    double u[NCH][Ns];
+    // children held at once: one memory round trip per block that fits 48 doubles
+    constexpr int CB = NPT == 1 ? NCH : avg_child_block<NCH, Ns>();
+    double s[NPT];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c)
+    for (int j = 0; j < NPT; ++j) s[j] = 0.0;
+#pragma unroll 1
+    for (int c0 = 0; c0 < NCH; c0 += CB) {
+        double u[CB][Ns];
 #pragma unroll
-        for (int k = 0; k < Ns; ++k) u[c][k] = Uw[((long)q * Ns + k) * NS + fs[c]];
-    double s = 0.0;
+        for (int c = 0; c < CB; ++c)
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        double v = 0.0;
+            for (int k = 0; k < Ns; ++k) u[c][k] = Uw[((long)q * Ns + k) * NS + fs[c0 + c]];
 #pragma unroll
-        for (int k = 0; k < Ns; ++k) v = fma(Pfc[(c * Ns + n) * Ns + k], u[c][k], v);
-        s += v;
+        for (int j = 0; j < NPT; ++j)
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < Ns; ++k) v = fma(Pfc[((c0 + c) * Ns + n0 + j) * Ns + k], u[c][k], v);
+                s[j] += v;
+            }
     }
-    Uw[((long)q * Ns + n) * NS + cs] = (1.0 / NCH) * (tab[to.Minv + n] * s);
+#pragma unroll
+    for (int j = 0; j < NPT; ++j)
+        Uw[((long)q * Ns + n0 + j) * NS + cs] = (1.0 / NCH) * (tab[to.Minv + n0 + j] * s[j]);
 }
 
 // ghost slot mode n: copy, or Minv[n] sum_k P_cf[child](n,k) u_c[k]
