@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle on the
 same seeded inputs.  Bar: FP64 form within 1e-12 normwise-relative per
-conserved variable (the north-star tolerance for U_w modes)."""
+conserved variable (the parity tolerance for U_w modes)."""
 import numpy as np
 import pytest
 
