@@ -620,8 +620,9 @@ int adg_average_down(adg_ctx* c) {
 static int launch_cfl(adg_ctx* c, double c_dt, double T, int advance) {
     const long nv = (long)c->H.valid_list.size();
     const double cfl = (1.0 / (2.0 * (double)c->P + 1.0)) * (1.0 / (double)c->D);
-    adg::CflArgs a{c->d_tab, c->to, c->Uw, c->valid_list, c->slot_lev, c->ts, c->geo, c->H.NS, nv, c->gamma, cfl};
-#define L_CFL(PP, DD) hipLaunchKernelGGL((adg::k_cfl<PP, DD>), dim3(nblocks(nv)), dim3(256), 0, c->stream, a)
+    const int k = adg::cfl_per_thread(nv);
+    adg::CflArgs a{c->d_tab, c->to, c->Uw, c->valid_list, c->slot_lev, c->ts, c->geo, c->H.NS, nv, c->gamma, cfl, k};
+#define L_CFL(PP, DD) hipLaunchKernelGGL((adg::k_cfl<PP, DD>), dim3(nblocks(nv, 256 * k)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_CFL);
 #undef L_CFL
     HIPCHK(c, hipGetLastError());
@@ -1503,7 +1504,8 @@ static adg::EpiArgs<T> epi_args(adg_ctx* c, double c_dt, double T_end) {
     a.valid_list = c->valid_list;
     a.slot_lev = c->slot_lev;
     a.nvalid = (long)c->H.valid_list.size();
-    a.ncfl_blocks = nblocks(a.nvalid);
+    a.cfl_k = adg::cfl_per_thread(a.nvalid);
+    a.ncfl_blocks = nblocks(a.nvalid, 256 * a.cfl_k);
     a.cfl = (1.0 / (2.0 * (double)c->P + 1.0)) * (1.0 / (double)c->D);
     a.c_dt = c_dt;
     a.Tend = T_end;
