@@ -596,7 +596,7 @@ int adg_fill_ghosts(adg_ctx* c) {
     const long ng = (long)c->H.g_slot.size();
     if (ng == 0) return ADG_OK;
     adg::GhostArgs a{c->d_tab, c->to, c->Uw, c->g_slot, c->g_src, c->g_kind, c->g_child, c->H.NS, ng};
-#define L_GH(PP, DD) hipLaunchKernelGGL((adg::k_ghosts<PP, DD>), dim3(nblocks(ng * c->Q * c->tab.Np_s)), dim3(256), 0, c->stream, a)
+#define L_GH(PP, DD) hipLaunchKernelGGL((adg::k_ghosts<PP, DD>), dim3(nblocks(adg::ghost_threads(ng, c->Q, c->tab.Np_s, c->D))), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_GH);
 #undef L_GH
     HIPCHK(c, hipGetLastError());
@@ -1543,7 +1543,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
             HIPCHK(c, hipGetLastError());
         }
     }
-    a.nblk_a = nblocks(a.nghost * c->Q * c->tab.Np_s);
+    a.nblk_a = nblocks(adg::ghost_threads(a.nghost, c->Q, c->tab.Np_s, c->D));
     const unsigned nb = a.nblk_a + a.ncfl_blocks;
 #define L_GC(PP, DD) hipLaunchKernelGGL((adg::k_ghost_cfl<T, PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_GC);

@@ -1526,7 +1526,15 @@ __device__ __forceinline__ void avgdown_item(const double* __restrict__ tab, con
         Uw[((long)q * Ns + n0 + j) * NS + cs] = (1.0 / NCH) * (tab[to.Minv + n0 + j] * s[j]);
 }
 
-// ghost slot mode n: copy, or Minv[n] sum_k P_cf[child](n,k) u_c[k]
+// Threads of a ghost fill over nghost ghosts: 2D one per (ghost, q, n), a
+// short latency-bound item; 3D one per (ghost, q) for every mode, so that an
+// interpolated ghost reads its parent's modes once, not once per mode.
+__host__ __device__ __forceinline__ long ghost_threads(long nghost, int Q, int Ns, int D) {
+    return nghost * Q * (D == 3 ? 1 : Ns);
+}
+
+// ghost slot mode n: copy, or Minv[n] sum_k P_cf[child](n,k) u_c[k], the same
+// arithmetic per mode in both mappings
 template <int P, int D>
 __device__ __forceinline__ void ghost_item(const double* __restrict__ tab, const TabOff& to, double* __restrict__ Uw,
                                            const int* __restrict__ g_slot, const int* __restrict__ g_src,
@@ -1534,27 +1542,39 @@ __device__ __forceinline__ void ghost_item(const double* __restrict__ tab, const
                                            long NS, long nghost, long t) {
     using Dm = Dims<P, D>;
     constexpr int Q = Dm::Q, Ns = Dm::Ns;
-    if (t >= nghost * Q * Ns) return;
+    constexpr int NPT = D == 3 ? Ns : 1;   // modes per thread
+    if (t >= nghost * Q * (Ns / NPT)) return;
     const int qn = (int)(t / nghost);
     const long g = t - (long)qn * nghost;
-    const int q = qn / Ns, n = qn - q * Ns;
+    const int q = NPT == Ns ? qn : qn / Ns, n0 = NPT == Ns ? 0 : qn - q * Ns;
     const long s = g_slot[g], src = g_src[g];
     const int kind = g_kind[g], mask = kind >> 2;
-    double v;
+    double v[NPT];
     if ((kind & 3) == 0) {
-        v = Uw[((long)q * Ns + n) * NS + src];
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) v[j] = Uw[((long)q * Ns + n0 + j) * NS + src];
     } else {
         const double* __restrict__ Pcf = tab + to.Pcf + (long)g_child[g] * Ns * Ns;
-        double a = 0.0;
+        double u[Ns];
 #pragma unroll
-        for (int k = 0; k < Ns; ++k) a = fma(Pcf[n * Ns + k], Uw[((long)q * Ns + k) * NS + src], a);
-        v = tab[to.Minv + n] * a;
+        for (int k = 0; k < Ns; ++k) u[k] = Uw[((long)q * Ns + k) * NS + src];
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < Ns; ++k) acc = fma(Pcf[(n0 + j) * Ns + k], u[k], acc);
+            v[j] = tab[to.Minv + n0 + j] * acc;
+        }
     }
     // transmissive boundary: mirror image, mode n times (-1)^{k_d} per mirrored d
 #pragma unroll
     for (int d = 0; d < D; ++d)
-        if (mask & (1 << d)) v *= tab[to.sgn + d * Ns + n];
-    Uw[((long)q * Ns + n) * NS + s] = v;
+        if (mask & (1 << d)) {
+#pragma unroll
+            for (int j = 0; j < NPT; ++j) v[j] *= tab[to.sgn + d * Ns + n0 + j];
+        }
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) Uw[((long)q * Ns + n0 + j) * NS + s] = v[j];
 }
 
 template <typename T, int P, int D>
