@@ -177,7 +177,10 @@ int adg_corrector(adg_ctx* ctx, int lev);
 /* Replaces the FluxRegister CrseAdd/FineAdd of AmrDG::ADER
  * (AmrDG.h:1138-1153) and AmrDG::AMR_flux_correction
  * (AmrDG_Level.tpp:416-446) -> L2ProjInterp::reflux
- * (AmrDG_Interpolater.tpp:7-72). crse_lev = -1: every coarse level. */
+ * (AmrDG_Interpolater.tpp:7-72). crse_lev = -1: every coarse level.
+ * Coarse cells covered by the finer level are not corrected: the average
+ * down that follows (AmrDG.h:1000) overwrites them, so U_w equals the
+ * reference's after adg_average_down, not between the two calls. */
 int adg_reflux(adg_ctx* ctx, int crse_lev);
 
 /* Replaces AmrDG::ADER (AmrDG.h:1071-1165): predictor, face flux,
