@@ -213,9 +213,9 @@ def test_mfma_layout(adg):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_step_graph_replays(adg, orc, precision):
-    """hipGraph replays of a whole step equal eager steps on every replay
-    (eager work and host syncs interleaved), and the FP64 form equals the
-    oracle."""
+    """hipGraph replays of a whole step equal eager steps on every replay,
+    U_w and (t, Dt) bitwise (eager work and host syncs interleaved), and the
+    FP64 form equals the oracle."""
     print(f"case graph replays KH P=3 16^2 3 levels precision={precision}", flush=True)
     levels = kh_bands(16, 3, 16)
     T, H, Uw, ctx = setup(adg, orc, 2, 3, levels, [16, 16], [0, 0], [1, 1], 1, [0.01, 1e-3, 7],
@@ -228,15 +228,16 @@ def test_step_graph_replays(adg, orc, precision):
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.set_dt(0.5, 1.0, 0.0)
     eager.set_dt(0.5, 1.0, 0.0)
-    for rep in range(4):
+    for rep in range(8):
         ctx.step_graph(0.5, 1.0, 1)
         eager.step(0.5, 1.0)
         ctx.sync()
         eager.totals()
+        assert ctx.get_time() == eager.get_time(), f"replay {rep}: (t, Dt)"
         for l in range(3):
             assert np.array_equal(ctx.download_Uw(l), eager.download_Uw(l)), f"replay {rep} level {l}"
     if precision == 0:
-        orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=4)
+        orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=8)
         for l in range(3):
             assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL
 
