@@ -1,6 +1,7 @@
 """GPU parity in 3D (Q = 5): the HIP path through the C-ABI against the CPU
 oracle on a smooth advected density wave (single level with several patches,
-two levels with reflux, p = 1 and 2) and on the spherical shock-tube state.
+two levels with reflux, p = 1, 2 and 3) and on the spherical shock-tube
+state.
 Bar: FP64 form within 1e-12 normwise-relative per conserved variable, each
 variable measured against the state scale where it is itself near zero;
 the FP32 form within its stated round-off and conserving mass and energy to
@@ -114,6 +115,25 @@ def test_3d_two_levels(adg, orc, P):
     m1 = ctx.totals()
     assert abs(m1[0] - m0[0]) <= 1e-13 * abs(m0[0])
     assert abs(m1[4] - m0[4]) <= 1e-13 * abs(m0[4])
+
+
+def test_3d_p3_two_levels(adg, orc):
+    """p = 3 in 3D (Np_s = 20, Np_st = 35, 256 space-time nodes): the MFMA
+    fragment sets exceed the LDS, so the FP64 form runs the VALU predictor;
+    two levels with reflux, 2 steps, against the oracle."""
+    print("case 3D two levels wave P=3 4^3 + centre", flush=True)
+    levels = two_level_centre(4, 4, D=3)
+    T, H, Uw, ctx = setup(adg, orc, 3, levels, 4)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    m0 = ctx.totals()
+    for _ in range(2):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=2)
+    for l in range(2):
+        assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL, f"level {l}"
+    m1 = ctx.totals()
+    assert abs(m1[0] - m0[0]) <= 1e-13 * abs(m0[0])
 
 
 def test_3d_fp32_form(adg, orc):
