@@ -441,7 +441,9 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     bool have = deal_next(deal, 0, wave, per_wave, nwave, &eb);
     if (have) load(eb);
     for (int i = threadIdx.x; i < kMaxLev * 3; i += blockDim.x) ldx[i] = a.geo.dx[i / 3][i % 3];
-    __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): the DMA has landed
+    // the global-to-LDS loads count in vmcnt: all of them have landed
+    // (0x0f70 = vmcnt(0) expcnt(7) lgkmcnt(15) in the gfx9 encoding)
+    __builtin_amdgcn_s_waitcnt(0x0f70);
     __syncthreads();
     const double Dt = a.ts->Dt;
     for (long it = 0; have; ++it) {
