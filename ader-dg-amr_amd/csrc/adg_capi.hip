@@ -12,6 +12,7 @@
 #include <cstring>
 #include <exception>
 #include <algorithm>
+#include <array>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -20,6 +21,7 @@
 #include "adg_hier.hpp"
 #include "adg_kernels.hpp"
 #include "adg_mlp.hpp"
+#include "adg_regrid.hpp"
 #include "adg_tables.hpp"
 
 namespace adg {
@@ -75,6 +77,22 @@ struct adg_ctx {
     int* uncovered = nullptr;
     double* unc_centre = nullptr;   // cell centres of the uncovered cells [n][3]
     adg::TimeState* ts = nullptr;
+    // TVB limiter of the finest level (Limiter_w): settings, the finest
+    // level's valid slots and resolved neighbour slots, cells-limited counter
+    bool lim_on = false;
+    double lim_M = 0.0;
+    double lim_tvb_c[adg::kMaxLev] = {};
+    int lim_t_limit = 1;
+    int* lim_cells = nullptr;
+    int* lim_nbr = nullptr;
+    long lim_n = 0;
+    unsigned long long* lim_count = nullptr;
+    // tagging-driven regrid: AMR settings, tagging criterion and thresholds
+    adg::AmrSettings amr;
+    bool amr_set = false;
+    int tag_crit = 2;
+    double tag_c[adg::kMaxLev] = {};
+    double tag_prm[4] = {5.0, 5.0, 0.0, 0.0};
     bool keep_hw = false;
     int iters_override = 0;    // > 0: Picard iterations of the next plain predictor launches (collection)
     bool independent = false;  // adg_set_levels builds a batch of independent periodic patches
@@ -250,16 +268,25 @@ int pack_tables(adg_ctx* c) {
     for (int d = 0; d < t.D; ++d)
         for (int k = 0; k < t.Np_s; ++k) sgn[(size_t)d * t.Np_s + k] = (t.mi_s[k][d] % 2) ? -1.0 : 1.0;
     c->to.sgn = add(sgn);
-    // spatial basis at the spatial nodes, node weights and coordinates (Lp norms)
-    {
-        const int Ms = t.qMp_s, Ns = t.Np_s, D = t.D;
+    // spatial basis at the points of the Lp norms, their weights and
+    // coordinates: the N^D solution points (L1, AmrDG.h:1580-1585) and the
+    // (2N)^D points of the L2 rule (AmrDG.h:1587-1655); point m has the
+    // node of dimension 0 slowest
+    for (int rule = 0; rule < 2; ++rule) {
+        const std::vector<double>& nd = rule ? t.nodes2 : t.nodes;
+        const std::vector<double>& wt = rule ? t.weights2 : t.weights;
+        const int n1 = (int)nd.size(), Ns = t.Np_s, D = t.D;
+        int Ms = 1;
+        for (int d = 0; d < D; ++d) Ms *= n1;
         std::vector<double> vs((size_t)Ms * Ns), ws((size_t)Ms), xs((size_t)Ms * D);
         for (int m = 0; m < Ms; ++m) {
             double w = 1.0;
+            int div = Ms;
             for (int d = 0; d < D; ++d) {
-                const int k = t.node_idx(m, d, D);
-                xs[(size_t)m * D + d] = t.nodes[k];
-                w *= t.weights[k];
+                div /= n1;
+                const int k = (m / div) % n1;
+                xs[(size_t)m * D + d] = nd[k];
+                w *= wt[k];
             }
             ws[m] = w;
             for (int n = 0; n < Ns; ++n) {
@@ -268,9 +295,10 @@ int pack_tables(adg_ctx* c) {
                 vs[(size_t)m * Ns + n] = ph;
             }
         }
-        c->to.Vs = add(vs);
-        c->to.ws = add(ws);
-        c->to.xs = add(xs);
+        c->to.lp[rule].Vs = add(vs);
+        c->to.lp[rule].ws = add(ws);
+        c->to.lp[rule].xs = add(xs);
+        c->to.lp[rule].M = Ms;
     }
     c->to.total = (int)buf.size();
     void* p = nullptr;
@@ -337,10 +365,12 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
     if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->ts = static_cast<adg::TimeState*>(tsp);
     void* lsp = nullptr;
-    if (hipMalloc(&lsp, 2 * sizeof(unsigned long long)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
+    if (hipMalloc(&lsp, 3 * sizeof(unsigned long long)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->lstats = static_cast<unsigned long long*>(lsp);
-    const unsigned long long z2[2] = {0ull, 0ull};
-    if (hipMemcpy(c->lstats, z2, sizeof(z2), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
+    c->lim_count = c->lstats + 2;
+    const unsigned long long z3[3] = {0ull, 0ull, 0ull};
+    if (hipMemcpy(c->lstats, z3, sizeof(z3), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
+    for (int l = 0; l < adg::kMaxLev; ++l) c->lim_tvb_c[l] = 1.0;
     adg::TimeState t0{0.0, 0.0, 0x7ff0000000000000ull, 0u, 0u};
     if (hipMemcpy(c->ts, &t0, sizeof(t0), hipMemcpyHostToDevice) != hipSuccess) { adg_destroy(c); return ADG_E_HIP; }
     *out = c;
@@ -422,6 +452,31 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     UP(c->re_mom, H.re_mom);
     UP(c->re_coef, H.re_coef);
     UP(c->uncovered, H.uncovered);
+    {
+        // the limiter's cells (valid cells of the finest level) and their
+        // neighbours per direction and side, a same-level ghost replaced by
+        // its source cell (FillBoundary's copy)
+        const adg::LevelLayout& fl = H.lev[nlev - 1];
+        const long n = fl.valid_end - fl.valid_begin;
+        std::vector<int> resolve((size_t)H.NS, -1);
+        for (size_t g = 0; g < H.g_slot.size(); ++g)
+            if (H.g_kind[g] == 0 && H.slot_lev[H.g_slot[g]] == nlev - 1) resolve[H.g_slot[g]] = H.g_src[g];
+        std::vector<int> cells((size_t)n), nbr((size_t)2 * D * n);
+        for (long i = 0; i < n; ++i) {
+            const long v = fl.valid_begin + i;
+            cells[i] = H.valid_list[v];
+            for (int d = 0; d < D; ++d)
+                for (int side = 0; side < 2; ++side) {
+                    const int f = H.cell_face[d][side][v];
+                    int sl = H.face_lr[d][2 * (size_t)f + (side == 0 ? 0 : 1)];
+                    if (resolve[sl] >= 0) sl = resolve[sl];
+                    nbr[(size_t)(2 * d + side) * n + i] = sl;
+                }
+        }
+        c->lim_n = n;
+        UP(c->lim_cells, cells);
+        UP(c->lim_nbr, nbr);
+    }
 #undef UP
     const size_t NS = (size_t)H.NS;
     if ((rc = dev_alloc(c, &c->Uw, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
@@ -561,6 +616,8 @@ int adg_set_independent_patches(adg_ctx* c, int on) {
 
 int adg_keep_Hw(adg_ctx* c, int on) {
     if (!c) return ADG_E_ARG;
+    // a captured step graph holds the H_w pointer of its predictor launch
+    if ((on != 0) != c->keep_hw) drop_graph(c);
     c->keep_hw = on != 0;
     if (c->keep_hw && c->levels_set && !c->Hw_dbg) {
         int rc = dev_alloc(c, &c->Hw_dbg, (size_t)c->Q * c->tab.Np_st * c->H.NS);
@@ -1553,11 +1610,44 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
 }
 }  // extern "C++"
 
+static int launch_limiter(adg_ctx* c, bool every_step) {
+    if (c->lim_n == 0) return ADG_OK;
+    const adg::LevelLayout& fl = c->H.lev[c->H.nlev - 1];
+    double h = fl.dx[0];
+    for (int d = 1; d < c->D; ++d) h = std::min(h, fl.dx[d]);
+    adg::LimArgs a{};
+    a.Uw = c->Uw;
+    a.cells = c->lim_cells;
+    a.nbr = c->lim_nbr;
+    a.count = c->lim_count;
+    a.ts = c->ts;
+    a.NS = c->H.NS;
+    a.n = c->lim_n;
+    a.thr = c->lim_tvb_c[c->H.nlev - 1] * c->lim_M * std::pow(h, 2.0);
+    a.gamma = c->gamma;
+    a.t_limit = every_step ? 1 : c->lim_t_limit;
+    for (int d = 0; d < c->D; ++d)
+        for (int n = 0; n < c->tab.Np_s; ++n) {
+            bool lin = c->tab.mi_s[n][d] == 1, others = true;
+            for (int e = 0; e < c->D; ++e)
+                if (e != d && c->tab.mi_s[n][e] != 0) others = false;
+            if (lin && others) { a.lin[d] = n; break; }
+        }
+#define L_LIM(PP, DD) hipLaunchKernelGGL((adg::k_limiter_tvb<PP, DD>), dim3(nblocks(c->lim_n)), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_LIM);
+#undef L_LIM
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
+}
+
 static int step_launches(adg_ctx* c, double c_dt, double T_end) {
     int rc;
     if ((rc = adg_predictor(c, -1, c->learned ? 1 : 0, nullptr)) != ADG_OK) return rc;
     if ((rc = adg_face_flux(c, -1)) != ADG_OK) return rc;
     if ((rc = adg_corrector(c, -1)) != ADG_OK) return rc;
+    // AmrDG::evolve: the limiter on the finest level after the ADER step,
+    // before the average down (AmrDG.h:990-1000)
+    if (c->lim_on && (rc = launch_limiter(c, false)) != ADG_OK) return rc;
     return c->precision == ADG_FP64 ? launch_epilogue<double>(c, c_dt, T_end) : launch_epilogue<float>(c, c_dt, T_end);
 }
 
@@ -1640,6 +1730,459 @@ int adg_conserved_totals(adg_ctx* c, double* totals) {
         totals[q] = s;
     }
     return ADG_OK;
+}
+
+int adg_set_limiter(adg_ctx* c, int type, double M, int ntvb, const double* tvb_c, int t_limit) {
+    if (!c) return ADG_E_ARG;
+    if (type != 0 && type != 1) return fail(c, ADG_E_ARG, "limiter type must be 0 (none) or 1 (TVB)");
+    if (type == 1) {
+        if (!(M >= 0.0)) return fail(c, ADG_E_ARG, "TVB M must be >= 0");
+        if (t_limit < 1) return fail(c, ADG_E_ARG, "t_limit must be >= 1");
+        if (ntvb < 0 || ntvb > adg::kMaxLev || (ntvb > 0 && !tvb_c)) return fail(c, ADG_E_ARG, "bad AMR_TVB_C");
+        for (int l = 0; l < adg::kMaxLev; ++l) c->lim_tvb_c[l] = l < ntvb ? tvb_c[l] : 1.0;
+        c->lim_M = M;
+        c->lim_t_limit = t_limit;
+    }
+    c->lim_on = type == 1;
+    drop_graph(c);
+    // the step counter n of the limiter's n % t_limit starts again at 0
+    const unsigned int z = 0u;
+    HIPCHK(c, hipMemcpyAsync(&c->ts->nstep, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
+int adg_limit(adg_ctx* c, unsigned long long* n_limited) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    unsigned long long before = 0;
+    if (n_limited) {
+        HIPCHK(c, hipMemcpyAsync(&before, c->lim_count, sizeof(before), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    int rc = launch_limiter(c, true);
+    if (rc != ADG_OK) return rc;
+    if (n_limited) {
+        unsigned long long after = 0;
+        HIPCHK(c, hipMemcpyAsync(&after, c->lim_count, sizeof(after), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *n_limited = after - before;
+    }
+    return ADG_OK;
+}
+
+int adg_limiter_stats(adg_ctx* c, unsigned long long* total, int reset) {
+    if (!c) return ADG_E_ARG;
+    if (total) {
+        HIPCHK(c, hipMemcpyAsync(total, c->lim_count, sizeof(*total), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (reset) {
+        const unsigned long long z = 0ull;
+        HIPCHK(c, hipMemcpyAsync(c->lim_count, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ADG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Level-index cells of the valid cells of level lev, in valid-list order
+// (patch by patch, x fastest).
+std::vector<std::array<int, 3>> level_valid_cells(const adg::Hierarchy& H, int lev) {
+    std::vector<std::array<int, 3>> out;
+    const int D = H.D;
+    for (int p = H.lev[lev].p_begin; p < H.lev[lev].p_end; ++p) {
+        const adg::PatchLayout& P = H.patch[p];
+        std::array<int, 3> iv{0, 0, 0};
+        for (iv[2] = (D > 2 ? P.lo[2] : 0); iv[2] <= (D > 2 ? P.hi[2] : 0); ++iv[2])
+            for (iv[1] = P.lo[1]; iv[1] <= P.hi[1]; ++iv[1])
+                for (iv[0] = P.lo[0]; iv[0] <= P.hi[0]; ++iv[0]) out.push_back(iv);
+    }
+    return out;
+}
+
+std::vector<std::vector<adg::IBox>> level_boxes(const adg::Hierarchy& H) {
+    std::vector<std::vector<adg::IBox>> out((size_t)H.nlev);
+    for (const adg::PatchLayout& P : H.patch) {
+        adg::IBox b;
+        for (int d = 0; d < 3; ++d) { b.lo[d] = P.lo[d]; b.hi[d] = P.hi[d]; }
+        out[(size_t)P.lev].push_back(b);
+    }
+    return out;
+}
+
+// Dense owner map of one level: level-index cell -> slot of its valid cell, or -1.
+struct OwnerMap {
+    int D = 2, dom[3] = {1, 1, 1};
+    int per[3] = {1, 1, 1};
+    std::vector<int> own;
+    void build(const adg::Hierarchy& H, int lev) {
+        D = H.D;
+        long n = 1;
+        for (int d = 0; d < 3; ++d) {
+            dom[d] = d < D ? H.lev[lev].dom[d] : 1;
+            per[d] = d < D ? H.per[d] : 1;
+            n *= dom[d];
+        }
+        own.assign((size_t)n, -1);
+        const std::vector<std::array<int, 3>> cells = level_valid_cells(H, lev);
+        const long vb = H.lev[lev].valid_begin;
+        for (size_t i = 0; i < cells.size(); ++i) own[(size_t)lin(cells[i].data())] = H.valid_list[(size_t)(vb + (long)i)];
+    }
+    long lin(const int* iv) const {
+        long k = 0;
+        for (int d = D - 1; d >= 0; --d) k = k * dom[d] + iv[d];
+        return k;
+    }
+    int find(const int* iv_in) const {
+        int iv[3] = {iv_in[0], iv_in[1], iv_in[2]};
+        for (int d = 0; d < D; ++d)
+            if (iv[d] < 0 || iv[d] >= dom[d]) {
+                if (!per[d]) return -1;
+                iv[d] = ((iv[d] % dom[d]) + dom[d]) % dom[d];
+            }
+        return own[(size_t)lin(iv)];
+    }
+};
+
+int floordiv2(int a) { return a >= 0 ? a / 2 : -((-a + 1) / 2); }
+
+}  // namespace
+
+extern "C" {
+
+int adg_host_make_grids(int dim, const int* dom0, const int* periodic, int max_level, int blocking_factor,
+                        int max_grid_size, double grid_eff, int n_error_buf, int n_proper, int nlev_old,
+                        const int* npatch_old, const int* boxes_old, const int* ntags, const int* tags,
+                        int* nlev_new, int* npatch_new, int* boxes_new, int cap_boxes) {
+    if ((dim != 2 && dim != 3) || !dom0 || !periodic || !npatch_old || !boxes_old || !ntags || !nlev_new ||
+        !npatch_new || nlev_old < 1 || max_level < 0 || max_level >= adg::kMaxLev || blocking_factor < 2 ||
+        max_grid_size < blocking_factor || !(grid_eff > 0.0 && grid_eff <= 1.0) || n_error_buf < 0 || n_proper < 0)
+        return ADG_E_ARG;
+    adg::AmrSettings st;
+    st.max_level = max_level;
+    st.blocking_factor = blocking_factor;
+    st.max_grid_size = max_grid_size;
+    st.grid_eff = grid_eff;
+    st.n_error_buf = n_error_buf;
+    st.n_proper = n_proper;
+    std::vector<std::vector<adg::IBox>> old((size_t)nlev_old);
+    const int* b = boxes_old;
+    for (int l = 0; l < nlev_old; ++l)
+        for (int k = 0; k < npatch_old[l]; ++k, b += 2 * dim) {
+            adg::IBox x;
+            for (int d = 0; d < dim; ++d) { x.lo[d] = b[d]; x.hi[d] = b[dim + d]; }
+            old[(size_t)l].push_back(x);
+        }
+    const int max_crse = std::min(nlev_old - 1, max_level - 1);
+    std::vector<adg::TagList> tl((size_t)std::max(max_crse + 1, 0));
+    const int* t = tags;
+    for (int l = 0; l <= max_crse; ++l)
+        for (int k = 0; k < ntags[l]; ++k, t += dim) {
+            std::array<int, 3> iv{0, 0, 0};
+            for (int d = 0; d < dim; ++d) iv[d] = t[d];
+            tl[(size_t)l].push_back(iv);
+        }
+    std::vector<std::vector<adg::IBox>> ng;
+    try {
+        ng = adg::make_new_grids(dim, dom0, periodic, st, old, tl);
+    } catch (const std::exception&) {
+        return ADG_E_ARG;
+    }
+    *nlev_new = 1 + (int)ng.size();
+    npatch_new[0] = npatch_old[0];
+    int nb = 0;
+    for (int k = 0; k < npatch_old[0]; ++k, ++nb) {
+        if (boxes_new && nb < cap_boxes)
+            for (int d = 0; d < 2 * dim; ++d) boxes_new[(size_t)nb * 2 * dim + d] = boxes_old[(size_t)k * 2 * dim + d];
+    }
+    for (size_t l = 0; l < ng.size(); ++l) {
+        npatch_new[l + 1] = (int)ng[l].size();
+        for (const adg::IBox& x : ng[l]) {
+            if (boxes_new && nb < cap_boxes)
+                for (int d = 0; d < dim; ++d) {
+                    boxes_new[(size_t)nb * 2 * dim + d] = x.lo[d];
+                    boxes_new[(size_t)nb * 2 * dim + dim + d] = x.hi[d];
+                }
+            ++nb;
+        }
+    }
+    return nb;
+}
+
+int adg_set_amr(adg_ctx* c, int max_level, int blocking_factor, int max_grid_size, double grid_eff,
+                int n_error_buf, int n_proper) {
+    if (!c) return ADG_E_ARG;
+    if (max_level < 0 || max_level >= adg::kMaxLev) return fail(c, ADG_E_ARG, "max_level out of range");
+    if (blocking_factor < 2 || blocking_factor % 2) return fail(c, ADG_E_ARG, "blocking factor must be even, >= 2");
+    if (max_grid_size < blocking_factor || max_grid_size % blocking_factor)
+        return fail(c, ADG_E_ARG, "max_grid_size must be a multiple of the blocking factor");
+    if (!(grid_eff > 0.0 && grid_eff <= 1.0) || n_error_buf < 0 || n_proper < 0)
+        return fail(c, ADG_E_ARG, "bad grid_eff, n_error_buf or n_proper");
+    c->amr.max_level = max_level;
+    c->amr.blocking_factor = blocking_factor;
+    c->amr.max_grid_size = max_grid_size;
+    c->amr.grid_eff = grid_eff;
+    c->amr.n_error_buf = n_error_buf;
+    c->amr.n_proper = n_proper;
+    c->amr_set = true;
+    return ADG_OK;
+}
+
+int adg_set_tagging(adg_ctx* c, int criterion, int n, const double* amr_c, int nprm, const double* params) {
+    if (!c) return ADG_E_ARG;
+    if (criterion < 0 || criterion > 2) return fail(c, ADG_E_ARG, "criterion must be 0, 1 or 2");
+    if (criterion == 0 && c->D != 2) return fail(c, ADG_E_UNSUP, "the vortex criterion is two-dimensional");
+    if (n < 1 || n > adg::kMaxLev || !amr_c) return fail(c, ADG_E_ARG, "amr_c must give 1..kMaxLev values");
+    if (nprm < 0 || nprm > 4 || (nprm > 0 && !params)) return fail(c, ADG_E_ARG, "at most 4 parameters");
+    c->tag_crit = criterion;
+    for (int l = 0; l < adg::kMaxLev; ++l) c->tag_c[l] = amr_c[l < n ? l : n - 1];
+    for (int i = 0; i < nprm; ++i) c->tag_prm[i] = params[i];
+    return ADG_OK;
+}
+
+int adg_tag_cells(adg_ctx* c, int lev, unsigned char* tags) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (lev < 0 || lev >= c->H.nlev || !tags) return fail(c, ADG_E_ARG, "bad level or buffer");
+    const adg::LevelLayout& L = c->H.lev[lev];
+    const long n = L.valid_end - L.valid_begin;
+    if (n == 0) return ADG_OK;
+    if (c->tag_crit == 0) {
+        // isentropic_vortex (Compressible_Euler.h:617-642): r <= amr_c^2
+        // about (x0, y0) with periodic distance, at the cell centre
+        const std::vector<std::array<int, 3>> cells = level_valid_cells(c->H, lev);
+        const double Lx = c->phi[0] - c->plo[0], Ly = c->phi[1] - c->plo[1];
+        for (long i = 0; i < n; ++i) {
+            const double x = c->plo[0] + (cells[i][0] + 0.5) * L.dx[0];
+            const double y = c->plo[1] + (cells[i][1] + 0.5) * L.dx[1];
+            double dx = x - c->tag_prm[0], dy = y - c->tag_prm[1];
+            if (dx > Lx / 2.0) dx -= Lx;
+            if (dx < -Lx / 2.0) dx += Lx;
+            if (dy > Ly / 2.0) dy -= Ly;
+            if (dy < -Ly / 2.0) dy += Ly;
+            const double r = std::sqrt(dx * dx + dy * dy);
+            tags[i] = r <= std::pow(c->tag_c[lev], 2) ? 1 : 0;
+        }
+        return ADG_OK;
+    }
+    uint8_t* d_out = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_out, (size_t)n));
+    adg::TagArgs a{};
+    a.Uw = c->Uw;
+    a.valid_list = c->valid_list;
+    for (int d = 0; d < c->D; ++d) {
+        a.cell_face[d][0] = c->cell_face[d][0];
+        a.cell_face[d][1] = c->cell_face[d][1];
+        a.face_lr[d] = c->face_lr[d];
+    }
+    a.out = d_out;
+    a.NS = c->H.NS;
+    a.vbegin = L.valid_begin;
+    a.n = n;
+    a.crit = c->tag_crit;
+    a.gamma = c->gamma;
+    if (c->tag_crit == 1) {
+        double h = L.dx[0];
+        for (int d = 1; d < c->D; ++d) h = std::min(h, L.dx[d]);
+        a.thr = c->lim_tvb_c[lev] * c->tag_c[lev] * std::pow(h, 2.0);
+    } else {
+        a.thr = c->tag_c[lev];
+    }
+    for (int d = 0; d < c->D; ++d)
+        for (int k = 0; k < c->tab.Np_s; ++k) {
+            bool lin = c->tab.mi_s[k][d] == 1, others = true;
+            for (int e = 0; e < c->D; ++e)
+                if (e != d && c->tab.mi_s[k][e] != 0) others = false;
+            if (lin && others) { a.lin[d] = k; break; }
+        }
+#define L_TAG(PP, DD) hipLaunchKernelGGL((adg::k_tag<PP, DD>), dim3(nblocks(n)), dim3(256), 0, c->stream, a)
+    ADG_DISPATCH(c, L_TAG);
+#undef L_TAG
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(tags, d_out, (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(c, ADG_E_HIP, std::string("adg_tag_cells: ") + hipGetErrorString(e));
+    return ADG_OK;
+}
+
+int adg_get_levels(adg_ctx* c, int* nlev, int* npatch, int* boxes, int cap) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (!nlev || !npatch) return fail(c, ADG_E_ARG, "null argument");
+    *nlev = c->H.nlev;
+    const int D = c->D;
+    for (int l = 0; l < c->H.nlev; ++l) npatch[l] = c->H.lev[l].p_end - c->H.lev[l].p_begin;
+    int k = 0;
+    for (const adg::PatchLayout& P : c->H.patch) {
+        if (boxes && k < cap)
+            for (int d = 0; d < D; ++d) {
+                boxes[(size_t)k * 2 * D + d] = P.lo[d];
+                boxes[(size_t)k * 2 * D + D + d] = P.hi[d];
+            }
+        ++k;
+    }
+    return k;
+}
+
+int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (nlev < 1 || nlev > adg::kMaxLev || !npatch || !boxes) return fail(c, ADG_E_ARG, "bad level arguments");
+    const int D = c->D, Q = c->Q, Ns = c->tab.Np_s;
+    // level 0 is not regridded
+    {
+        const adg::LevelLayout& L0 = c->H.lev[0];
+        if (npatch[0] != L0.p_end - L0.p_begin) return fail(c, ADG_E_ARG, "level 0 boxes differ");
+        for (int p = 0; p < npatch[0]; ++p)
+            for (int d = 0; d < D; ++d)
+                if (boxes[(size_t)p * 2 * D + d] != c->H.patch[p].lo[d] ||
+                    boxes[(size_t)p * 2 * D + D + d] != c->H.patch[p].hi[d])
+                    return fail(c, ADG_E_ARG, "level 0 boxes differ");
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    adg::Hierarchy old = c->H;
+    double* old_uw = c->Uw;
+    c->level_allocs.erase(std::remove(c->level_allocs.begin(), c->level_allocs.end(), (void*)old_uw),
+                          c->level_allocs.end());
+    int rc = adg_set_levels(c, nlev, npatch, boxes);
+    if (rc != ADG_OK) { (void)hipFree(old_uw); return rc; }
+    const adg::Hierarchy& H = c->H;
+    // level 0: the same slots
+    HIPCHK(c, hipMemcpy2DAsync(c->Uw, (size_t)H.NS * sizeof(double), old_uw, (size_t)old.NS * sizeof(double),
+                               (size_t)H.lev[0].nslot * sizeof(double), (size_t)Q * Ns, hipMemcpyDeviceToDevice,
+                               c->stream));
+    int* d_slot = nullptr;
+    int* d_src = nullptr;
+    uint8_t* d_kind = nullptr;
+    uint8_t* d_child = nullptr;
+    std::string err;
+    for (int l = 1; l < H.nlev && err.empty(); ++l) {
+        OwnerMap old_own, crse_own;
+        const bool had = l < old.nlev;
+        if (had) old_own.build(old, l);
+        crse_own.build(H, l - 1);
+        std::vector<int> es, esrc;
+        std::vector<uint8_t> ek, ech;
+        for (int p = H.lev[l].p_begin; p < H.lev[l].p_end && err.empty(); ++p) {
+            const adg::PatchLayout& P = H.patch[p];
+            int iv[3] = {0, 0, 0};
+            long s = P.off;
+            const int z0 = D > 2 ? P.lo[2] - 1 : 0, z1 = D > 2 ? P.hi[2] + 1 : 0;
+            for (iv[2] = z0; iv[2] <= z1; ++iv[2])
+                for (iv[1] = P.lo[1] - 1; iv[1] <= P.hi[1] + 1; ++iv[1])
+                    for (iv[0] = P.lo[0] - 1; iv[0] <= P.hi[0] + 1; ++iv[0], ++s) {
+                        int mv[3] = {iv[0], iv[1], iv[2]}, mask = 0;
+                        for (int d = 0; d < D; ++d) {
+                            const int n = H.lev[l].dom[d];
+                            if (H.per[d]) continue;
+                            if (iv[d] < 0) { mv[d] = -1 - iv[d]; mask |= 1 << d; }
+                            if (iv[d] >= n) { mv[d] = 2 * n - 1 - iv[d]; mask |= 1 << d; }
+                        }
+                        const int os = had ? old_own.find(mv) : -1;
+                        es.push_back((int)s);
+                        if (os >= 0) {
+                            esrc.push_back(os);
+                            ek.push_back((uint8_t)(mask << 2));
+                            ech.push_back(0);
+                            continue;
+                        }
+                        int civ[3] = {0, 0, 0}, child = 0;
+                        for (int d = 0; d < D; ++d) {
+                            civ[d] = floordiv2(mv[d]);
+                            child += (mv[d] - 2 * civ[d]) << (D - 1 - d);
+                        }
+                        const int cs = crse_own.find(civ);
+                        if (cs < 0) { err = "new level " + std::to_string(l) + " is not nested in level " + std::to_string(l - 1); break; }
+                        esrc.push_back(cs);
+                        ek.push_back((uint8_t)(1 | (mask << 2)));
+                        ech.push_back((uint8_t)child);
+                    }
+        }
+        if (!err.empty()) break;
+        const long ne = (long)es.size();
+        hipError_t e = hipMalloc((void**)&d_slot, ne * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&d_src, ne * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&d_kind, (size_t)ne);
+        if (e == hipSuccess) e = hipMalloc((void**)&d_child, (size_t)ne);
+        if (e == hipSuccess) e = hipMemcpy(d_slot, es.data(), ne * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_src, esrc.data(), ne * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_kind, ek.data(), (size_t)ne, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_child, ech.data(), (size_t)ne, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            adg::RemakeArgs a{c->d_tab, c->to, c->Uw, old_uw, d_slot, d_src, d_kind, d_child, H.NS, old.NS, ne};
+#define L_RM(PP, DD) hipLaunchKernelGGL((adg::k_remake<PP, DD>), dim3(nblocks(ne * Q)), dim3(256), 0, c->stream, a)
+            switch (c->P * 10 + D) {
+                case 12: L_RM(1, 2); break;
+                case 22: L_RM(2, 2); break;
+                case 32: L_RM(3, 2); break;
+                case 13: L_RM(1, 3); break;
+                case 23: L_RM(2, 3); break;
+                case 33: L_RM(3, 3); break;
+                default: break;
+            }
+#undef L_RM
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
+        if (e != hipSuccess) err = std::string("adg_remake: ") + hipGetErrorString(e);
+        (void)hipFree(d_slot); (void)hipFree(d_src); (void)hipFree(d_kind); (void)hipFree(d_child);
+        d_slot = d_src = nullptr;
+        d_kind = d_child = nullptr;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(old_uw);
+    if (!err.empty()) return fail(c, ADG_E_ARG, err);
+    return ADG_OK;
+}
+
+int adg_regrid(adg_ctx* c, double time, int* changed) {
+    (void)time;   // the criteria do not depend on time (Compressible_Euler.h:628-629)
+    if (changed) *changed = 0;
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (!c->amr_set) return fail(c, ADG_E_STATE, "adg_set_amr must come first");
+    const adg::Hierarchy& H = c->H;
+    const int D = c->D;
+    const int max_crse = std::min(H.nlev - 1, c->amr.max_level - 1);
+    std::vector<adg::TagList> tl((size_t)std::max(max_crse + 1, 0));
+    for (int l = 0; l <= max_crse; ++l) {
+        const long n = H.lev[l].valid_end - H.lev[l].valid_begin;
+        std::vector<unsigned char> tg((size_t)n);
+        int rc = adg_tag_cells(c, l, tg.data());
+        if (rc != ADG_OK) return rc;
+        const std::vector<std::array<int, 3>> cells = level_valid_cells(H, l);
+        for (long i = 0; i < n; ++i)
+            if (tg[(size_t)i]) tl[(size_t)l].push_back(cells[(size_t)i]);
+    }
+    std::vector<std::vector<adg::IBox>> old = level_boxes(H), ng;
+    try {
+        ng = adg::make_new_grids(D, c->dom0, c->per, c->amr, old, tl);
+    } catch (const std::exception& e) {
+        return fail(c, ADG_E_ARG, std::string("regrid: ") + e.what());
+    }
+    std::vector<int> np(1 + ng.size()), bx;
+    np[0] = (int)old[0].size();
+    for (const adg::IBox& b : old[0]) {
+        for (int d = 0; d < D; ++d) bx.push_back(b.lo[d]);
+        for (int d = 0; d < D; ++d) bx.push_back(b.hi[d]);
+    }
+    bool same = (int)ng.size() + 1 == H.nlev;
+    for (size_t l = 0; l < ng.size(); ++l) {
+        np[l + 1] = (int)ng[l].size();
+        if (same && ng[l].size() != old[l + 1].size()) same = false;
+        for (size_t k = 0; k < ng[l].size(); ++k) {
+            const adg::IBox& b = ng[l][k];
+            for (int d = 0; d < D; ++d) bx.push_back(b.lo[d]);
+            for (int d = 0; d < D; ++d) bx.push_back(b.hi[d]);
+            if (same)
+                for (int d = 0; d < D; ++d)
+                    if (b.lo[d] != old[l + 1][k].lo[d] || b.hi[d] != old[l + 1][k].hi[d]) same = false;
+        }
+    }
+    if (same) return ADG_OK;
+    int rc = adg_remake(c, (int)np.size(), np.data(), bx.data());
+    if (rc == ADG_OK && changed) *changed = 1;
+    return rc;
 }
 
 int adg_lp_error(adg_ctx* c, int q, int p, int ic_case, const double* params, double t, double* out) {

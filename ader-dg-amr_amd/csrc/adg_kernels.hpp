@@ -80,7 +80,9 @@ __host__ __device__ __forceinline__ long vol_idx(int k, int q, long slot, long N
 struct TabOff {
     int A0, B, V, Skc, Vbd, Mkbdm, Mkbdp, Mkc, Minv, Pcf, Pfc, Pfl, Pfh, phi0, sgn;
     int Mkbdpf;       // Mkbdp rounded to FP32 (two per double entry), the face moments of the FP32 form
-    int Vs, ws, xs;   // spatial basis at the spatial nodes [qMp_s][Np_s], node weights, node coordinates [qMp_s][D]
+    // Lp-norm rules: [0] the N^D solution points (L1), [1] the (2N)^D points
+    // of the L2 norm; basis values [M][Np_s], weights [M], coordinates [M][D]
+    struct LpRule { int Vs, ws, xs, M; } lp[2];
     int total;
 };
 // Offsets (in 64-entry fragments) of the MFMA A-operand fragment sets.
@@ -101,7 +103,7 @@ struct TimeState {
     double Dt;
     unsigned long long dtmin_bits;
     unsigned int done;   // CFL blocks finished in the current reduction
-    unsigned int pad;
+    unsigned int nstep;  // steps advanced since the counter was last reset (the limiter's n)
 };
 
 static constexpr double kNumLimit = 10.0 * 2.220446049250313e-16;  // 10 * DBL_EPSILON
@@ -1373,7 +1375,10 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
 // Finalize: optional t += Dt, then Dt = c_dt * min, clamped to T - t.
 __global__ void k_dt_finalize(TimeState* ts, double c_dt, double T, int advance) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (advance) ts->t = ts->t + ts->Dt;
+    if (advance) {
+        ts->t = ts->t + ts->Dt;
+        ts->nstep = ts->nstep + 1u;
+    }
     double m = __longlong_as_double((long long)ts->dtmin_bits);
     double Dt = c_dt * m;
     if (T - ts->t < Dt) Dt = T - ts->t;
@@ -1652,10 +1657,288 @@ __global__ void __launch_bounds__(256) k_ghost_cfl(EpiArgs<T> a) {
         if (a.Tend - tn < Dt) Dt = a.Tend - tn;
         a.ts->t = tn;
         a.ts->Dt = Dt;
+        a.ts->nstep = a.ts->nstep + 1u;
         a.ts->dtmin_bits = 0x7ff0000000000000ull;
         a.ts->done = 0u;
         __threadfence();
     }
+}
+
+// ---------------------------------------------------------------------------
+// TVB limiter of the finest level (Limiter_w + Limiter_linear_tvb,
+// AmrDG.h:1657-1855; minmodB / minmod, AmrDG_Limiter.tpp:41-84; the
+// characteristic matrices pde_EV_Lmatrix / pde_EV_Rmatrix,
+// Compressible_Euler.cpp:138-277, evaluated at U_w mode 0), in place, one
+// thread per valid cell.  A cell reads its mode 0 and linear modes and its
+// neighbours' mode 0: a neighbour that is a same-level ghost is read at its
+// source cell (what FillBoundary copies there), a coarse-fine or boundary
+// ghost keeps the value of the last ghost fill, as in the reference.  Mode 0
+// is never written, so the cells are independent.  The arithmetic is not
+// contracted into FMAs and follows the reference's expression order, so the
+// threshold, sign and minimum decisions see the same rounded values.
+struct LimArgs {
+    double* __restrict__ Uw;
+    const int* __restrict__ cells;   // finest-level valid slots
+    const int* __restrict__ nbr;     // [d][side][cell] neighbour slots (same-level ghosts resolved)
+    unsigned long long* __restrict__ count;   // cells limited (accumulated)
+    const TimeState* __restrict__ ts;
+    long NS, n;
+    double thr;      // AMR_TVB_C[lev] * M * h^2, h = min dx of the level
+    double gamma;
+    int t_limit;     // limit when nstep % t_limit == 0
+    int lin[3];      // lin_mode_idx (AmrDG.tpp:174-184)
+};
+
+template <int D>
+__device__ __forceinline__ void tvb_ev(int d, const double* u, double g_, double L[D + 2][D + 2],
+                                       double R[D + 2][D + 2]) {
+#pragma clang fp contract(off)
+    constexpr int Q = D + 2;
+    // Soundspeed / Pressure with the clamps (Compressible_Euler.h:562-604)
+    double prs = 0.0;
+    for (int e = 0; e < D; ++e) prs += (u[e + 1] * u[e + 1]) / u[0];
+    prs *= (-0.5);
+    prs += u[D + 1];
+    prs *= (g_ - 1.0);
+    prs = fmax(prs, kNumLimit);
+    const double c = sqrt(g_ * (prs / fmax(u[0], kNumLimit)));
+    const double u1 = u[1] / u[0], u2 = u[2] / u[0], u3 = D == 3 ? u[3] / u[0] : 0.0;
+    const double ek = D == 2 ? 0.5 * ((u1 * u1) + (u2 * u2)) : 0.5 * ((u1 * u1) + (u2 * u2) + (u3 * u3));
+    const double hh = ((c * c) / (g_ - 1.0)) + ek;
+    const double b = 1.0 / (2.0 * (c * c));
+    const double gm = g_ - 1.0;
+    const double ph = gm * ek;
+    for (int i = 0; i < Q; ++i)
+        for (int j = 0; j < Q; ++j) { L[i][j] = 0.0; R[i][j] = 0.0; }
+    const double ud = d == 0 ? u1 : (d == 1 ? u2 : u3);
+    // rows 0..2 of L and columns 0..2 of R: acoustic and entropy fields
+    L[0][0] = b * (ph + c * ud);
+    L[1][0] = 1.0 - 2.0 * b * ph;
+    L[2][0] = b * (ph - c * ud);
+    const double uv[3] = {u1, u2, u3};
+    for (int e = 0; e < D; ++e) {
+        L[0][1 + e] = e == d ? -b * (gm * uv[e] + c) : -b * gm * uv[e];
+        L[1][1 + e] = 2.0 * b * gm * uv[e];
+        // the reference writes -b*(g*u) for the first component past the
+        // direction (u2 for x, u1 for y and z) and -b*g*u for the other
+        L[2][1 + e] = e == d ? -b * (gm * uv[e] - c)
+                             : (e == (d == 0 ? 1 : 0) ? -b * (gm * uv[e]) : -b * gm * uv[e]);
+    }
+    L[0][D + 1] = b * gm;
+    L[1][D + 1] = -2.0 * b * gm;
+    L[2][D + 1] = b * gm;
+    R[0][0] = 1.0; R[0][1] = 1.0; R[0][2] = 1.0;
+    for (int e = 0; e < D; ++e) {
+        R[1 + e][0] = e == d ? uv[e] - c : uv[e];
+        R[1 + e][1] = uv[e];
+        R[1 + e][2] = e == d ? uv[e] + c : uv[e];
+    }
+    R[D + 1][0] = hh - c * ud;
+    R[D + 1][1] = ek;
+    R[D + 1][2] = hh + c * ud;
+    // shear fields (rows / columns 3 and 4), as the reference lists them
+    if (D == 2) {
+        if (d == 0) {
+            L[3][0] = u2; L[3][2] = -1.0;
+            R[2][3] = -1.0; R[3][3] = -u2;
+        } else {
+            L[3][0] = -u1; L[3][1] = 1.0;
+            R[1][3] = 1.0; R[3][3] = u1;
+        }
+    } else {
+        if (d == 0) {
+            L[3][0] = u2; L[3][2] = -1.0;
+            L[4][0] = -u3; L[4][3] = 1.0;
+            R[2][3] = -1.0; R[Q - 1][3] = -u2;
+            R[3][4] = 1.0; R[Q - 1][4] = u3;
+        } else if (d == 1) {
+            L[3][0] = -u1; L[3][1] = 1.0;
+            L[4][0] = u3; L[4][3] = -1.0;
+            R[1][3] = 1.0; R[Q - 1][3] = u1;
+            R[3][4] = -1.0; R[Q - 1][4] = -u3;
+        } else {
+            L[3][0] = u1; L[3][1] = -1.0;
+            L[4][0] = -u2; L[4][2] = 1.0;
+            R[1][3] = -1.0; R[Q - 1][3] = -u1;
+            R[2][4] = 1.0; R[Q - 1][4] = u2;
+        }
+    }
+}
+
+__device__ __forceinline__ double tvb_minmod(double a1, double a2, double a3) {
+    const bool same = (signbit(a1) == signbit(a2)) && (signbit(a2) == signbit(a3));
+    const double sign = signbit(a1) ? -1.0 : 1.0;
+    if (same) return sign * fmin(fmin(fabs(a1), fabs(a2)), fabs(a3));
+    return 0.0;
+}
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_limiter_tvb(LimArgs a) {
+#pragma clang fp contract(off)
+    using Dm = Dims<P, D>;
+    constexpr int Q = D + 2, Ns = Dm::Ns;
+    __shared__ unsigned int nlim;
+    if (threadIdx.x == 0) nlim = 0u;
+    __syncthreads();
+    if (a.t_limit > 1 && (a.ts->nstep % (unsigned)a.t_limit) != 0u) return;   // uniform over the grid
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) {
+        const long s = a.cells[i];
+        double u0[Q];
+        for (int q = 0; q < Q; ++q) u0[q] = a.Uw[((long)q * Ns) * a.NS + s];
+        double res[D][Q];
+        bool troubled = false;
+        for (int d = 0; d < D; ++d) {
+            const long sm = a.nbr[(long)(2 * d) * a.n + i], sp = a.nbr[(long)(2 * d + 1) * a.n + i];
+            double um[Q], up[Q], ul[Q];
+            for (int q = 0; q < Q; ++q) {
+                um[q] = a.Uw[((long)q * Ns) * a.NS + sm];
+                up[q] = a.Uw[((long)q * Ns) * a.NS + sp];
+                ul[q] = a.Uw[((long)q * Ns + a.lin[d]) * a.NS + s];
+            }
+            double L[Q][Q], R[Q][Q];
+            tvb_ev<D>(d, u0, a.gamma, L, R);
+            double vw[Q];
+            for (int q = 0; q < Q; ++q) {
+                double Dmv = 0.0, Dpv = 0.0, Du = 0.0;
+                for (int r = 0; r < Q; ++r) {
+                    Dmv += 0.5 * L[q][r] * (u0[r] - um[r]);
+                    Dpv += 0.5 * L[q][r] * (up[r] - u0[r]);
+                    Du += L[q][r] * ul[r];
+                }
+                vw[q] = fabs(Du) <= a.thr ? Du : tvb_minmod(Du, Dmv, Dpv);
+                troubled = troubled || (vw[q] != Du);
+            }
+            for (int q = 0; q < Q; ++q) {
+                double sum = 0.0;
+                for (int r = 0; r < Q; ++r) sum += R[q][r] * vw[r];
+                res[d][q] = sum;
+            }
+        }
+        if (troubled) {
+            for (int q = 0; q < Q; ++q)
+                for (int n = 1; n < Ns; ++n) {
+                    double v = 0.0;
+                    for (int d = 0; d < D; ++d)
+                        if (n == a.lin[d]) v = res[d][q];
+                    a.Uw[((long)q * Ns + n) * a.NS + s] = v;
+                }
+            atomicAdd(&nlim, 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && nlim) atomicAdd(a.count, (unsigned long long)nlim);
+}
+
+// ---------------------------------------------------------------------------
+// Refinement tags of one level (AmrDG::AMR_tag_cell_refinement,
+// AmrDG_TagCellRef.tpp:4-52), one thread per valid cell, one byte out:
+//   crit 1: the reference's Kelvin-Helmholtz indicator
+//           (Compressible_Euler::pde_tag_cell_refinement,
+//           Compressible_Euler.h:645-679): some characteristic slope
+//           |L_d U_lin,d| exceeds AMR_TVB_C[lev] * amr_c[lev] * h^2 (the
+//           troubled branch of minmodB);
+//   crit 2: density gradient (no counterpart in the reference): the cell
+//           mean density differs from a face neighbour's by more than
+//           amr_c[lev] times its own.
+// Neighbours are read at the slots beside the cell (ghost slots hold the
+// last ghost fill, which FillBoundary would reproduce).
+struct TagArgs {
+    const double* __restrict__ Uw;
+    const int* __restrict__ valid_list;
+    const int* __restrict__ cell_face[3][2];
+    const int* __restrict__ face_lr[3];
+    uint8_t* __restrict__ out;
+    long NS, vbegin, n;
+    int crit;
+    double thr;      // crit 1: AMR_TVB_C * amr_c * h^2; crit 2: amr_c
+    double gamma;
+    int lin[3];
+};
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_tag(TagArgs a) {
+#pragma clang fp contract(off)
+    using Dm = Dims<P, D>;
+    constexpr int Q = D + 2, Ns = Dm::Ns;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const long v = a.vbegin + i;
+    const long s = a.valid_list[v];
+    bool tag = false;
+    if (a.crit == 1) {
+        double u0[Q];
+        for (int q = 0; q < Q; ++q) u0[q] = a.Uw[((long)q * Ns) * a.NS + s];
+        for (int d = 0; d < D && !tag; ++d) {
+            double ul[Q];
+            for (int q = 0; q < Q; ++q) ul[q] = a.Uw[((long)q * Ns + a.lin[d]) * a.NS + s];
+            double L[Q][Q], R[Q][Q];
+            tvb_ev<D>(d, u0, a.gamma, L, R);
+            for (int q = 0; q < Q; ++q) {
+                double Du = 0.0;
+                for (int r = 0; r < Q; ++r) Du += L[q][r] * ul[r];
+                if (!(fabs(Du) <= a.thr)) tag = true;
+            }
+        }
+    } else {
+        const double r0 = a.Uw[s];
+        for (int d = 0; d < D; ++d) {
+            const long sl = a.face_lr[d][2 * (long)a.cell_face[d][0][v]];
+            const long sr = a.face_lr[d][2 * (long)a.cell_face[d][1][v] + 1];
+            const double j = fmax(fabs(a.Uw[sr] - r0), fabs(r0 - a.Uw[sl]));
+            if (j > a.thr * r0) tag = true;
+        }
+    }
+    a.out[i] = tag ? 1 : 0;
+}
+
+// RemakeLevel / MakeNewLevelFromCoarse data of one new level
+// (AmrDG_Level.tpp:52-80,120-170 through FillPatchTwoLevels /
+// InterpFromCoarseLevel with the L2ProjInterp mapper): every slot (valid and
+// ghost) of the new level takes the old level's valid cell at the same place
+// (kind 0, read from the old field), else the L2 projection of the new
+// coarse level's parent cell (kind 1, AmrDG_Interpolater.tpp:344-364);
+// kind >> 2 marks the mirrored directions of a transmissive boundary ghost.
+// One thread per (entry, variable).
+struct RemakeArgs {
+    const double* __restrict__ tab;
+    TabOff to;
+    double* __restrict__ Uw;            // new field
+    const double* __restrict__ Uw_old;  // old field
+    const int* __restrict__ slot;
+    const int* __restrict__ src;
+    const uint8_t* __restrict__ kind;
+    const uint8_t* __restrict__ child;
+    long NS, NS_old, n;
+};
+
+template <int P, int D>
+__global__ void __launch_bounds__(256) k_remake(RemakeArgs a) {
+    using Dm = Dims<P, D>;
+    constexpr int Q = D + 2, Ns = Dm::Ns;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n * Q) return;
+    const int q = (int)(t / a.n);
+    const long e = t - (long)q * a.n;
+    const long s = a.slot[e], src = a.src[e];
+    const int kind = a.kind[e], mask = kind >> 2;
+    double v[Ns];
+    if ((kind & 3) == 0) {
+        for (int n = 0; n < Ns; ++n) v[n] = a.Uw_old[((long)q * Ns + n) * a.NS_old + src];
+    } else {
+        const double* __restrict__ Pcf = a.tab + a.to.Pcf + (long)a.child[e] * Ns * Ns;
+        double u[Ns];
+        for (int k = 0; k < Ns; ++k) u[k] = a.Uw[((long)q * Ns + k) * a.NS + src];
+        for (int n = 0; n < Ns; ++n) {
+            double acc = 0.0;
+            for (int k = 0; k < Ns; ++k) acc = fma(Pcf[n * Ns + k], u[k], acc);
+            v[n] = a.tab[a.to.Minv + n] * acc;
+        }
+    }
+    for (int d = 0; d < D; ++d)
+        if (mask & (1 << d))
+            for (int n = 0; n < Ns; ++n) v[n] *= a.tab[a.to.sgn + d * Ns + n];
+    for (int n = 0; n < Ns; ++n) a.Uw[((long)q * Ns + n) * a.NS + s] = v[n];
 }
 
 // Conserved totals: per-block partial sums of vol * Uw[q][0] (deterministic
@@ -1761,16 +2044,18 @@ __device__ void exact_state(const LpArgs& a, const double* xin, double* u) {
 template <int P, int D>
 __global__ void __launch_bounds__(256) k_lp_error(LpArgs a) {
     using Dm = Dims<P, D>;
-    constexpr int Ns = Dm::Ns, Ms = Dm::Ms;
+    constexpr int Ns = Dm::Ns;
     __shared__ double red[256];
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double cell = 0.0;
     if (i < a.n) {
         const long s = a.list[i];
         const int lev = a.slot_lev[s];
-        const double* Vs = a.tab + a.to.Vs;
-        const double* ws = a.tab + a.to.ws;
-        const double* xs = a.tab + a.to.xs;
+        const TabOff::LpRule& r = a.to.lp[a.p == 2 ? 1 : 0];
+        const int Ms = r.M;
+        const double* Vs = a.tab + r.Vs;
+        const double* ws = a.tab + r.ws;
+        const double* xs = a.tab + r.xs;
         double U[Ns];
         for (int n = 0; n < Ns; ++n) U[n] = a.Uw[((long)a.q * Ns + n) * a.NS + s];
         for (int m = 0; m < Ms; ++m) {

@@ -124,6 +124,41 @@ ElementTables build_element_tables(int P, int D) {
             t.weights[q] = 2.0 / ((1.0 - x * x) * d * d);
         }
     }
+    // The L2 norm's rule: 2N nodes from the asymptotic start
+    // cos(theta) (1 - 1/(8 N2^2) + 1/(8 N2^3) - ...) of AmrDG.h:1595-1609,
+    // Newton on P_N2 to a step of 1e-15 (AmrDG_Quadrature.tpp:249-268),
+    // weights 2 / ((1 - x^2) P'_N2(x)^2).
+    {
+        const double pi = 3.14159265358979323846;
+        const int N2 = 2 * N;
+        const double n2 = (double)N2;
+        t.nodes2.assign(N2, 0.0);
+        t.weights2.assign(N2, 0.0);
+        int at = 0;
+        for (int i = 1; i <= N2 / 2; ++i) {
+            const double th = pi * (i - 0.25) / (n2 + 0.5);
+            double x;
+            if (i <= (int)((1.0 / 3.0) * n2)) {
+                const double s = std::sin(th);
+                x = (1.0 - 0.125 / (n2 * n2) + 0.125 / (n2 * n2 * n2) -
+                     (1.0 / 384.0) / (n2 * n2 * n2 * n2) * (39.0 - 28.0 / (s * s))) * std::cos(th);
+            } else {
+                x = (1.0 - 1.0 / (8.0 * n2 * n2) + 1.0 / (8.0 * n2 * n2 * n2)) * std::cos(th);
+            }
+            for (int it = 0; it < 1000; ++it) {
+                const double xn = x - leg(N2, x) / dleg(N2, x);
+                const double e = std::fabs(xn - x);
+                x = xn;
+                if (e <= 1e-15) break;
+            }
+            t.nodes2[at++] = x;
+            t.nodes2[at++] = -x;
+        }
+        for (int q = 0; q < N2; ++q) {
+            const double x = t.nodes2[q], d = dleg(N2, x);
+            t.weights2[q] = 2.0 / ((1.0 - x * x) * d * d);
+        }
+    }
     multi_indices(P, D, t.mi_s);
     multi_indices(P, D + 1, t.mi_st);
     t.Np_s = (int)t.mi_s.size();
@@ -432,6 +467,8 @@ Fragments build_fragments(const ElementTables& t, bool fp64_rows, bool node_lane
 const std::vector<double>* table_by_name(const ElementTables& t, const std::string& n) {
     if (n == "nodes") return &t.nodes;
     if (n == "weights") return &t.weights;
+    if (n == "nodes2") return &t.nodes2;
+    if (n == "weights2") return &t.weights2;
     if (n == "V") return &t.V;
     if (n == "Vinv") return &t.Vinv;
     if (n == "Mk_h_w") return &t.Mk_h_w;

@@ -17,6 +17,9 @@ struct ElementTables {
     int Np_s = 0, Np_st = 0, qMp_s = 0, qMp_st = 0, qMp_st_bd = 0;
     std::vector<std::array<int, 4>> mi_s, mi_st;   // total-degree multi-indices
     std::vector<double> nodes, weights;            // paired order [x1,-x1,...,0]
+    // the 2N-point Gauss-Legendre rule of the L2 norm (AmrDG.h:1587-1655),
+    // paired order [x1,-x1,x2,-x2,...]
+    std::vector<double> nodes2, weights2;
 
     // Reference matrices, row-major with the reference's (row, col) meaning.
     std::vector<double> V;            // [qMp_st][Np_st]

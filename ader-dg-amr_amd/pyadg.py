@@ -30,6 +30,7 @@ EXPORTS = [
     "adg_mlp_dataset", "adg_mlp_dataset_clear", "adg_mlp_fit_normalization", "adg_mlp_train",
     "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats", "adg_lp_error",
     "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
+    "adg_set_limiter", "adg_limit", "adg_limiter_stats",
 ]
 
 
@@ -69,6 +70,9 @@ def lib():
     L.adg_set_time.argtypes = [C.c_void_p, C.c_double]
     L.adg_get_time.argtypes = [C.c_void_p, _dp, _dp]
     L.adg_conserved_totals.argtypes = [C.c_void_p, _dp]
+    L.adg_set_limiter.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.c_int]
+    L.adg_limit.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
+    L.adg_limiter_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
     L.adg_sync.argtypes = [C.c_void_p]
     L.adg_stream.argtypes = [C.c_void_p]
     L.adg_stream.restype = C.c_void_p
@@ -253,6 +257,22 @@ class Context:
         out = np.zeros(self.Q)
         self._chk(self.L.adg_conserved_totals(self.h, _d(out)), "adg_conserved_totals")
         return out
+
+    def set_limiter(self, on=True, M=0.0, tvb_c=(1.0,), t_limit=1):
+        """TVB limiter on the finest level in every step (on) or none."""
+        c = np.ascontiguousarray(tvb_c, dtype=np.float64)
+        self._chk(self.L.adg_set_limiter(self.h, 1 if on else 0, M, c.size, _d(c), t_limit), "adg_set_limiter")
+
+    def limit(self):
+        """Limiter_w on the finest level now; returns the cells limited."""
+        n = C.c_ulonglong(0)
+        self._chk(self.L.adg_limit(self.h, C.byref(n)), "adg_limit")
+        return n.value
+
+    def limiter_stats(self, reset=False):
+        n = C.c_ulonglong(0)
+        self._chk(self.L.adg_limiter_stats(self.h, C.byref(n), 1 if reset else 0), "adg_limiter_stats")
+        return n.value
 
     def project_ic(self, lev, ic_case, params):
         prm = np.array(params, dtype=np.float64)

@@ -146,7 +146,12 @@ int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
  * wavefront.  ms_out[7] (nullable) gets the average launch time of each
  * candidate (-1 where it does not apply), choice_out[3] (nullable) the
  * chosen {kind, waves, warm-start tiles (1, 2, or 0 for the 8-element
- * tile)}.  No counterpart in the reference. */
+ * tile)}.  The predictor kernels round differently (FP64: VALU, quad-gather
+ * and node-lane agree to ~1e-15 per step, not bitwise; the node-lane shapes
+ * with 8 and 16 waves give the same bits), so a run that must be
+ * bit-reproducible across processes records choice_out and pins it with
+ * adg_set_predictor_kernel / adg_set_predictor_waves / adg_set_warm_tiles
+ * instead of re-tuning.  No counterpart in the reference. */
 int adg_autotune(adg_ctx* ctx, int reps, double* ms_out, int* choice_out);
 /* Wavefronts per workgroup of the node-lane predictor: 8 (default) or 16.
  * No counterpart in the reference: a launch-shape knob of this
@@ -205,6 +210,72 @@ int adg_set_time(adg_ctx* ctx, double t);
 /* Blocking copy of the device-resident time t and step size Dt (the members
  * t, Dt of Solver, Core/Solver.h; either pointer nullable). */
 int adg_get_time(adg_ctx* ctx, double* t_out, double* dt_out);
+
+/* TVB limiter settings, the limiter arguments of
+ * AmrDG::setNumericalSettings (AmrDG.tpp:50-58: limiter_type, TVB_M,
+ * AMR_TVB_C, t_limit).  type 0: no limiter; type 1: "TVB".  tvb_c[ntvb]
+ * holds AMR_TVB_C per level (levels past ntvb use 1.0).  With type 1 every
+ * step (adg_step, adg_step_graph) runs Limiter_w on the finest level after
+ * the corrector and before the average down when n % t_limit == 0, n the
+ * step count since this call (AmrDG.h:990-997).  Drops a captured step
+ * graph. */
+int adg_set_limiter(adg_ctx* ctx, int type, double M, int ntvb, const double* tvb_c, int t_limit);
+/* Replaces AmrDG::Limiter_w (AmrDG.h:1657-1770) with Limiter_linear_tvb
+ * (AmrDG.h:1772-1855): the characteristic TVB limiting of the linear modes
+ * of every valid cell of the finest level, now, with the settings of
+ * adg_set_limiter (M and AMR_TVB_C; used whatever its type).
+ * n_limited (nullable) receives the number of cells limited (blocking). */
+int adg_limit(adg_ctx* ctx, unsigned long long* n_limited);
+/* Cells limited by every limiter launch so far (n_limited_total of
+ * AmrDG::evolve, AmrDG.h:1036-1046); reset != 0 clears the count after
+ * reading it.  total nullable. */
+int adg_limiter_stats(adg_ctx* ctx, unsigned long long* total, int reset);
+
+/* ---- tagging-driven regrid (AmrCore::regrid(0, t) of Mesh, Core/Mesh.h:128-196,
+ * at the regrid points of AmrDG::evolve, AmrDG.h:936-986) ---- */
+
+/* AMR settings of Mesh (Core/Mesh.h:128-147): max_level (index of the finest
+ * level allowed), blocking factor (the reference sets 4, Mesh.h:146),
+ * max_grid_size, grid efficiency, n_error_buf, n_proper (AMReX's defaults:
+ * 128 in 2D / 32 in 3D, 0.7, 1, 1). */
+int adg_set_amr(adg_ctx* ctx, int max_level, int blocking_factor, int max_grid_size, double grid_eff,
+                int n_error_buf, int n_proper);
+/* Refinement criterion of Compressible_Euler::pde_tag_cell_refinement
+ * (Compressible_Euler.h:605-682) with amr_c per level (Mesh::amr_c; n
+ * values, the last repeated):
+ *   0: isentropic_vortex, cell centre within amr_c^2 of (params[0], params[1])
+ *      (periodic distance; the reference's centre is (5, 5));
+ *   1: kelvin_helmolz_instability, the TVB troubled-cell indicator: some
+ *      characteristic slope above AMR_TVB_C[lev] * amr_c[lev] * h^2
+ *      (AMR_TVB_C from adg_set_limiter);
+ *   2: density gradient (no counterpart in the reference): a face
+ *      neighbour's cell-mean density differs by more than amr_c[lev] times
+ *      the cell's own. */
+int adg_set_tagging(adg_ctx* ctx, int criterion, int n, const double* amr_c, int nprm, const double* params);
+/* Replaces AmrDG::AMR_tag_cell_refinement (AmrDG_TagCellRef.tpp:4-52) on
+ * level lev: one byte per valid cell of the level (patch order, x fastest),
+ * 1 = tagged (blocking). */
+int adg_tag_cells(adg_ctx* ctx, int lev, unsigned char* tags);
+/* Replaces Mesh::regrid(0, t) (AmrDG.h:948): tags levels 0..min(finest,
+ * max_level-1), makes the new fine-level boxes (AmrCore::MakeNewGrids:
+ * n_error_buf buffer, blocking-factor coarsening, Berger-Rigoutsos
+ * clustering at grid_eff, nesting of each new level in the one below it by
+ * n_proper, max_grid_size), and when they differ from the current ones
+ * remakes the hierarchy as adg_remake.  *changed (nullable) = 1 when the
+ * boxes changed. */
+int adg_regrid(adg_ctx* ctx, double time, int* changed);
+/* Replaces Mesh::RemakeLevel / MakeNewLevelFromCoarse / ClearLevel
+ * (Core/Mesh.h:157-196 -> AmrDG_Level.tpp:52-224) for a given layout: level 0
+ * must keep its boxes; every slot (valid and ghost) of a new level l >= 1
+ * takes the old level-l valid cell at its place, else the L2 projection of
+ * the new level l-1 parent (FillPatchTwoLevels / InterpFromCoarseLevel with
+ * L2ProjInterp); transmissive boundary ghosts mirror; then the flux
+ * registers, ghost lists and face masks are rebuilt (AMR_set_flux_registers).
+ * Drops a captured step graph. */
+int adg_remake(adg_ctx* ctx, int nlev, const int* npatch_per_lev, const int* boxes);
+/* Current layout: *nlev, npatch[nlev], boxes (lo[dim], hi[dim] per patch,
+ * at most cap patches written); returns the number of patches. */
+int adg_get_levels(adg_ctx* ctx, int* nlev, int* npatch, int* boxes, int cap);
 
 /* Conserved totals sum(vol * U_w[q][0]) over uncovered valid cells, FP64
  * (the quantity LpNorm_DG_AMR integrates over, AmrDG.h:1399-1578). */

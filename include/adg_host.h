@@ -25,6 +25,17 @@ int adg_host_inspect(int dim, const int* dom_n, const int* periodic, int nlev, c
  * of doubles or < 0. */
 int adg_host_table(int P, int D, const char* name, double* out, int cap);
 
+/* New fine-level boxes from refinement tags, on the host (the grid
+ * generation of adg_regrid, AmrCore::MakeNewGrids semantics): the old layout
+ * (nlev_old, npatch_old, boxes_old as adg_set_levels takes them), the tagged
+ * cells of levels 0..min(nlev_old-1, max_level-1) (ntags[l] cells of level l,
+ * dim ints each, concatenated).  Writes *nlev_new, npatch_new[] (level 0
+ * unchanged) and up to cap_boxes boxes; returns the number of boxes or < 0. */
+int adg_host_make_grids(int dim, const int* dom0, const int* periodic, int max_level, int blocking_factor,
+                        int max_grid_size, double grid_eff, int n_error_buf, int n_proper, int nlev_old,
+                        const int* npatch_old, const int* boxes_old, const int* ntags, const int* tags,
+                        int* nlev_new, int* npatch_new, int* boxes_new, int cap_boxes);
+
 /* Runs one 16x16x4 MFMA on the current device and returns the 64x4
  * accumulator (prec 0: FP64, 1: FP32; mode 0 probes A rows, mode 1 B
  * columns).  Needs a GPU. */

@@ -1541,6 +1541,9 @@ extern "C" double orc_set_dt(const orc_tables* t, const orc_hier* h, double** Uw
     return c_dt * dt_min;
 }
 
+extern "C" long orc_limiter_tvb(const orc_tables* t, const orc_hier* h, double** Uw, int l, double M,
+                                const double* tvb_c, double gamma);
+
 extern "C" int orc_evolve(const orc_tables* t, const orc_hier* h, double** Uw, double* t_io,
                           double* Dt_io, double T, double c_dt, double gamma, int nsteps_max) {
     // AmrDG::evolve (AmrDG.h:853-1055) without regrid, limiter and plots
@@ -1560,12 +1563,94 @@ extern "C" int orc_evolve(const orc_tables* t, const orc_hier* h, double** Uw, d
     return n;
 }
 
+// NewtonRhapson (AmrDG_Quadrature.tpp:249-268): Newton on P_n with the
+// derivative from assoc_legendre(n,1,x)/sqrt(1-x^2), step tolerance 1e-15.
+static void orc_newton_rhapson(double& x, int n) {
+    for (int it = 0; it < 1000; ++it) {
+        double df = (std::assoc_legendre(n, 1, x)) / (std::sqrt(1.0 - std::pow(x, 2.0)));
+        double f = std::legendre(n, x);
+        double x_new = x - (f / df);
+        double err = std::abs(x_new - x);
+        x = x_new;
+        if (err <= 1e-15) break;
+    }
+}
+
+// The 2(P+1)-point rule of L2Norm_DG_AMR (AmrDG.h:1587-1655): asymptotic
+// starting values, Newton, order [x1,-x1,x2,-x2,...]; points of the D-fold
+// product with dimension 0 slowest.
+static std::vector<std::vector<double>> orc_l2_points(int N, int D) {
+    std::vector<double> gl;
+    for (int i = 1; i <= (int)(N / 2); ++i) {
+        double theta = kPi * (i - 0.25) / ((double)N + 0.5), xiq;
+        if ((1 <= i) && (i <= (int)((1.0 / 3.0) * (double)N))) {
+            xiq = (1 - 0.125 * (1.0 / std::pow(N, 2)) + 0.125 * (1.0 / std::pow(N, 3)) -
+                   (1.0 / 384.0) * (1.0 / std::pow(N, 4)) * (39.0 - 28.0 * (1.0 / std::pow(std::sin(theta), 2)))) *
+                  std::cos(theta);
+        } else {
+            xiq = (1.0 - (1.0 / (8.0 * std::pow((double)N, 2))) + (1.0 / (8.0 * std::pow((double)N, 3)))) *
+                  std::cos(theta);
+        }
+        orc_newton_rhapson(xiq, N);
+        gl.push_back(xiq);
+        gl.push_back(-xiq);
+    }
+    const int M = ipow(N, D);
+    std::vector<std::vector<double>> pts(M, std::vector<double>(D));
+    for (int m = 0; m < M; ++m) {
+        int div = M;
+        for (int d = 0; d < D; ++d) {
+            div /= N;
+            pts[m][d] = gl[(m / div) % N];
+        }
+    }
+    return pts;
+}
+
+extern "C" int orc_l2_nodes(int P, double* out) {
+    // the 1D node set of the L2 rule (2(P+1) values), for tests
+    const int N = 2 * (P + 1);
+    std::vector<std::vector<double>> pts = orc_l2_points(N, 1);
+    for (int i = 0; i < N; ++i) out[i] = pts[i][0];
+    return N;
+}
+
+extern "C" int orc_evolve_lim(const orc_tables* t, const orc_hier* h, double** Uw, double* t_io,
+                              double* Dt_io, double T, double c_dt, double gamma, int nsteps_max, double M,
+                              const double* tvb_c, int t_limit, long* n_limited) {
+    // AmrDG::evolve (AmrDG.h:930-1030) without regrid and plots, with the
+    // TVB limiter on the finest level after the ADER step every t_limit
+    // steps (AmrDG.h:990-997), before the average down
+    int n = 0;
+    long lim = 0;
+    double tt = *t_io, Dt = *Dt_io;
+    while (tt < T && (nsteps_max <= 0 || n < nsteps_max)) {
+        orc_ader(t, h, Uw, Dt, gamma, nullptr);
+        if (t_limit > 0 && (n % t_limit == 0)) lim += orc_limiter_tvb(t, h, Uw, h->nlev - 1, M, tvb_c, gamma);
+        orc_average_down(t, h, Uw);
+        orc_fill_patch(t, h, Uw);
+        n += 1;
+        tt += Dt;
+        Dt = orc_set_dt(t, h, Uw, c_dt, gamma);
+        if (T - tt < Dt) Dt = T - tt;
+    }
+    *t_io = tt;
+    *Dt_io = Dt;
+    if (n_limited) *n_limited = lim;
+    return n;
+}
+
 extern "C" double orc_lp_error(const orc_tables* t, const orc_hier* h, double** Uw, int qq, int p,
                                int ic_case, const double* prm, double gamma, double time) {
-    // LpNorm_DG_AMR (AmrDG.h:1399-1578) with (P+1)^D GL points (L1 path);
-    // the exact solution is the IC advected by the background velocity
+    // LpNorm_DG_AMR (AmrDG.h:1399-1578): L1Norm_DG_AMR on the (P+1)^D
+    // solution points, L2Norm_DG_AMR on the (2(P+1))^D points of its own rule
+    // (AmrDG.h:1580-1655); weights 2/assoc_legendre(N,1,x)^2 per dimension.
+    // The exact solution is the IC advected by the background velocity
     // (periodic shift), which reduces to the reference's IC comparison at time 0.
-    const int D = t->D, Ns = t->Np_s, Ms = t->qMp_s, N = t->N;
+    const int D = t->D, Ns = t->Np_s;
+    const int N = p == 2 ? 2 * t->N : t->N;
+    const std::vector<std::vector<double>> pts = p == 2 ? orc_l2_points(N, D) : t->xs;
+    const int Ms = (int)pts.size();
     double total = 0.0, Vamr = 0.0;
     for (int l = 0; l < h->nlev; ++l) {
         const OLevel& lv = h->L[l];
@@ -1584,16 +1669,16 @@ extern "C" double orc_lp_error(const orc_tables* t, const orc_hier* h, double** 
             double cell = 0.0;
             for (int m = 0; m < Ms; ++m) {
                 double w = 1.0;
-                for (int d = 0; d < D; ++d) w *= 2.0 / std::pow(std::assoc_legendre(N, 1, t->xs[m][d]), 2);
+                for (int d = 0; d < D; ++d) w *= 2.0 / std::pow(std::assoc_legendre(N, 1, pts[m][d]), 2);
                 double uh = 0.0;
                 for (int n = 0; n < Ns; ++n) {
                     double phi = 1.0;
-                    for (int d = 0; d < D; ++d) phi *= cx_legendre(t->mi_s[n][d], t->xs[m][d]);
+                    for (int d = 0; d < D; ++d) phi *= cx_legendre(t->mi_s[n][d], pts[m][d]);
                     uh += Uw[l][((long)qq * Ns + n) * ns + s] * phi;
                 }
                 double x[3], u[5];
                 for (int d = 0; d < D; ++d) {
-                    x[d] = (lv.dx[d] / 2.0) * t->xs[m][d] + (h->plo[d] + (iv[d] + 0.5) * lv.dx[d]);
+                    x[d] = (lv.dx[d] / 2.0) * pts[m][d] + (h->plo[d] + (iv[d] + 0.5) * lv.dx[d]);
                     if ((ic_case == 0 && d < 2) || ic_case == 5) {
                         // translate by the background velocity on the periodic domain
                         const double vel5[3] = {1.0, 0.5, 0.25};
@@ -1613,6 +1698,223 @@ extern "C" double orc_lp_error(const orc_tables* t, const orc_hier* h, double** 
         Vamr += vol * (double)ncell;
     }
     return std::pow(total / Vamr, 1.0 / (double)p);
+}
+
+// ---------------------------------------------------------------------------
+// TVB limiter: Limiter_w (AmrDG.h:1657-1770), Limiter_linear_tvb
+// (AmrDG.h:1772-1855), minmodB / minmod (AmrDG_Limiter.tpp:41-84) and the
+// characteristic matrices pde_EV_Lmatrix / pde_EV_Rmatrix
+// (Compressible_Euler.cpp:138-277), evaluated at U_w mode 0 (the arrays the
+// limiter passes are the modal ones, component m = 0).
+
+// pde_EV_Rmatrix (Compressible_Euler.cpp:138-205), R[row][col]
+static void ev_right(int D, int d, const double* u, double gamma, double R[5][5]) {
+    const double c = soundspeed_(D, u, gamma);
+    const double u1 = u[1] / u[0], u2 = u[2] / u[0], u3 = D == 3 ? u[3] / u[0] : 0.0;
+    const double ek = D == 2 ? 0.5 * (std::pow(u1, 2) + std::pow(u2, 2))
+                             : 0.5 * (std::pow(u1, 2) + std::pow(u2, 2) + std::pow(u3, 2));
+    const double hh = (std::pow(c, 2) / (gamma - 1.0)) + ek;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) R[i][j] = 0.0;
+    if (D == 2) {
+        if (d == 0) {
+            const double r[4][4] = {{1.0, 1.0, 1.0, 0.0}, {u1 - c, u1, u1 + c, 0.0}, {u2, u2, u2, -1.0},
+                                    {hh - c * u1, ek, hh + c * u1, -u2}};
+            for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) R[i][j] = r[i][j];
+        } else {
+            const double r[4][4] = {{1.0, 1.0, 1.0, 0.0}, {u1, u1, u1, 1.0}, {u2 - c, u2, u2 + c, 0.0},
+                                    {hh - c * u2, ek, hh + c * u2, u1}};
+            for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) R[i][j] = r[i][j];
+        }
+    } else {
+        if (d == 0) {
+            const double r[5][5] = {{1.0, 1.0, 1.0, 0.0, 0.0}, {u1 - c, u1, u1 + c, 0.0, 0.0},
+                                    {u2, u2, u2, -1.0, 0.0}, {u3, u3, u3, 0.0, 1.0},
+                                    {hh - c * u1, ek, hh + c * u1, -u2, u3}};
+            for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) R[i][j] = r[i][j];
+        } else if (d == 1) {
+            const double r[5][5] = {{1.0, 1.0, 1.0, 0.0, 0.0}, {u1, u1, u1, 1.0, 0.0},
+                                    {u2 - c, u2, u2 + c, 0.0, 0.0}, {u3, u3, u3, 0.0, -1.0},
+                                    {hh - c * u2, ek, hh + c * u2, u1, -u3}};
+            for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) R[i][j] = r[i][j];
+        } else {
+            const double r[5][5] = {{1.0, 1.0, 1.0, 0.0, 0.0}, {u1, u1, u1, -1.0, 0.0},
+                                    {u2, u2, u2, 0.0, 1.0}, {u3 - c, u3, u3 + c, 0.0, 0.0},
+                                    {hh - c * u3, ek, hh + c * u3, -u1, u2}};
+            for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) R[i][j] = r[i][j];
+        }
+    }
+}
+
+// pde_EV_Lmatrix (Compressible_Euler.cpp:207-277), L[row][col]
+static void ev_left(int D, int d, const double* u, double gamma, double L[5][5]) {
+    const double c = soundspeed_(D, u, gamma);
+    const double u1 = u[1] / u[0], u2 = u[2] / u[0], u3 = D == 3 ? u[3] / u[0] : 0.0;
+    const double ek = D == 2 ? 0.5 * (std::pow(u1, 2) + std::pow(u2, 2))
+                             : 0.5 * (std::pow(u1, 2) + std::pow(u2, 2) + std::pow(u3, 2));
+    const double b = 1.0 / (2.0 * std::pow(c, 2));
+    const double g = gamma - 1.0;
+    const double ph = g * ek;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) L[i][j] = 0.0;
+    if (D == 2) {
+        if (d == 0) {
+            const double l[4][4] = {{b * (ph + c * u1), -b * (g * u1 + c), -b * g * u2, b * g},
+                                    {1.0 - 2.0 * b * ph, 2.0 * b * g * u1, 2.0 * b * g * u2, -2.0 * b * g},
+                                    {b * (ph - c * u1), -b * (g * u1 - c), -b * (g * u2), b * g},
+                                    {u2, 0.0, -1.0, 0.0}};
+            for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) L[i][j] = l[i][j];
+        } else {
+            const double l[4][4] = {{b * (ph + c * u2), -b * g * u1, -b * (g * u2 + c), b * g},
+                                    {1.0 - 2.0 * b * ph, 2.0 * b * g * u1, 2.0 * b * g * u2, -2.0 * b * g},
+                                    {b * (ph - c * u2), -b * (g * u1), -b * (g * u2 - c), b * g},
+                                    {-u1, 1.0, 0.0, 0.0}};
+            for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) L[i][j] = l[i][j];
+        }
+    } else {
+        if (d == 0) {
+            const double l[5][5] = {
+                {b * (ph + c * u1), -b * (g * u1 + c), -b * g * u2, -b * g * u3, b * g},
+                {1.0 - 2.0 * b * ph, 2.0 * b * g * u1, 2.0 * b * g * u2, 2.0 * b * g * u3, -2.0 * b * g},
+                {b * (ph - c * u1), -b * (g * u1 - c), -b * (g * u2), -b * g * u3, b * g},
+                {u2, 0.0, -1.0, 0.0, 0.0},
+                {-u3, 0.0, 0.0, 1.0, 0.0}};
+            for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) L[i][j] = l[i][j];
+        } else if (d == 1) {
+            const double l[5][5] = {
+                {b * (ph + c * u2), -b * g * u1, -b * (g * u2 + c), -b * g * u3, b * g},
+                {1.0 - 2.0 * b * ph, 2.0 * b * g * u1, 2.0 * b * g * u2, 2.0 * b * g * u3, -2.0 * b * g},
+                {b * (ph - c * u2), -b * (g * u1), -b * (g * u2 - c), -b * g * u3, b * g},
+                {-u1, 1.0, 0.0, 0.0, 0.0},
+                {u3, 0.0, 0.0, -1.0, 0.0}};
+            for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) L[i][j] = l[i][j];
+        } else {
+            const double l[5][5] = {
+                {b * (ph + c * u3), -b * g * u1, -b * g * u2, -b * (g * u3 + c), b * g},
+                {1.0 - 2.0 * b * ph, 2.0 * b * g * u1, 2.0 * b * g * u2, 2.0 * b * g * u3, -2.0 * b * g},
+                {b * (ph - c * u3), -b * (g * u1), -b * g * u2, -b * (g * u3 - c), b * g},
+                {u1, -1.0, 0.0, 0.0, 0.0},
+                {-u2, 0.0, 1.0, 0.0, 0.0}};
+            for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) L[i][j] = l[i][j];
+        }
+    }
+}
+
+// minmod (AmrDG_Limiter.tpp:62-84): the smallest magnitude with the common
+// sign (signbit), or 0 when the three disagree
+static double minmod_(double a1, double a2, double a3) {
+    const bool same = (std::signbit(a1) == std::signbit(a2)) && (std::signbit(a2) == std::signbit(a3));
+    const int sign = std::signbit(a1) == std::signbit(-1) ? -1 : +1;
+    if (same) return sign * std::min({std::abs(a1), std::abs(a2), std::abs(a3)});
+    return 0;
+}
+// minmodB (AmrDG_Limiter.tpp:41-60): a1 kept when |a1| <= C M h^2
+static double minmodB_(double a1, double a2, double a3, double C, double M, double h, bool* troubled) {
+    if (std::abs(a1) <= C * M * std::pow(h, 2.0)) {
+        *troubled = false;
+        return a1;
+    }
+    *troubled = true;
+    return minmod_(a1, a2, a3);
+}
+
+extern "C" void orc_fill_boundary(const orc_tables* t, const orc_hier* h, double** Uw, int l) {
+    // MultiFab::FillBoundary(periodicity) of one level: the ghost cells a
+    // valid cell of the same level (directly or through a periodic image)
+    // covers; coarse-fine and physical-boundary ghosts keep their values
+    const int Q = t->Q, Ns = t->Np_s;
+    const OLevel& lv = h->L[l];
+    const long ns = lv.nslot;
+    for (size_t g = 0; g < lv.gslot.size(); ++g) {
+        if (lv.gkind[g] != 0 || lv.gmask[g] != 0) continue;
+        const long s = lv.gslot[g], src = lv.gsrc[g];
+        for (int q = 0; q < Q; ++q)
+            for (int n = 0; n < Ns; ++n) Uw[l][((long)q * Ns + n) * ns + s] = Uw[l][((long)q * Ns + n) * ns + src];
+    }
+}
+
+extern "C" long orc_limiter_tvb(const orc_tables* t, const orc_hier* h, double** Uw, int l, double M,
+                                const double* tvb_c, double gamma) {
+    // Limiter_w on level l: FillBoundary, Limiter_linear_tvb on every valid
+    // cell in place (it writes modes n >= 1 only and reads the neighbours'
+    // mode 0, so the cell order does not matter), FillBoundary; returns the
+    // number of cells limited.
+    const int D = t->D, Q = t->Q, Ns = t->Np_s;
+    const OLevel& lv = h->L[l];
+    const long ns = lv.nslot;
+    orc_fill_boundary(t, h, Uw, l);
+    double hmin = lv.dx[0];
+    for (int d = 1; d < D; ++d) hmin = std::min(hmin, lv.dx[d]);
+    int lin_mode_idx[3] = {0, 0, 0};   // AmrDG.tpp:174-184
+    for (int d = 0; d < D; ++d)
+        for (int n = 0; n < Ns; ++n) {
+            bool lin = t->mi_s[n][d] == 1, others = true;
+            for (int dd = 0; dd < D; ++dd)
+                if (dd != d && t->mi_s[n][dd] != 0) others = false;
+            if (lin && others) { lin_mode_idx[d] = n; break; }
+        }
+    auto U = [&](int q, int n, long s) -> double& { return Uw[l][((long)q * Ns + n) * ns + s]; };
+    long count = 0;
+    for (long s = 0; s < ns; ++s) {
+        if (!lv.valid[s]) continue;
+        int b, iv[3];
+        h->slot_decode(l, s, &b, iv);
+        double res_limit[3][5] = {};
+        double vw[5][3];
+        bool troubled = false;
+        for (int li = 0; li < D; ++li) {
+            const int sm = lin_mode_idx[li];
+            int sh[3] = {0, 0, 0}, dd = 0;
+            for (int d = 0; d < D; ++d)
+                if (t->mi_s[sm][d] == 1) { sh[d] = 1; dd = d; break; }
+            int ivm[3] = {iv[0] - sh[0], iv[1] - sh[1], iv[2] - sh[2]};
+            int ivp[3] = {iv[0] + sh[0], iv[1] + sh[1], iv[2] + sh[2]};
+            const long sm1 = h->slot_of(l, b, ivm), sp1 = h->slot_of(l, b, ivp);
+            double u0[5];
+            for (int q = 0; q < Q; ++q) u0[q] = U(q, 0, s);
+            double Lm[5][5], Rm[5][5];
+            ev_left(D, dd, u0, gamma, Lm);
+            for (int q = 0; q < Q; ++q) {
+                double Dm = 0.0, Dp = 0.0, Du = 0.0;
+                for (int r = 0; r < Q; ++r) {
+                    Dm += 0.5 * Lm[q][r] * (U(r, 0, s) - U(r, 0, sm1));
+                    Dp += 0.5 * Lm[q][r] * (U(r, 0, sp1) - U(r, 0, s));
+                    Du += Lm[q][r] * U(r, sm, s);
+                }
+                bool tf = false;
+                vw[q][li] = minmodB_(Du, Dm, Dp, tvb_c[l], M, hmin, &tf);
+                troubled = troubled || (vw[q][li] != Du);
+            }
+            ev_right(D, dd, u0, gamma, Rm);
+            for (int q = 0; q < Q; ++q) {
+                double sum = 0.0;
+                for (int r = 0; r < Q; ++r) sum += Rm[q][r] * vw[r][li];
+                res_limit[li][q] = sum;
+            }
+        }
+        if (troubled) {
+            for (int q = 0; q < Q; ++q) {
+                for (int n = 1; n < Ns; ++n) U(q, n, s) = 0.0;
+                for (int li = 0; li < D; ++li) U(q, lin_mode_idx[li], s) = res_limit[li][q];
+            }
+            ++count;
+        }
+    }
+    orc_fill_boundary(t, h, Uw, l);
+    return count;
+}
+
+extern "C" void orc_ev_matrices(int D, int d, const double* u, double gamma, double* Lout, double* Rout) {
+    // the characteristic matrices at state u, row-major Q x Q (for tests)
+    double Lm[5][5], Rm[5][5];
+    ev_left(D, d, u, gamma, Lm);
+    ev_right(D, d, u, gamma, Rm);
+    const int Q = D + 2;
+    for (int i = 0; i < Q; ++i)
+        for (int j = 0; j < Q; ++j) {
+            Lout[i * Q + j] = Lm[i][j];
+            Rout[i * Q + j] = Rm[i][j];
+        }
 }
 
 extern "C" void orc_totals(const orc_tables* t, const orc_hier* h, double** Uw, double* out) {

@@ -113,6 +113,26 @@ int         orc_evolve(const orc_tables*, const orc_hier*, double** Uw,
 double      orc_lp_error(const orc_tables*, const orc_hier*, double** Uw, int q,
                          int p, int ic_case, const double* params, double gamma,
                          double t);
+/* TVB limiter of one level (Limiter_w + Limiter_linear_tvb, AmrDG.h:1657-1855):
+ * FillBoundary, then per valid cell the characteristic minmodB limiting of
+ * the linear modes (threshold tvb_c[lev] * M * min(dx)^2); a troubled cell
+ * keeps mode 0, gets the limited linear modes and loses every higher mode.
+ * Returns the number of cells limited. */
+long        orc_limiter_tvb(const orc_tables*, const orc_hier*, double** Uw, int lev,
+                            double M, const double* tvb_c, double gamma);
+/* MultiFab::FillBoundary of one level: same-level and periodic ghost copies. */
+void        orc_fill_boundary(const orc_tables*, const orc_hier*, double** Uw, int lev);
+/* pde_EV_Lmatrix / pde_EV_Rmatrix at state u (row-major Q x Q). */
+void        orc_ev_matrices(int D, int d, const double* u, double gamma, double* L, double* R);
+/* orc_evolve with the limiter on the finest level every t_limit steps
+ * (t_limit <= 0: never); *n_limited (nullable) gets the cells limited. */
+int         orc_evolve_lim(const orc_tables*, const orc_hier*, double** Uw,
+                           double* t_io, double* Dt_io, double T, double c_dt,
+                           double gamma, int nsteps_max, double M, const double* tvb_c,
+                           int t_limit, long* n_limited);
+/* The 1D nodes of the L2 norm's 2(P+1)-point rule (L2Norm_DG_AMR), in the
+ * reference's order; returns their count. */
+int         orc_l2_nodes(int P, double* out);
 /* Conserved totals: sum over uncovered valid cells of vol * Uw[q][0] */
 void        orc_totals(const orc_tables*, const orc_hier*, double** Uw, double* out);
 

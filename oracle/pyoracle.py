@@ -63,6 +63,17 @@ def lib():
         L.orc_lp_error.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), C.c_int, C.c_int, C.c_int,
                                    dptr, C.c_double, C.c_double]
         L.orc_totals.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), dptr]
+        L.orc_limiter_tvb.restype = C.c_long
+        L.orc_limiter_tvb.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), C.c_int, C.c_double, dptr,
+                                      C.c_double]
+        L.orc_fill_boundary.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), C.c_int]
+        L.orc_ev_matrices.argtypes = [C.c_int, C.c_int, dptr, C.c_double, dptr, dptr]
+        L.orc_evolve_lim.restype = C.c_int
+        L.orc_evolve_lim.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), dptr, dptr, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, C.c_double, dptr, C.c_int,
+                                     C.POINTER(C.c_long)]
+        L.orc_l2_nodes.restype = C.c_int
+        L.orc_l2_nodes.argtypes = [C.c_int, dptr]
         fptr = C.POINTER(C.c_float)
         L.orc_predictor_iters.argtypes = [C.c_void_p, C.c_void_p, C.c_int, dptr, C.c_double, C.c_double,
                                           C.c_int, dptr]
@@ -139,6 +150,11 @@ class Hier:
         lib().orc_hier_valid_mask(self.h, l, _i(out))
         return out.astype(bool)
 
+    def slot_centre(self, l, slot):
+        x = np.zeros(3)
+        lib().orc_hier_slot_centre(self.h, l, slot, _d(x))
+        return x[:self.D]
+
     def face_mask(self, l, d, kind):
         out = np.zeros(self.nfslot(l, d), dtype=np.int32)
         lib().orc_hier_face_mask(self.h, l, d, kind, _i(out))
@@ -208,6 +224,43 @@ def evolve(T, H, Uw, t, Dt, Tend, c_dt, gamma=1.4, nsteps_max=0):
 def lp_error(T, H, Uw, q, p, ic_case, params, t, gamma=1.4):
     prm = np.array(params, dtype=np.float64)
     return lib().orc_lp_error(T.h, H.h, _ptrs(Uw), q, p, ic_case, _d(prm), gamma, t)
+
+
+def limiter_tvb(T, H, Uw, lev, M, tvb_c, gamma=1.4):
+    """TVB limiter of level lev in place; returns the number of cells limited."""
+    c = np.ascontiguousarray(tvb_c, dtype=np.float64)
+    return lib().orc_limiter_tvb(T.h, H.h, _ptrs(Uw), lev, M, _d(c), gamma)
+
+
+def fill_boundary(T, H, Uw, lev):
+    lib().orc_fill_boundary(T.h, H.h, _ptrs(Uw), lev)
+
+
+def ev_matrices(D, d, u, gamma=1.4):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    Lm = np.zeros((D + 2) * (D + 2))
+    Rm = np.zeros((D + 2) * (D + 2))
+    lib().orc_ev_matrices(D, d, _d(u), gamma, _d(Lm), _d(Rm))
+    return Lm.reshape(D + 2, D + 2), Rm.reshape(D + 2, D + 2)
+
+
+def evolve_lim(T, H, Uw, t, Dt, Tend, c_dt, M, tvb_c, t_limit=1, gamma=1.4, nsteps_max=0):
+    """evolve with the TVB limiter on the finest level; returns
+    (steps, t, Dt, cells limited)."""
+    tt = np.array([t], dtype=np.float64)
+    dd = np.array([Dt], dtype=np.float64)
+    c = np.ascontiguousarray(tvb_c, dtype=np.float64)
+    nl = C.c_long(0)
+    n = lib().orc_evolve_lim(T.h, H.h, _ptrs(Uw), _d(tt), _d(dd), Tend, c_dt, gamma, nsteps_max, M, _d(c),
+                             t_limit, C.byref(nl))
+    return n, float(tt[0]), float(dd[0]), nl.value
+
+
+def l2_nodes(P):
+    """The 1D node set of the L2 norm's 2(P+1)-point rule, in its order."""
+    out = np.zeros(2 * (P + 1))
+    n = lib().orc_l2_nodes(P, _d(out))
+    return out[:n]
 
 
 def totals(T, H, Uw):

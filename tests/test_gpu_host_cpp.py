@@ -11,6 +11,8 @@ from math import comb
 import numpy as np
 import pytest
 
+from bars import mode_err
+
 from layouts import two_level_centre
 
 pytestmark = pytest.mark.gpu
@@ -21,12 +23,9 @@ TOL = 1e-12
 
 
 def rel_err(a, b):
-    top = np.abs(b).max()
-    out = 0.0
-    for q in range(b.shape[0]):
-        den = max(np.abs(b[q]).max(), 1e-6 * top, 1e-300)
-        out = max(out, np.abs(a[q] - b[q]).max() / den)
-    return out
+    """Per-(variable, mode) relative error (tests/bars.py: each mode against
+    its own size, floored at 1 % of its variable group's scale)."""
+    return mode_err(a, b)[0]
 
 
 def run_driver(dim, args, dump):

@@ -1,8 +1,11 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle on the
-same seeded inputs.  Bar: FP64 form within 1e-12 normwise-relative per
-conserved variable (the parity tolerance for U_w modes)."""
+same seeded inputs.  Bar: FP64 form within 1e-12 relative per
+(conserved variable, mode), each mode against its own size (floored at 1 %
+of its variable group's scale, tests/bars.py)."""
 import numpy as np
 import pytest
+
+from bars import mode_err
 
 from layouts import kh_bands, single_level, two_level_centre
 
@@ -13,12 +16,9 @@ VORTEX = [5.0, 5.0, 5.0, 1.0, 0.0]
 
 
 def rel_err(a, b):
-    """max over variables q of max|a-b| / max|b| (arrays [q][...])."""
-    out = 0.0
-    for q in range(b.shape[0]):
-        den = max(np.abs(b[q]).max(), 1e-300)
-        out = max(out, np.abs(a[q] - b[q]).max() / den)
-    return out
+    """Per-(variable, mode) relative error (tests/bars.py: each mode against
+    its own size, floored at 1 % of its variable group's scale)."""
+    return mode_err(a, b)[0]
 
 
 def setup(adg, orc, D, P, levels, dom, plo, phi, ic_case, params, gamma=1.4, kind=0, precision=0):
@@ -297,3 +297,30 @@ def test_kh_outflow_y_three_levels(adg, orc):
     orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=3)
     for l in range(3):
         assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL, f"level {l}"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nodelane_wave_counts_agree_bitwise(adg, precision):
+    """The node-lane predictor with 8 and 16 wavefronts per workgroup gives
+    the same bits (the autotuner's choice between them cannot change a
+    result); 3 steps of the 3-level KH state."""
+    print(f"case node-lane 8 vs 16 waves bitwise, precision {precision}", flush=True)
+    levels = kh_bands(16, 3, 16)
+    out = []
+    for waves in (8, 16):
+        ctx = adg.Context(2, 3, 1.4, precision)
+        ctx.set_predictor_kernel(3)
+        ctx.set_predictor_waves(waves)
+        ctx.set_geometry([16, 16], [0, 0], [1, 1])
+        ctx.set_levels(levels)
+        for l in range(3):
+            ctx.project_ic(l, 1, [0.01, 1e-3, 7])
+        ctx.average_down()
+        ctx.fill_ghosts()
+        ctx.set_dt(0.5, 1.0, 0.0)
+        for _ in range(3):
+            ctx.step(0.5, 1.0)
+        out.append([ctx.download_Uw(l) for l in range(3)])
+        ctx.close()
+    for l in range(3):
+        assert np.array_equal(out[0][l], out[1][l]), f"level {l}"
