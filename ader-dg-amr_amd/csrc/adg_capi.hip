@@ -13,6 +13,7 @@
 #include <exception>
 #include <algorithm>
 #include <array>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -31,8 +32,45 @@ void project_ic_patch(const ElementTables& t, const Hierarchy& H, int p, int ic_
                       std::vector<double>& out);
 }
 
+// Device scratch of adg_conserved_totals_noalloc: the per-workgroup partial
+// sums [Q][blocks] of the totals, kScratchDoubles doubles (320 KiB) on every
+// GPU the library runs on, held until the process ends.  Every workgroup
+// writes its own entries before the second stage reads them, so no call
+// needs it cleared; calls that overlap on one GPU (several streams or
+// contexts) race on it.  Host code never passes the name `scratch` itself to
+// a kernel or copy (on the host it is only a stand-in symbol): it uses the
+// device address that hipGetSymbolAddress returns for the current device,
+// resolved once per device by adg_create.
+constexpr long kScratchDoubles = 5L * 8192;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (nullptr on failure).
+hipError_t resolve_scratch(double** out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
 struct adg_ctx {
     int D = 2, P = 1, Q = 4, precision = 0;
+    double* scratch_dev = nullptr;   // this device's address of `scratch`
     double gamma = 1.4;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -361,6 +399,7 @@ int adg_create(adg_ctx** out, int dim, int p, double gamma, int precision, void*
             c->ncu = ncu;
     }
     if (precision == ADG_FP32_MFMA && !quad_available(c) && !nl_available(c, 8)) { adg_destroy(c); return ADG_E_UNSUP; }
+    if (resolve_scratch(&c->scratch_dev) != hipSuccess || !c->scratch_dev) { adg_destroy(c); return ADG_E_HIP; }
     void* tsp = nullptr;
     if (hipMalloc(&tsp, sizeof(adg::TimeState)) != hipSuccess) { adg_destroy(c); return ADG_E_NOMEM; }
     c->ts = static_cast<adg::TimeState*>(tsp);
@@ -2183,6 +2222,25 @@ int adg_regrid(adg_ctx* c, double time, int* changed) {
     int rc = adg_remake(c, (int)np.size(), np.data(), bx.data());
     if (rc == ADG_OK && changed) *changed = 1;
     return rc;
+}
+
+int adg_conserved_totals_noalloc(adg_ctx* c, double* d_totals) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (!d_totals) return fail(c, ADG_E_ARG, "null device output");
+    const long n = (long)c->H.uncovered.size();
+    const unsigned nb = nblocks(n);
+    if (n == 0) return fail(c, ADG_E_ARG, "no uncovered cells");
+    if ((long)c->Q * (long)nb > kScratchDoubles)
+        return fail(c, ADG_E_ARG, "adg_conserved_totals_noalloc: " + std::to_string(n) +
+                                      " uncovered cells exceed the static scratch (at most " +
+                                      std::to_string(kScratchDoubles / c->Q * 256) + ")");
+    adg::TotArgs a{c->Uw, c->uncovered, c->slot_lev, c->scratch_dev, c->geo, c->H.NS, n, c->tab.Np_s, c->Q};
+    hipLaunchKernelGGL(adg::k_totals, dim3(nb), dim3(256), 0, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(adg::k_totals_finish, dim3(1), dim3(256), 0, c->stream, (const double*)c->scratch_dev,
+                       (int)nb, c->Q, d_totals);
+    HIPCHK(c, hipGetLastError());
+    return ADG_OK;
 }
 
 int adg_lp_error(adg_ctx* c, int q, int p, int ic_case, const double* params, double t, double* out) {

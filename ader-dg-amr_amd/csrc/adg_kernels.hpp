@@ -1973,6 +1973,28 @@ __global__ void __launch_bounds__(256) k_totals(TotArgs a) {
     }
 }
 
+// Second stage of the totals on the device: out[q] = sum over b of
+// partial[q][b], added in the order b = 0, 1, ... (the order of the host
+// loop that reads the same partials back), so both give the same bits.
+// One workgroup of 256 threads: the partials pass through LDS in chunks of
+// 256 per variable, and thread q adds its variable's chunk.
+__global__ void __launch_bounds__(256) k_totals_finish(const double* __restrict__ partial, int nb, int Q,
+                                                       double* __restrict__ out) {
+    __shared__ double chunk[5][256];
+    double acc = 0.0;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+        const int b = b0 + (int)threadIdx.x;
+        for (int q = 0; q < Q; ++q) chunk[q][threadIdx.x] = b < nb ? partial[(long)q * nb + b] : 0.0;
+        __syncthreads();
+        if ((int)threadIdx.x < Q) {
+            const int m = min(256, nb - b0);
+            for (int i = 0; i < m; ++i) acc += chunk[threadIdx.x][i];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < Q) out[threadIdx.x] = acc;
+}
+
 }  // namespace adg
 
 namespace adg {

@@ -30,7 +30,7 @@ EXPORTS = [
     "adg_mlp_dataset", "adg_mlp_dataset_clear", "adg_mlp_fit_normalization", "adg_mlp_train",
     "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats", "adg_lp_error",
     "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
-    "adg_set_limiter", "adg_limit", "adg_limiter_stats",
+    "adg_conserved_totals_noalloc", "adg_set_limiter", "adg_limit", "adg_limiter_stats",
 ]
 
 
@@ -70,6 +70,7 @@ def lib():
     L.adg_set_time.argtypes = [C.c_void_p, C.c_double]
     L.adg_get_time.argtypes = [C.c_void_p, _dp, _dp]
     L.adg_conserved_totals.argtypes = [C.c_void_p, _dp]
+    L.adg_conserved_totals_noalloc.argtypes = [C.c_void_p, C.c_void_p]
     L.adg_set_limiter.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.c_int]
     L.adg_limit.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
     L.adg_limiter_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
@@ -273,6 +274,10 @@ class Context:
         n = C.c_ulonglong(0)
         self._chk(self.L.adg_limiter_stats(self.h, C.byref(n), 1 if reset else 0), "adg_limiter_stats")
         return n.value
+
+    def totals_noalloc(self, d_out):
+        """Conserved totals into device memory at address d_out (Q doubles)."""
+        self._chk(self.L.adg_conserved_totals_noalloc(self.h, C.c_void_p(d_out)), "adg_conserved_totals_noalloc")
 
     def project_ic(self, lev, ic_case, params):
         prm = np.array(params, dtype=np.float64)

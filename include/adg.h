@@ -281,6 +281,17 @@ int adg_get_levels(adg_ctx* ctx, int* nlev, int* npatch, int* boxes, int cap);
  * (the quantity LpNorm_DG_AMR integrates over, AmrDG.h:1399-1578). */
 int adg_conserved_totals(adg_ctx* ctx, double* totals);
 
+/* The same totals, written to DEVICE memory d_totals (Q doubles) on the
+ * context stream, bitwise equal to adg_conserved_totals; no counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): it calls no hipMalloc / hipFree and no host
+ * synchronisation, and keeps its per-workgroup partial sums in a static
+ * device array of 5*8192 doubles per GPU, so it accepts at most
+ * 256*(5*8192/Q) uncovered cells, 2 621 440 in 2D and 2 097 152 in 3D
+ * (ADG_E_ARG beyond that).  Calls that overlap on
+ * one GPU (several streams or contexts at once) race on that array. */
+int adg_conserved_totals_noalloc(adg_ctx* ctx, double* d_totals);
+
 /* Composite Lp error (p = 1 or 2) of conserved variable q over the uncovered
  * valid cells against the exact solution at time t, as LpNorm_DG_AMR
  * (AmrDG.h:1399-1578): (P+1)^D Gauss-Legendre points per cell, normalised
