@@ -31,6 +31,7 @@ EXPORTS = [
     "adg_mlp_loss_grad", "adg_mlp_warm_start", "adg_set_learned", "adg_learned_stats", "adg_lp_error",
     "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
     "adg_conserved_totals_noalloc", "adg_set_limiter", "adg_limit", "adg_limiter_stats",
+    "adg_set_amr", "adg_set_tagging", "adg_tag_cells", "adg_regrid", "adg_remake", "adg_get_levels",
 ]
 
 
@@ -74,6 +75,15 @@ def lib():
     L.adg_set_limiter.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.c_int]
     L.adg_limit.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
     L.adg_limiter_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
+    L.adg_set_amr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
+    L.adg_set_tagging.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _dp]
+    L.adg_tag_cells.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_ubyte)]
+    L.adg_regrid.argtypes = [C.c_void_p, C.c_double, _ip]
+    L.adg_remake.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+    L.adg_get_levels.argtypes = [C.c_void_p, _ip, _ip, _ip, C.c_int]
+    L.adg_host_make_grids.restype = C.c_int
+    L.adg_host_make_grids.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, C.c_int]
     L.adg_sync.argtypes = [C.c_void_p]
     L.adg_stream.argtypes = [C.c_void_p]
     L.adg_stream.restype = C.c_void_p
@@ -120,6 +130,49 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_ip)
+
+
+def _flatten_levels(levels):
+    npatch = np.array([len(l) for l in levels], dtype=np.int32)
+    flat = []
+    for lev in levels:
+        for lo, hi in lev:
+            flat += list(lo) + list(hi)
+    return npatch, np.array(flat, dtype=np.int32)
+
+
+def make_grids(dim, dom0, levels, tags, max_level, periodic=None, blocking_factor=4, max_grid_size=None,
+               grid_eff=0.7, n_error_buf=1, n_proper=1):
+    """New layout from tags on the host (adg_host_make_grids): levels is the
+    old layout, tags[l] an array [n][dim] of tagged cells of level l."""
+    L = lib()
+    per = np.array(periodic if periodic is not None else [1] * dim, dtype=np.int32)
+    npatch, boxes = _flatten_levels(levels)
+    nt = np.array([len(t) for t in tags], dtype=np.int32)
+    tg = np.concatenate([np.asarray(t, dtype=np.int32).reshape(-1, dim) for t in tags]) if len(tags) else \
+        np.zeros((0, dim), dtype=np.int32)
+    tg = np.ascontiguousarray(tg, dtype=np.int32)
+    if tg.size == 0:
+        tg = np.zeros(dim, dtype=np.int32)
+    mgs = max_grid_size or (128 if dim == 2 else 32)
+    nl = C.c_int(0)
+    npn = np.zeros(16, dtype=np.int32)
+    args = (dim, _i(np.array(dom0, dtype=np.int32)), _i(per), max_level, blocking_factor, mgs, grid_eff,
+            n_error_buf, n_proper, len(levels), _i(npatch), _i(boxes), _i(nt), _i(tg), C.byref(nl), _i(npn))
+    n = L.adg_host_make_grids(*args, None, 0)
+    if n < 0:
+        raise AdgError(f"adg_host_make_grids failed rc={n}")
+    out_b = np.zeros(max(n, 1) * 2 * dim, dtype=np.int32)
+    L.adg_host_make_grids(*args, _i(out_b), n)
+    out, k = [], 0
+    for l in range(nl.value):
+        lev = []
+        for _ in range(npn[l]):
+            b = out_b[k * 2 * dim:(k + 1) * 2 * dim]
+            lev.append((tuple(int(v) for v in b[:dim]), tuple(int(v) for v in b[dim:])))
+            k += 1
+        out.append(lev)
+    return out
 
 
 class Context:
@@ -263,6 +316,54 @@ class Context:
         """TVB limiter on the finest level in every step (on) or none."""
         c = np.ascontiguousarray(tvb_c, dtype=np.float64)
         self._chk(self.L.adg_set_limiter(self.h, 1 if on else 0, M, c.size, _d(c), t_limit), "adg_set_limiter")
+
+    def set_amr(self, max_level, blocking_factor=4, max_grid_size=None, grid_eff=0.7, n_error_buf=1, n_proper=1):
+        mgs = max_grid_size or (128 if self.dim == 2 else 32)
+        self._chk(self.L.adg_set_amr(self.h, max_level, blocking_factor, mgs, grid_eff, n_error_buf, n_proper),
+                  "adg_set_amr")
+
+    def set_tagging(self, criterion, amr_c, params=()):
+        c = np.ascontiguousarray(amr_c, dtype=np.float64)
+        prm = np.ascontiguousarray(params, dtype=np.float64) if len(params) else np.zeros(1)
+        self._chk(self.L.adg_set_tagging(self.h, criterion, c.size, _d(c), len(params), _d(prm)), "adg_set_tagging")
+
+    def tag_cells(self, lev):
+        n = self.nvalid(lev)
+        out = np.zeros(n, dtype=np.uint8)
+        self._chk(self.L.adg_tag_cells(self.h, lev, out.ctypes.data_as(C.POINTER(C.c_ubyte))), "adg_tag_cells")
+        return out
+
+    def regrid(self, time=0.0):
+        ch = C.c_int(0)
+        self._chk(self.L.adg_regrid(self.h, time, C.byref(ch)), "adg_regrid")
+        return bool(ch.value)
+
+    def remake(self, levels):
+        npatch, boxes = _flatten_levels(levels)
+        self._chk(self.L.adg_remake(self.h, len(levels), _i(npatch), _i(boxes)), "adg_remake")
+        self.levels = [list(l) for l in levels]
+
+    def get_levels(self):
+        nl = C.c_int(0)
+        npatch = np.zeros(16, dtype=np.int32)
+        n = self.L.adg_get_levels(self.h, C.byref(nl), _i(npatch), None, 0)
+        if n < 0:
+            raise AdgError(f"adg_get_levels failed rc={n}: {self.L.adg_last_error(self.h).decode()}")
+        boxes = np.zeros(max(n, 1) * 2 * self.dim, dtype=np.int32)
+        self.L.adg_get_levels(self.h, C.byref(nl), _i(npatch), _i(boxes), n)
+        out, k = [], 0
+        for l in range(nl.value):
+            lev = []
+            for _ in range(npatch[l]):
+                b = boxes[k * 2 * self.dim:(k + 1) * 2 * self.dim]
+                lev.append((tuple(int(v) for v in b[:self.dim]), tuple(int(v) for v in b[self.dim:])))
+                k += 1
+            out.append(lev)
+        self.levels = out
+        return out
+
+    def nvalid(self, lev):
+        return sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(self.dim)])) for lo, hi in self.get_levels()[lev])
 
     def limit(self):
         """Limiter_w on the finest level now; returns the cells limited."""

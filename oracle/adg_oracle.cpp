@@ -1904,6 +1904,165 @@ extern "C" long orc_limiter_tvb(const orc_tables* t, const orc_hier* h, double**
     return count;
 }
 
+// AMR_tag_cell_refinement (AmrDG_TagCellRef.tpp:4-52): FillBoundary, then
+// pde_tag_cell_refinement per valid cell (Compressible_Euler.h:605-682);
+// one byte per valid cell of level l, patch by patch, x fastest.
+//   crit 0 isentropic_vortex: r <= amr_c^2 about (prm[0], prm[1]), periodic distance
+//   crit 1 kelvin_helmolz_instability: minmodB's troubled test on every
+//          characteristic slope, threshold tvb_c * amr_c * h^2
+//   crit 2 density gradient (the build's own criterion, no reference
+//          counterpart): a face neighbour's cell-mean density differs by more
+//          than amr_c times the cell's own
+extern "C" long orc_tag_cells(const orc_tables* t, const orc_hier* h, double** Uw, int l, int crit, double amr_c,
+                              double tvb_c, const double* prm, double gamma, unsigned char* out) {
+    const int D = t->D, Q = t->Q, Ns = t->Np_s;
+    const OLevel& lv = h->L[l];
+    const long ns = lv.nslot;
+    orc_fill_boundary(t, h, Uw, l);
+    double hmin = lv.dx[0];
+    for (int d = 1; d < D; ++d) hmin = std::min(hmin, lv.dx[d]);
+    int lmi[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d)
+        for (int n = 0; n < Ns; ++n) {
+            bool lin = t->mi_s[n][d] == 1, others = true;
+            for (int dd = 0; dd < D; ++dd)
+                if (dd != d && t->mi_s[n][dd] != 0) others = false;
+            if (lin && others) { lmi[d] = n; break; }
+        }
+    auto U = [&](int q, int n, long s) { return Uw[l][((long)q * Ns + n) * ns + s]; };
+    long k = 0, ntag = 0;
+    for (int b = 0; b < (int)lv.p.size(); ++b) {
+        const OPatch& p = lv.p[b];
+        int iv[3] = {0, 0, 0};
+        for (iv[2] = (D > 2 ? p.lo[2] : 0); iv[2] <= (D > 2 ? p.hi[2] : 0); ++iv[2])
+            for (iv[1] = p.lo[1]; iv[1] <= p.hi[1]; ++iv[1])
+                for (iv[0] = p.lo[0]; iv[0] <= p.hi[0]; ++iv[0]) {
+                    const long s = h->slot_of(l, b, iv);
+                    bool tag = false;
+                    if (crit == 0) {
+                        const double Lx = h->phi[0] - h->plo[0], Ly = h->phi[1] - h->plo[1];
+                        const double x = h->plo[0] + (iv[0] + 0.5) * lv.dx[0];
+                        const double y = h->plo[1] + (iv[1] + 0.5) * lv.dx[1];
+                        double dxv = x - prm[0], dyv = y - prm[1];
+                        if (dxv > Lx / 2.0) dxv -= Lx;
+                        if (dxv < -Lx / 2.0) dxv += Lx;
+                        if (dyv > Ly / 2.0) dyv -= Ly;
+                        if (dyv < -Ly / 2.0) dyv += Ly;
+                        const double r = std::sqrt(dxv * dxv + dyv * dyv);
+                        tag = r <= std::pow(amr_c, 2);
+                    } else if (crit == 1) {
+                        double u0[5];
+                        for (int q = 0; q < Q; ++q) u0[q] = U(q, 0, s);
+                        for (int d = 0; d < D && !tag; ++d) {
+                            int ivm[3] = {iv[0], iv[1], iv[2]}, ivp[3] = {iv[0], iv[1], iv[2]};
+                            ivm[d] -= 1;
+                            ivp[d] += 1;
+                            const long sm = h->slot_of(l, b, ivm), sp = h->slot_of(l, b, ivp);
+                            double Lm[5][5], Rm[5][5];
+                            ev_left(D, d, u0, gamma, Lm);
+                            (void)Rm;
+                            for (int q = 0; q < Q && !tag; ++q) {
+                                double Du = 0.0, Dm = 0.0, Dp = 0.0;
+                                for (int r = 0; r < Q; ++r) {
+                                    Du += Lm[q][r] * U(r, lmi[d], s);
+                                    Dm += 0.5 * Lm[q][r] * (U(r, 0, s) - U(r, 0, sm));
+                                    Dp += 0.5 * Lm[q][r] * (U(r, 0, sp) - U(r, 0, s));
+                                }
+                                bool troubled = false;
+                                minmodB_(Du, Dm, Dp, tvb_c, amr_c, hmin, &troubled);
+                                if (troubled) tag = true;
+                            }
+                        }
+                    } else {
+                        const double r0 = U(0, 0, s);
+                        for (int d = 0; d < D; ++d) {
+                            int ivm[3] = {iv[0], iv[1], iv[2]}, ivp[3] = {iv[0], iv[1], iv[2]};
+                            ivm[d] -= 1;
+                            ivp[d] += 1;
+                            const double j = std::max(std::fabs(U(0, 0, h->slot_of(l, b, ivp)) - r0),
+                                                      std::fabs(r0 - U(0, 0, h->slot_of(l, b, ivm))));
+                            if (j > amr_c * r0) tag = true;
+                        }
+                    }
+                    out[k++] = tag ? 1 : 0;
+                    ntag += tag;
+                }
+    }
+    return ntag;
+}
+
+// RemakeLevel / MakeNewLevelFromCoarse (AmrDG_Level.tpp:52-80,120-170) for a
+// new layout: level 0 keeps its boxes and is copied; every slot (valid and
+// ghost) of a new level l >= 1 takes, through FillPatchTwoLevels /
+// InterpFromCoarseLevel with L2ProjInterp (AmrDG_Level.tpp:126-224,
+// AmrDG_Interpolater.tpp:251-281,344-364), the old level-l valid cell at its
+// (periodic) place, else the projection of its parent in the new level l-1;
+// outside a non-periodic boundary the mirror image of the cell inside.
+// Levels are remade coarse to fine.  Returns 0, or -1 when a new cell has
+// no parent (a layout that does not nest).
+extern "C" int orc_remake(const orc_tables* t, const orc_hier* ho, double** Uo, const orc_hier* hn, double** Un) {
+    const int D = t->D, Q = t->Q, Ns = t->Np_s;
+    if (ho->L[0].nslot != hn->L[0].nslot) return -1;
+    {
+        const long ns = hn->L[0].nslot;
+        for (long i = 0; i < (long)Q * Ns * ns; ++i) Un[0][i] = Uo[0][i];
+    }
+    for (int l = 1; l < hn->nlev; ++l) {
+        const OLevel& lv = hn->L[l];
+        const long ns = lv.nslot, cns = hn->L[l - 1].nslot;
+        const bool had = l < ho->nlev;
+        const long ons = had ? ho->L[l].nslot : 0;
+        for (long s = 0; s < ns; ++s) {
+            int b, iv[3];
+            hn->slot_decode(l, s, &b, iv);
+            int mv[3] = {iv[0], iv[1], iv[2]}, mask = 0;
+            for (int d = 0; d < D; ++d) {
+                const int n = lv.dom[d];
+                if (hn->per[d]) continue;
+                if (iv[d] < 0) { mv[d] = -1 - iv[d]; mask |= 1 << d; }
+                if (iv[d] >= n) { mv[d] = 2 * n - 1 - iv[d]; mask |= 1 << d; }
+            }
+            const long os = had ? ho->find_valid(l, mv) : -1;
+            double v[5][64];
+            if (os >= 0) {
+                for (int q = 0; q < Q; ++q)
+                    for (int n = 0; n < Ns; ++n) v[q][n] = Uo[l][((long)q * Ns + n) * ons + os];
+            } else {
+                int civ[3] = {0, 0, 0}, child = 0;
+                for (int d = 0; d < D; ++d) {
+                    const int c = (mv[d] >= 0) ? mv[d] / 2 : -((-mv[d] + 1) / 2);
+                    civ[d] = c;
+                    child += (mv[d] - c * 2) * (1 << (D - 1 - d));
+                }
+                const long cs = hn->find_valid(l - 1, civ);
+                if (cs < 0) return -1;
+                const Mat& Pc = t->P_cf[child];
+                for (int q = 0; q < Q; ++q) {
+                    double tmp[64];
+                    for (int j = 0; j < Ns; ++j) {
+                        double a = 0.0;
+                        for (int k = 0; k < Ns; ++k) a += Pc(j, k) * Un[l - 1][((long)q * Ns + k) * cns + cs];
+                        tmp[j] = a;
+                    }
+                    for (int n = 0; n < Ns; ++n) {
+                        double a = 0.0;
+                        for (int k = 0; k < Ns; ++k) a += t->Minv(n, k) * tmp[k];
+                        v[q][n] = a;
+                    }
+                }
+            }
+            for (int q = 0; q < Q; ++q)
+                for (int n = 0; n < Ns; ++n) {
+                    int par = 0;
+                    for (int d = 0; d < D; ++d)
+                        if (mask & (1 << d)) par += t->mi_s[n][d];
+                    Un[l][((long)q * Ns + n) * ns + s] = (par & 1) ? -v[q][n] : v[q][n];
+                }
+        }
+    }
+    return 0;
+}
+
 extern "C" void orc_ev_matrices(int D, int d, const double* u, double gamma, double* Lout, double* Rout) {
     // the characteristic matrices at state u, row-major Q x Q (for tests)
     double Lm[5][5], Rm[5][5];

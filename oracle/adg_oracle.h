@@ -130,6 +130,18 @@ int         orc_evolve_lim(const orc_tables*, const orc_hier*, double** Uw,
                            double* t_io, double* Dt_io, double T, double c_dt,
                            double gamma, int nsteps_max, double M, const double* tvb_c,
                            int t_limit, long* n_limited);
+/* Refinement tags of level lev (AMR_tag_cell_refinement + the model's
+ * pde_tag_cell_refinement): crit 0 vortex geometry (prm = centre), 1 the TVB
+ * troubled-cell indicator (threshold tvb_c * amr_c * min(dx)^2), 2 density
+ * gradient (relative cell-mean jump above amr_c).  One byte per valid cell,
+ * patch by patch, x fastest; returns the number tagged. */
+long        orc_tag_cells(const orc_tables*, const orc_hier*, double** Uw, int lev, int crit,
+                          double amr_c, double tvb_c, const double* prm, double gamma, unsigned char* out);
+/* RemakeLevel / MakeNewLevelFromCoarse onto the layout of hn (level 0 equal):
+ * old valid data where the old level had it, else the L2 projection of the
+ * new coarse parent; 0, or -1 when the layout does not nest. */
+int         orc_remake(const orc_tables*, const orc_hier* h_old, double** Uw_old,
+                       const orc_hier* h_new, double** Uw_new);
 /* The 1D nodes of the L2 norm's 2(P+1)-point rule (L2Norm_DG_AMR), in the
  * reference's order; returns their count. */
 int         orc_l2_nodes(int P, double* out);

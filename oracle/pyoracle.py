@@ -72,6 +72,11 @@ def lib():
         L.orc_evolve_lim.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), dptr, dptr, C.c_double,
                                      C.c_double, C.c_double, C.c_int, C.c_double, dptr, C.c_int,
                                      C.POINTER(C.c_long)]
+        L.orc_tag_cells.restype = C.c_long
+        L.orc_tag_cells.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), C.c_int, C.c_int, C.c_double,
+                                    C.c_double, dptr, C.c_double, C.POINTER(C.c_ubyte)]
+        L.orc_remake.restype = C.c_int
+        L.orc_remake.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(dptr), C.c_void_p, C.POINTER(dptr)]
         L.orc_l2_nodes.restype = C.c_int
         L.orc_l2_nodes.argtypes = [C.c_int, dptr]
         fptr = C.POINTER(C.c_float)
@@ -254,6 +259,26 @@ def evolve_lim(T, H, Uw, t, Dt, Tend, c_dt, M, tvb_c, t_limit=1, gamma=1.4, nste
     n = lib().orc_evolve_lim(T.h, H.h, _ptrs(Uw), _d(tt), _d(dd), Tend, c_dt, gamma, nsteps_max, M, _d(c),
                              t_limit, C.byref(nl))
     return n, float(tt[0]), float(dd[0]), nl.value
+
+
+def tag_cells(T, H, Uw, lev, crit, amr_c, tvb_c=1.0, params=(5.0, 5.0), gamma=1.4):
+    """Refinement tags of level lev, one per valid cell (patch order, x fastest)."""
+    n = int(H.valid_mask(lev).sum())
+    out = np.zeros(n, dtype=np.uint8)
+    prm = np.zeros(4)
+    prm[:len(params)] = params
+    lib().orc_tag_cells(T.h, H.h, _ptrs(Uw), lev, crit, amr_c, tvb_c, _d(prm), gamma,
+                        out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    return out
+
+
+def remake(T, H_old, Uw_old, H_new):
+    """U_w on the layout of H_new from the state on H_old (RemakeLevel)."""
+    Un = alloc_levels(T, H_new)
+    rc = lib().orc_remake(T.h, H_old.h, _ptrs(Uw_old), H_new.h, _ptrs(Un))
+    if rc != 0:
+        raise ValueError("new layout does not nest in the one below it")
+    return Un
 
 
 def l2_nodes(P):

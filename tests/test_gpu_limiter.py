@@ -133,6 +133,7 @@ def test_limited_shock_tube_3d(adg, orc):
     ctx.fill_ghosts()
     _limit_ic_oracle(orc, T, H, Uw, 0.0)
     assert n0 == 12 ** 3
+    ctx.limiter_stats(reset=True)
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.set_dt(0.5, 1.0, 0.0)
     for _ in range(3):

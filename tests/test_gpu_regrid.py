@@ -1,0 +1,157 @@
+"""Tagging-driven regrid on the GPU against the oracle.
+- refinement tags (AMR_tag_cell_refinement): the device's bytes equal the
+  oracle's, for the reference's TVB indicator and the density gradient;
+- remake (RemakeLevel / MakeNewLevelFromCoarse) onto the layout the host
+  grid generation makes from those tags: U_w per (variable, mode) at 1e-12;
+- evolve with regrid every 2 steps from full-domain levels (the
+  reference's InitFromScratch and first-step regrid), 2D KH on 3 levels and
+  the 3D shock tube on 2 levels: each regrid's layout equals what the grid
+  generation makes from the oracle's tags, and the state matches the
+  oracle's, which is remade onto the same layouts."""
+import numpy as np
+import pytest
+
+import pyadg
+from bars import mode_err
+from layouts import kh_bands, single_level, tiles
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+KH = [0.01, 1e-3, 7]
+SHOCK = [0.4, 1.0, 1.0, 0.125, 0.1, 0.5, 0.5, 0.5]
+
+
+def _pair(adg, orc, D, P, levels, n0, ic, prm):
+    T = orc.Tables(P, D)
+    H = orc.Hier(D, levels, [n0] * D, [0] * D, [1] * D)
+    Uw = orc.project_ic(T, H, ic, prm)
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    ctx = adg.Context(D, P, 1.4, 0)
+    ctx.set_geometry([n0] * D, [0] * D, [1] * D)
+    ctx.set_levels(levels)
+    for l in range(len(levels)):
+        ctx.upload_Uw(l, Uw[l])
+    return T, H, Uw, ctx
+
+
+def _valid_cells(level_boxes, D):
+    out = []
+    for lo, hi in level_boxes:
+        for idx in np.ndindex(*[hi[d] - lo[d] + 1 for d in reversed(range(D))]):
+            out.append(tuple(lo[d] + idx[D - 1 - d] for d in range(D)))
+    return out
+
+
+def _tag_lists(tags_by_level, levels, D):
+    out = []
+    for l, tg in enumerate(tags_by_level):
+        cells = _valid_cells(levels[l], D)
+        out.append([cells[i] for i in np.nonzero(tg)[0]])
+    return out
+
+
+@pytest.mark.parametrize("crit,amr_c", [(1, 20.0), (2, 0.05)])
+def test_tags_match_oracle(adg, orc, crit, amr_c):
+    print(f"case tags KH P=3 3 levels after 2 steps, criterion {crit}", flush=True)
+    levels = kh_bands(16, 3, 16)
+    T, H, Uw, ctx = _pair(adg, orc, 2, 3, levels, 16, 1, KH)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    for _ in range(2):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=2)
+    ctx.set_tagging(crit, [amr_c])
+    some = False
+    for l in range(3):
+        g = ctx.tag_cells(l)
+        r = orc.tag_cells(T, H, Uw, l, crit, amr_c)
+        print(f"  level {l}: {int(g.sum())} of {g.size} tagged (oracle {int(r.sum())})", flush=True)
+        assert np.array_equal(g, r), f"level {l}"
+        some = some or 0 < g.sum() < g.size
+    assert some
+
+
+def test_remake_matches_oracle(adg, orc):
+    print("case remake KH P=3 onto the layout of its density-gradient tags", flush=True)
+    levels = kh_bands(16, 3, 16)
+    T, H, Uw, ctx = _pair(adg, orc, 2, 3, levels, 16, 1, KH)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    for _ in range(2):
+        ctx.step(0.5, 1.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 1.0, 0.5, nsteps_max=2)
+    tags = [orc.tag_cells(T, H, Uw, l, 2, 0.05) for l in range(2)]
+    new = pyadg.make_grids(2, [16, 16], levels, _tag_lists(tags, levels, 2), 2)
+    print(f"  new layout: {[len(l) for l in new]} boxes per level", flush=True)
+    assert new != levels and len(new) == 3
+    ctx.remake(new)
+    Hn = orc.Hier(2, new, [16, 16], [0, 0], [1, 1])
+    Un = orc.remake(T, H, Uw, Hn)
+    for l in range(3):
+        assert ctx.nslot(l) == Hn.nslot(l)
+        e, at = mode_err(ctx.download_Uw(l), Un[l])
+        print(f"  level {l}: per-mode error {e:.2e} at {at}", flush=True)
+        assert e < TOL, f"level {l}"
+
+
+def _evolve_with_regrid(adg, orc, D, P, n0, nlev, mg, ic, prm, crit, amr_c, nsteps, every, limiter=None):
+    levels = [tiles(n0 << l, mg, D) for l in range(nlev)]   # full-domain levels
+    T, H, Uw, ctx = _pair(adg, orc, D, P, levels, n0, ic, prm)
+    if limiter is not None:
+        ctx.set_limiter(True, limiter, [1.0] * nlev, 1)
+        ctx.limit()
+        ctx.average_down()
+        ctx.fill_ghosts()
+        orc.limiter_tvb(T, H, Uw, nlev - 1, limiter, [1.0] * nlev)
+        orc.average_down(T, H, Uw)
+        orc.fill_patch(T, H, Uw)
+    ctx.set_amr(nlev - 1)
+    ctx.set_tagging(crit, [amr_c])
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 1.0, 0.0)
+    t = 0.0
+    layouts = []
+    m0 = ctx.totals()
+    for n in range(nsteps):
+        if n % every == 0:
+            cur = ctx.get_levels()
+            max_crse = min(len(cur) - 1, nlev - 2)
+            tags = [orc.tag_cells(T, H, Uw, l, crit, amr_c) for l in range(max_crse + 1)]
+            for l in range(max_crse + 1):
+                assert np.array_equal(ctx.tag_cells(l), tags[l]), f"tags, step {n} level {l}"
+            expect = pyadg.make_grids(D, [n0] * D, cur, _tag_lists(tags, cur, D), nlev - 1)
+            ctx.regrid(t)
+            got = ctx.get_levels()
+            assert got == expect, f"layout at step {n}"
+            layouts.append([len(l) for l in got])
+            Hn = orc.Hier(D, got, [n0] * D, [0] * D, [1] * D)
+            Uw = orc.remake(T, H, Uw, Hn)
+            H = Hn
+        ctx.step(0.5, 1.0)
+        if limiter is None:
+            _, t, Dt = orc.evolve(T, H, Uw, t, Dt, 1.0, 0.5, nsteps_max=1)
+        else:
+            _, t, Dt, _ = orc.evolve_lim(T, H, Uw, t, Dt, 1.0, 0.5, limiter, [1.0] * nlev, 1, nsteps_max=1)
+    print(f"  layouts (boxes per level) at the regrids: {layouts}", flush=True)
+    worst = 0.0
+    for l in range(len(ctx.get_levels())):
+        e, at = mode_err(ctx.download_Uw(l), Uw[l])
+        print(f"  level {l}: per-mode error {e:.2e} at {at}", flush=True)
+        worst = max(worst, e)
+    m1 = ctx.totals()
+    return worst, abs(m1[0] - m0[0]) / abs(m0[0])
+
+
+@pytest.mark.parametrize("crit,amr_c", [(2, 0.05), (1, 20.0)])
+def test_kh_evolve_with_regrid(adg, orc, crit, amr_c):
+    print(f"case KH P=3 evolve with regrid every 2 steps, criterion {crit}", flush=True)
+    e, dm = _evolve_with_regrid(adg, orc, 2, 3, 16, 3, 16, 1, KH, crit, amr_c, 6, 2)
+    assert e < TOL
+
+
+def test_shock_tube_3d_evolve_with_regrid(adg, orc):
+    print("case 3D shock tube P=2 8^3 + 1 level, limiter, regrid every 2 steps", flush=True)
+    e, dm = _evolve_with_regrid(adg, orc, 3, 2, 8, 2, 8, 3, SHOCK, 2, 0.2, 4, 2, limiter=0.0)
+    assert e < TOL
