@@ -13,13 +13,18 @@
 // (include/adg.h); the modal data stay on the GPU between calls.
 //
 // Differences from the reference, by design of this library:
-//  - the hierarchy is static: the fine boxes are given to Mesh::set_levels
-//    (tagging-driven regrid is not part of the library; a regrid request is
-//    an error);
+//  - without Mesh::set_levels the fine levels start as full-domain grids
+//    (InitFromScratch) and the first step regrids them from the tags, as in
+//    the reference; with set_levels they start from the given boxes;
+//  - the refinement criterion of a case is the reference's (vortex geometry,
+//    the Kelvin-Helmholtz TVB indicator) or, for the cases the reference
+//    does not tag (the noisy / multi-mode KH states, the shock tubes), the
+//    library's density-gradient criterion;
 //  - errors are exceptions (AdgError) on this side of the C-ABI, carrying
 //    adg_last_error(); the reference has no error reporting;
-//  - plotfiles, restart and the TVB limiter are not part of the library and
-//    requesting them is an error;
+//  - setLimitInitialState(true) applies the limiter once to the projected
+//    initial state (a discontinuous state projected at p >= 2 has negative
+//    nodal density and pressure, and the first step diverges without it);
 //  - the space dimension is the compile-time ADG_SPACEDIM (default 2), as
 //    AMREX_SPACEDIM is in the reference.
 #ifndef ADG_HOST_AMRDG_HPP
@@ -216,19 +221,37 @@ class Mesh {
         for (int l = 0; l < max_level; ++l)
             for (int d = 0; d < SPACEDIM; ++d)
                 if (ref_ratios[l][d] != 2) throw AdgError("Mesh: refinement ratio must be 2");
-        if (dtn_regrid > 0 || dt_regrid > 0)
-            throw AdgError("Mesh: tagging-driven regrid is not part of this library; give the fine boxes to "
-                           "Mesh::set_levels and pass dtn_regrid = dt_regrid = -1");
-        std::array<int, 3> lo{{0, 0, 0}}, hi{{n_cell[0] - 1, n_cell[1] - 1, SPACEDIM == 3 ? n_cell[2] - 1 : 0}};
-        levels_.push_back(tile_boxes(lo, hi, max_grid_size));
+        if (max_level > 0 && (int)amr_c.size() < max_level) throw AdgError("Mesh: one amr_c per coarse level");
+        this->dtn_regrid = dtn_regrid;
+        this->dt_regrid = dt_regrid;
+        max_grid_size_ = max_grid_size;
+        // AmrCore::InitFromScratch: every level 0..max_level as a full-domain
+        // grid (Solver.h:566-569); the first step's regrid replaces the fine ones
+        for (int l = 0; l <= max_level; ++l) {
+            std::array<int, 3> lo{{0, 0, 0}};
+            std::array<int, 3> hi{{(n_cell[0] << l) - 1, (n_cell[1] << l) - 1,
+                                   SPACEDIM == 3 ? (n_cell[2] << l) - 1 : 0}};
+            levels_.push_back(tile_boxes(lo, hi, max_grid_size));
+        }
+        if (max_grid_size % 4) throw AdgError("Mesh: max_grid_size must be a multiple of the blocking factor 4");
     }
 
-    // Fine levels 1..max_level, each a list of boxes in its own index space.
+    // Regrid step interval and physical-time interval (Core/Mesh.h:107-111).
+    int dtn_regrid = 0;
+    double dt_regrid = 0;
+    int get_max_grid_size() const { return max_grid_size_; }
+    // The layout after a regrid (boxes per level, level 0 unchanged).
+    void set_layout(const std::vector<std::vector<Box>>& levels) { levels_ = levels; }
+
+    // Fine levels 1..max_level, each a list of boxes in its own index space;
+    // a hierarchy given this way stays as given (no regrid).
     void set_levels(const std::vector<std::vector<Box>>& fine) {
         if ((int)fine.size() != max_level_) throw AdgError("Mesh::set_levels: one box list per fine level");
         levels_.resize(1);
         for (const auto& l : fine) levels_.push_back(l);
+        static_ = true;
     }
+    bool is_static() const { return static_; }
     // Level 1 over the central half of the domain (the layout of the
     // reference's vortex run after its first regrid).
     void set_centre_level(int max_grid_size = 32) {
@@ -241,6 +264,7 @@ class Mesh {
         }
         levels_.resize(1);
         levels_.push_back(tile_boxes(lo, hi, max_grid_size));
+        static_ = true;
     }
 
     int get_finest_lev() const { return (int)levels_.size() - 1; }
@@ -258,6 +282,8 @@ class Mesh {
     std::array<int, 3> is_periodic_;
     std::vector<double> amr_c_;
     std::vector<std::vector<Box>> levels_;
+    int max_grid_size_ = 32;
+    bool static_ = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -352,16 +378,23 @@ class AmrDG : public Solver<AmrDG<P>> {
     AmrDG(const AmrDG&) = delete;
     AmrDG& operator=(const AmrDG&) = delete;
 
-    // AmrDG.h:407-410.  The TVB limiter is not part of this library.
+    // AmrDG.h:407-410 (AmrDG.tpp:50-58): final time, CFL factor and the
+    // limiter ("TVB" with M, AMR_TVB_C per level and t_limit; "" or
+    // t_limit <= 0: none).
     void settings(double T_, double c_dt_, const std::string& limiter_type = "", double TVB_M = 0.0,
                   const std::vector<double>& AMR_TVB_C = {}, int t_limit = -1) {
-        (void)TVB_M;
-        (void)AMR_TVB_C;
-        if (!limiter_type.empty() && t_limit > 0)
-            throw AdgError("AmrDG::settings: the TVB limiter is not part of this library");
+        if (!limiter_type.empty() && limiter_type != "TVB")
+            throw AdgError("AmrDG::settings: unknown limiter '" + limiter_type + "'");
+        limiter_on_ = limiter_type == "TVB" && t_limit > 0;
+        TVB_M_ = TVB_M;
+        AMR_TVB_C_ = AMR_TVB_C;
+        t_limit_ = t_limit;
         this->T = T_;
         this->c_dt = c_dt_;
     }
+    // Apply the limiter once to the projected initial state (not done by
+    // the reference; needed for discontinuous initial states at p >= 2).
+    void setLimitInitialState(bool on) { limit_ic_ = on; }
 
     // Device forms: ADG_FP64 (plain) or ADG_FP32_MFMA (precision form).
     void setPrecision(int precision) { precision_ = precision; }
@@ -384,8 +417,6 @@ class AmrDG : public Solver<AmrDG<P>> {
         (void)data_dir;
         if (dtn_outplt > 0 || dt_outplt > 0) throw AdgError("AmrDG::init: plotfile output is not part of this library");
         if (restart_tstep > 0) throw AdgError("AmrDG::init: restart is not part of this library");
-        if (!this->flag_analytical_ic)
-            throw AdgError("AmrDG::init: only validation mode (analytic initial state on every level)");
         mesh_ = mesh;
         if (ctx_) {
             adg_destroy(ctx_);
@@ -415,31 +446,89 @@ class AmrDG : public Solver<AmrDG<P>> {
                 for (int d = 0; d < SPACEDIM; ++d) boxes.push_back(b.hi[d]);
             }
         }
-        chk(adg_set_levels(ctx_, nlev, np.data(), boxes.data()), "adg_set_levels");
-        // AmrDG::set_initial_condition (AmrDG.h:782-836) on every level
+        // Solver::set_initial_condition (Core/Solver.h:711-735): the analytic
+        // initial state on level 0; on the finer levels the analytic state
+        // too in validation mode, else the projection of the coarser level
+        // (AMR_interpolate_initial_condition, AmrDG_Level.tpp:82-86); then
+        // average down and FillPatch (AMR_sync_initial_condition)
         int ic = 0;
         std::vector<double> prm;
         model_pde->pde_IC_case(rb, &ic, &prm);
-        for (int l = 0; l < nlev; ++l) chk(adg_project_ic(ctx_, l, ic, prm.data()), "adg_project_ic");
+        if (this->flag_analytical_ic || nlev == 1) {
+            chk(adg_set_levels(ctx_, nlev, np.data(), boxes.data()), "adg_set_levels");
+            for (int l = 0; l < nlev; ++l) chk(adg_project_ic(ctx_, l, ic, prm.data()), "adg_project_ic");
+        } else {
+            chk(adg_set_levels(ctx_, 1, np.data(), boxes.data()), "adg_set_levels");
+            chk(adg_project_ic(ctx_, 0, ic, prm.data()), "adg_project_ic");
+            AMR_FillPatch();
+            chk(adg_remake(ctx_, nlev, np.data(), boxes.data()), "adg_remake");
+        }
         AMR_average_fine_coarse();
         AMR_FillPatch();
+        // the limiter (AmrDG::settings) and the regrid (Mesh) settings
+        {
+            std::vector<double> tvb = AMR_TVB_C_;
+            if (tvb.empty()) tvb.assign((size_t)mesh->get_max_level() + 1, 1.0);
+            chk(adg_set_limiter(ctx_, limiter_on_ ? 1 : 0, TVB_M_, (int)tvb.size(), tvb.data(),
+                                t_limit_ > 0 ? t_limit_ : 1), "adg_set_limiter");
+            if (limit_ic_) {
+                chk(adg_limit(ctx_, nullptr), "adg_limit");
+                AMR_average_fine_coarse();
+                AMR_FillPatch();
+            }
+        }
+        dynamic_ = mesh->get_max_level() > 0 && !mesh->is_static();
+        if (dynamic_) {
+            chk(adg_set_amr(ctx_, mesh->get_max_level(), 4, mesh->get_max_grid_size(), 0.7, 1, 1), "adg_set_amr");
+            const std::string& c = model_pde->model_case;
+            // Compressible_Euler::pde_tag_cell_refinement (Compressible_Euler.h:605-682)
+            const int crit = c == "isentropic_vortex" ? 0 : c == "kelvin_helmolz_instability" ? 1 : 2;
+            const double centre[2] = {5.0, 5.0};
+            const std::vector<double>& ac = mesh->get_amr_c();
+            chk(adg_set_tagging(ctx_, crit, (int)ac.size(), ac.data(), crit == 0 ? 2 : 0, centre), "adg_set_tagging");
+        }
         if (autotune_reps_ > 0) chk(adg_autotune(ctx_, autotune_reps_, nullptr, nullptr), "adg_autotune");
         if (!weights_.empty()) load_network();
     }
 
-    // AmrDG::evolve (AmrDG.h:840-1060) with a static hierarchy.
+    // AmrDG::evolve (AmrDG.h:853-1055): regrid at the first step and at the
+    // regrid points (every dtn_regrid steps or dt_regrid of time,
+    // AmrDG.h:936-986), ADER, the limiter on the finest level, average down,
+    // FillPatch, t += Dt, set_Dt and the clamp to T.  Runs of steps between
+    // regrid points are replayed from the step graph.
     template <typename EquationType, typename BC>
     void evolve(const std::shared_ptr<ModelEquation<EquationType>>& model_pde, const std::shared_ptr<BC>& bdcond) {
         this->n = 0;
         this->t = 0.0;
+        n_regrids_ = 0;
+        double t_last_regrid = 0.0;
+        bool first_step = true;
         if (verbose_) L1Norm_DG_AMR(model_pde);
         set_Dt(model_pde);
+        chk(adg_limiter_stats(ctx_, nullptr, 1), "adg_limiter_stats");
+        const int dtn = mesh_->dtn_regrid;
+        const double dtr = mesh_->dt_regrid;
         while (this->t < this->T && (max_steps_ < 0 || this->n < max_steps_)) {
+            if (dynamic_) {
+                const bool due = first_step || (dtn > 0 && this->n % dtn == 0) ||
+                                 (dtr > 0 && this->t - t_last_regrid >= dtr - 1e-12);
+                if (due) {
+                    int changed = 0;
+                    chk(adg_regrid(ctx_, this->t, &changed), "adg_regrid");
+                    if (changed || first_step) ++n_regrids_;
+                    if (changed) sync_mesh_layout();
+                    t_last_regrid = this->t;
+                }
+                first_step = false;
+            }
             int batch = 1;
             if (graph_batch_ > 0 && this->Dt > 0.0) {
-                // stay clear of the final clamped steps, which run one at a time
-                const double room = 0.5 * (this->T - this->t) / this->Dt;
+                // stay clear of the final clamped steps and of the next regrid
+                // point, which run one at a time
+                double room = 0.5 * (this->T - this->t) / this->Dt;
+                if (dynamic_ && dtr > 0) room = std::min(room, 0.5 * (t_last_regrid + dtr - this->t) / this->Dt);
                 batch = (int)std::min<double>(graph_batch_, std::floor(room));
+                if (dynamic_ && dtn > 0) batch = std::min(batch, dtn - this->n % dtn);
                 if (max_steps_ >= 0) batch = std::min(batch, max_steps_ - this->n);
             }
             if (batch >= 2) {
@@ -447,6 +536,7 @@ class AmrDG : public Solver<AmrDG<P>> {
                 this->n += batch;
             } else {
                 time_integration(model_pde, bdcond, this->t);
+                if (limiter_on_ && this->n % t_limit_ == 0) chk(adg_limit(ctx_, nullptr), "adg_limit");
                 AMR_average_fine_coarse();
                 AMR_FillPatch();
                 chk(adg_advance(ctx_, this->c_dt, this->T, nullptr), "adg_advance");   // t += Dt; set_Dt
@@ -456,10 +546,20 @@ class AmrDG : public Solver<AmrDG<P>> {
         }
         if (verbose_) {
             std::printf("Total number of time steps: %d\n", this->n);
+            std::printf("Total number of regrids: %d\n", n_regrids_);
+            std::printf("Total number of cells limited: %llu\n", cells_limited());
             L1Norm_DG_AMR(model_pde);
             L2Norm_DG_AMR(model_pde);
         }
     }
+
+    // Cells limited since the start of evolve (n_limited_total, AmrDG.h:864).
+    unsigned long long cells_limited() {
+        unsigned long long nl = 0;
+        chk(adg_limiter_stats(ctx_, &nl, 0), "adg_limiter_stats");
+        return nl;
+    }
+    int get_n_regrids() const { return n_regrids_; }
 
     template <typename EquationType, typename BC>
     void time_integration(const std::shared_ptr<ModelEquation<EquationType>>& model_pde,
@@ -541,6 +641,26 @@ class AmrDG : public Solver<AmrDG<P>> {
     adg_ctx* get_ctx() const { return ctx_; }
 
   private:
+    // The Mesh's boxes after a regrid, from the library's layout.
+    void sync_mesh_layout() {
+        int nl = 0, np[16] = {0};
+        const int nb = adg_get_levels(ctx_, &nl, np, nullptr, 0);
+        if (nb < 0) chk(nb, "adg_get_levels");
+        std::vector<int> bx((size_t)nb * 2 * SPACEDIM);
+        chk(adg_get_levels(ctx_, &nl, np, bx.data(), nb) < 0 ? ADG_E_STATE : ADG_OK, "adg_get_levels");
+        std::vector<std::vector<Box>> lv((size_t)nl);
+        int k = 0;
+        for (int l = 0; l < nl; ++l)
+            for (int i = 0; i < np[l]; ++i, ++k) {
+                Box b;
+                for (int d = 0; d < SPACEDIM; ++d) {
+                    b.lo[d] = bx[(size_t)k * 2 * SPACEDIM + d];
+                    b.hi[d] = bx[(size_t)k * 2 * SPACEDIM + SPACEDIM + d];
+                }
+                lv[(size_t)l].push_back(b);
+            }
+        mesh_->set_layout(lv);
+    }
     void chk(int rc, const char* what) const {
         if (rc != ADG_OK)
             throw AdgError(std::string(what) + " failed (" + std::to_string(rc) + "): " +
@@ -568,6 +688,11 @@ class AmrDG : public Solver<AmrDG<P>> {
     int autotune_reps_ = 0;
     int max_steps_ = -1;
     bool verbose_ = true;
+    bool limiter_on_ = false, limit_ic_ = false, dynamic_ = false;
+    double TVB_M_ = 0.0;
+    std::vector<double> AMR_TVB_C_;
+    int t_limit_ = -1;
+    int n_regrids_ = 0;
 };
 
 // ---------------------------------------------------------------------------

@@ -2,13 +2,20 @@
 // (settings block, case overrides, Simulation<AmrDG<p>,Compressible_Euler>
 // construction, run), over the C++ host side in AmrDG.hpp.
 //
-// Usage: adg_main_2d [--case NAME] [--p 1|2|3] [--n CELLS] [--max_level 0|1]
+// Usage: adg_main_2d [--case NAME] [--p 1|2|3] [--n CELLS] [--max_level L]
 //                    [--T T] [--c_dt C] [--precision fp64|fp32]
 //                    [--weights FILE --tol TOL --max_iters K]
+//                    [--regrid] [--dtn_regrid N] [--dt_regrid T] [--amr_c C]
+//                    [--limiter TVB --tvb_m M --t_limit N] [--limit_ic]
+//                    [--projection_ic]
 //                    [--graph_steps K] [--max_steps N] [--dump FILE] [--quiet]
+// Without --regrid a fine level is the static centre half of the domain; with
+// --regrid the levels start full-domain and follow the tags at the case's
+// regrid points (the reference's settings for the vortex and its KH case).
 // Prints the reference's norm lines and, last, one JSON summary line.
 // --dump writes int32 nlev, then per level int32 nslot and the FP64 U_w
-// [q][n][slot] of that level.
+// [q][n][slot] of that level, then per level int32 npatch and the boxes
+// (int32 lo[dim], hi[dim] each).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,14 +32,20 @@ struct Options {
     double T = -1.0, c_dt = -1.0, tol = 1e-4;
     std::string precision = "fp64", weights, dump;
     bool quiet = false;
+    bool regrid = false, limit_ic = false, projection_ic = false;
+    int dtn_regrid = -2, t_limit = -2;
+    double dt_regrid = -2.0, amr_c = -1.0, tvb_m = -1.0;
+    std::string limiter = "-";
 };
 
 void usage() {
     std::printf(
         "usage: adg_main [--case isentropic_vortex|kelvin_helmolz_instability|kelvin_helmholtz_noise|\n"
         "                 kelvin_helmholtz_multimode|radial_shock_tube|spherical_shock_tube|density_wave]\n"
-        "                [--p 1|2|3] [--n CELLS] [--max_level 0|1] [--T T] [--c_dt C]\n"
+        "                [--p 1|2|3] [--n CELLS] [--max_level L] [--T T] [--c_dt C]\n"
         "                [--precision fp64|fp32] [--weights FILE --tol TOL --max_iters K]\n"
+        "                [--regrid] [--dtn_regrid N] [--dt_regrid T] [--amr_c C]\n"
+        "                [--limiter TVB --tvb_m M --t_limit N] [--limit_ic] [--projection_ic]\n"
         "                [--graph_steps K] [--max_steps N] [--dump FILE] [--quiet]\n"
         "space dimension: %d\n",
         adg::SPACEDIM);
@@ -60,6 +73,15 @@ bool parse(int argc, char** argv, Options* o) {
         else if (a == "--max_steps") o->max_steps = std::atoi(next());
         else if (a == "--dump") o->dump = next();
         else if (a == "--quiet") o->quiet = true;
+        else if (a == "--regrid") o->regrid = true;
+        else if (a == "--dtn_regrid") o->dtn_regrid = std::atoi(next());
+        else if (a == "--dt_regrid") o->dt_regrid = std::atof(next());
+        else if (a == "--amr_c") o->amr_c = std::atof(next());
+        else if (a == "--limiter") o->limiter = next();
+        else if (a == "--tvb_m") o->tvb_m = std::atof(next());
+        else if (a == "--t_limit") o->t_limit = std::atoi(next());
+        else if (a == "--limit_ic") o->limit_ic = true;
+        else if (a == "--projection_ic") o->projection_ic = true;
         else throw adg::AdgError("unknown option " + a);
     }
     return true;
@@ -96,11 +118,39 @@ int run(const Options& opt) {
         T = 1.0; c_dt = 0.5; max_level = 1;
         L_lo = 0.0; L_hi = 1.0; n_cell_x = n_cell_y = n_cell_z = 16;
     }
+    // tagging-driven regrid: the reference's settings of its two cases
+    // (main.cpp:62-124), the library's density-gradient tagging otherwise
+    int dtn_regrid = -1;
+    double dt_regrid = -1.0;
+    std::vector<double> amr_c;
+    bool limit_ic = opt.limit_ic;
+    if (opt.regrid) {
+        if (simulation_case == "isentropic_vortex") {
+            max_level = 1; dtn_regrid = 1; amr_c = {1.4};
+        } else if (simulation_case == "kelvin_helmolz_instability") {
+            max_level = 2; dt_regrid = 0.05; amr_c = {1300.0, 2100.0};
+            limiter_type = "TVB"; TVB_M = 5000.0; t_limit = 1;
+        } else if (simulation_case == "radial_shock_tube" || simulation_case == "spherical_shock_tube") {
+            max_level = 2; dtn_regrid = 4; amr_c = {0.2, 0.2};
+            limiter_type = "TVB"; TVB_M = 0.0; t_limit = 1; limit_ic = true;
+        } else {
+            max_level = 2; dtn_regrid = 8; amr_c = {0.05, 0.05};
+        }
+    }
     if (opt.T > 0) T = opt.T;
     if (opt.c_dt > 0) c_dt = opt.c_dt;
     if (opt.n > 0) n_cell_x = n_cell_y = n_cell_z = opt.n;
     if (opt.max_level >= 0) max_level = opt.max_level;
-    if (max_level > 1) throw AdgError("this driver builds at most one fine level (the centre half)");
+    if (opt.dtn_regrid > -2) dtn_regrid = opt.dtn_regrid;
+    if (opt.dt_regrid > -2.0) dt_regrid = opt.dt_regrid;
+    if (opt.limiter != "-") limiter_type = opt.limiter;
+    if (opt.tvb_m >= 0.0) TVB_M = opt.tvb_m;
+    if (opt.t_limit > -2) t_limit = opt.t_limit;
+    if (!opt.regrid && max_level > 1)
+        throw AdgError("a static layout has at most one fine level (the centre half); use --regrid");
+    if (max_level > 4) throw AdgError("max_level must be at most 4");
+    if (opt.amr_c > 0.0) amr_c.assign((size_t)max_level, opt.amr_c);
+    amr_c.resize((size_t)max_level, amr_c.empty() ? 1.0 : amr_c.back());
 
     std::vector<int> n_cell{n_cell_x, n_cell_y};
     if (SPACEDIM == 3) n_cell.push_back(n_cell_z);
@@ -110,17 +160,19 @@ int run(const Options& opt) {
         domain.hi[d] = L_hi;
     }
     std::vector<std::array<int, 3>> amr_ratio(max_level, std::array<int, 3>{{2, 2, 2}});
-    std::vector<double> amr_c(max_level, 1.0);
+    std::vector<double> amr_tvb_c((size_t)max_level + 1, 1.0);   // main.cpp:116
 
     // ------------------------------ SIMULATION CONSTRUCTION (main.cpp:143-151)
     Simulation<AmrDG<p>, Compressible_Euler> sim;
     sim.setModelSettings(simulation_case);
-    sim.setNumericalSettings(T, c_dt, limiter_type, TVB_M, std::vector<double>{}, t_limit);
-    sim.setValidationMode(true);
+    sim.setNumericalSettings(T, c_dt, limiter_type, TVB_M, amr_tvb_c, t_limit);
+    sim.setValidationMode(!opt.projection_ic);
     sim.setIO(-1, -1.0);
     sim.setBoundaryConditions(bc_type, bc_type);
-    sim.setGeometrySettings(domain, max_level, n_cell, 0, amr_ratio, is_periodic, amr_c, -1, -1.0, 1);
-    if (max_level == 1) sim.mesh->set_centre_level();
+    sim.setGeometrySettings(domain, max_level, n_cell, 0, amr_ratio, is_periodic, amr_c,
+                            opt.regrid ? dtn_regrid : -1, opt.regrid ? dt_regrid : -1.0, 1);
+    if (!opt.regrid && max_level == 1) sim.mesh->set_centre_level();
+    sim.solver->setLimitInitialState(limit_ic);
 
     sim.solver->setPrecision(opt.precision == "fp32" ? ADG_FP32_MFMA : ADG_FP64);
     if (!opt.weights.empty()) sim.solver->setLearnedWarmStart(opt.weights, opt.tol, opt.max_iters);
@@ -145,10 +197,21 @@ int run(const Options& opt) {
             std::fwrite(&ns, sizeof(int), 1, f);
             std::fwrite(u.data(), sizeof(double), u.size(), f);
         }
+        for (int l = 0; l < nlev; ++l) {
+            const auto& ba = sim.mesh->get_BoxArray(l);
+            const int nb = (int)ba.size();
+            std::fwrite(&nb, sizeof(int), 1, f);
+            for (const Box& b : ba) {
+                std::fwrite(b.lo.data(), sizeof(int), SPACEDIM, f);
+                std::fwrite(b.hi.data(), sizeof(int), SPACEDIM, f);
+            }
+        }
         std::fclose(f);
     }
-    std::printf("{\"case\": \"%s\", \"p\": %d, \"dim\": %d, \"steps\": %d, \"t\": %.17g, \"Dt\": %.17g, \"L1\": [",
-                simulation_case.c_str(), p, SPACEDIM, s.get_n(), s.get_t(), s.get_Dt());
+    std::printf("{\"case\": \"%s\", \"p\": %d, \"dim\": %d, \"steps\": %d, \"t\": %.17g, \"Dt\": %.17g, "
+                "\"regrids\": %d, \"limited\": %llu, \"levels\": %d, \"L1\": [",
+                simulation_case.c_str(), p, SPACEDIM, s.get_n(), s.get_t(), s.get_Dt(), s.get_n_regrids(),
+                s.cells_limited(), sim.mesh->get_finest_lev() + 1);
     for (size_t q = 0; q < l1.size(); ++q) std::printf("%s%.17g", q ? ", " : "", l1[q]);
     std::printf("], \"totals\": [");
     for (size_t q = 0; q < tot.size(); ++q) std::printf("%s%.17g", q ? ", " : "", tot[q]);

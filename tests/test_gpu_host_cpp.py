@@ -35,12 +35,17 @@ def run_driver(dim, args, dump):
     assert r.returncode == 0, r.stderr
     summary = json.loads(r.stdout.strip().splitlines()[-1])
     Q, Ns = dim + 2, comb(summary["p"] + dim, dim)
-    levels = []
+    levels, layout = [], []
     with open(dump, "rb") as f:
         nlev = int(np.fromfile(f, dtype=np.int32, count=1)[0])
         for _ in range(nlev):
             ns = int(np.fromfile(f, dtype=np.int32, count=1)[0])
             levels.append(np.fromfile(f, dtype=np.float64, count=Q * Ns * ns).reshape(Q, Ns, ns))
+        for _ in range(nlev):
+            nb = int(np.fromfile(f, dtype=np.int32, count=1)[0])
+            b = np.fromfile(f, dtype=np.int32, count=2 * dim * nb).reshape(nb, 2 * dim)
+            layout.append([(tuple(int(v) for v in r[:dim]), tuple(int(v) for v in r[dim:])) for r in b])
+    summary["layout"] = layout
     return summary, levels
 
 
@@ -122,3 +127,51 @@ def test_cpp_driver_reference_kh(orc, tmp_path):
     e = rel_err(got[0], Uw[0])
     print(f"  relative error {e:.2e}", flush=True)
     assert e < TOL
+
+
+def test_cpp_driver_kh_regrid_limiter(orc, tmp_path):
+    """The noisy KH state at p=3 on 16^2 with two tagged levels through the
+    C++ host (--regrid: full-domain levels, density-gradient tags every 2
+    steps, max_grid_size 32) with the TVB limiter (M = 200) and 2-step graph
+    batches between regrid points, 6 steps, against the oracle remade onto
+    the layouts the host grid generation makes from the oracle's own tags."""
+    import pyadg
+    from layouts import tiles
+    print("case C++ driver KH p=3 16^2 --regrid (2 levels, every 2 steps) + TVB limiter, 6 steps", flush=True)
+    summary, got = run_driver(2, ["--case", "kelvin_helmholtz_noise", "--p", "3", "--n", "16", "--regrid",
+                                  "--dtn_regrid", "2", "--amr_c", "0.05", "--limiter", "TVB", "--tvb_m", "200",
+                                  "--t_limit", "1", "--max_steps", "6", "--graph_steps", "4", "--T", "1.0"],
+                              str(tmp_path / "uw.bin"))
+    T = orc.Tables(3, 2)
+    cur = [tiles(16 << l, 32, 2) for l in range(3)]
+    H = orc.Hier(2, cur, [16, 16], [0, 0], [1, 1])
+    Uw = orc.project_ic(T, H, 1, [0.01, 1e-3, 1])
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    t, nl_tot = 0.0, 0
+    for n in range(6):
+        if n % 2 == 0:
+            tags = []
+            for l in range(min(len(cur) - 1, 1) + 1):
+                tg = orc.tag_cells(T, H, Uw, l, 2, 0.05)
+                cells = [c for lo, hi in cur[l] for c in
+                         ((lo[0] + i, lo[1] + j) for j in range(hi[1] - lo[1] + 1) for i in range(hi[0] - lo[0] + 1))]
+                tags.append([cells[i] for i in np.nonzero(tg)[0]])
+            new = pyadg.make_grids(2, [16, 16], cur, tags, 2, max_grid_size=32)
+            if new != cur:
+                Hn = orc.Hier(2, new, [16, 16], [0, 0], [1, 1])
+                Uw = orc.remake(T, H, Uw, Hn)
+                H, cur = Hn, new
+        _, t, Dt, nl = orc.evolve_lim(T, H, Uw, t, Dt, 1.0, 0.5, 200.0, [1.0] * 3, 1, nsteps_max=1)
+        nl_tot += nl
+    print(f"  driver: {summary['regrids']} regrids, {summary['limited']} cells limited, layout "
+          f"{[len(l) for l in summary['layout']]}; oracle {nl_tot} limited, layout {[len(l) for l in cur]}",
+          flush=True)
+    assert summary["steps"] == 6
+    assert summary["layout"] == cur
+    assert summary["limited"] == nl_tot
+    for l in range(len(cur)):
+        e = rel_err(got[l], Uw[l])
+        print(f"  level {l}: per-mode error {e:.2e}", flush=True)
+        assert e < TOL, f"level {l}"

@@ -29,6 +29,8 @@ def test_driver_help(dim):
     (["--case", "no_such_case"], "unknown case"),
     (["--p", "4"], "p must be 1, 2 or 3"),
     (["--max_level", "2"], "at most one fine level"),
+    (["--regrid", "--max_level", "5"], "max_level must be at most 4"),
+    (["--limiter", "MUSCL"], "unknown limiter"),
     (["--bogus"], "unknown option"),
 ])
 def test_driver_refuses_before_the_device(args, msg):
