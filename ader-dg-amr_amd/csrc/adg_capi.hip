@@ -1791,6 +1791,15 @@ int adg_set_limiter(adg_ctx* c, int type, double M, int ntvb, const double* tvb_
     return ADG_OK;
 }
 
+int adg_set_step_count(adg_ctx* c, int n) {
+    if (!c) return ADG_E_ARG;
+    if (n < 0) return fail(c, ADG_E_ARG, "step count must be >= 0");
+    const unsigned int v = (unsigned int)n;
+    HIPCHK(c, hipMemcpyAsync(&c->ts->nstep, &v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ADG_OK;
+}
+
 int adg_limit(adg_ctx* c, unsigned long long* n_limited) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     unsigned long long before = 0;

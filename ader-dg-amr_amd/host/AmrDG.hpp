@@ -25,6 +25,8 @@
 //  - setLimitInitialState(true) applies the limiter once to the projected
 //    initial state (a discontinuous state projected at p >= 2 has negative
 //    nodal density and pressure, and the first step diverges without it);
+//  - plotfiles go to Simulation::setDataDir when given, else to the
+//    reference's "../Results/Simulation Data/<case>" directory;
 //  - the space dimension is the compile-time ADG_SPACEDIM (default 2), as
 //    AMREX_SPACEDIM is in the reference.
 #ifndef ADG_HOST_AMRDG_HPP
@@ -34,6 +36,8 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <dirent.h>
+#include <unistd.h>
 #include <fstream>
 #include <limits>
 #include <memory>
@@ -42,6 +46,7 @@
 #include <vector>
 
 #include "adg.h"
+#include "plotfile.hpp"
 
 #ifndef ADG_SPACEDIM
 #define ADG_SPACEDIM 2
@@ -414,10 +419,11 @@ class AmrDG : public Solver<AmrDG<P>> {
     template <typename EquationType>
     void init(const std::shared_ptr<ModelEquation<EquationType>>& model_pde, std::shared_ptr<Mesh<AmrDG<P>>> mesh,
               int dtn_outplt, double dt_outplt, const std::string& data_dir, int restart_tstep = 0) {
-        (void)data_dir;
-        if (dtn_outplt > 0 || dt_outplt > 0) throw AdgError("AmrDG::init: plotfile output is not part of this library");
-        if (restart_tstep > 0) throw AdgError("AmrDG::init: restart is not part of this library");
         mesh_ = mesh;
+        dtn_outplt_ = dtn_outplt;
+        dt_outplt_ = dt_outplt;
+        data_dir_ = data_dir;
+        restart_tstep_ = restart_tstep;
         if (ctx_) {
             adg_destroy(ctx_);
             ctx_ = nullptr;
@@ -454,7 +460,11 @@ class AmrDG : public Solver<AmrDG<P>> {
         int ic = 0;
         std::vector<double> prm;
         model_pde->pde_IC_case(rb, &ic, &prm);
-        if (this->flag_analytical_ic || nlev == 1) {
+        if (restart_tstep > 0) {
+            // Solver::init with restart (Core/Solver.h:571-591): the hierarchy
+            // and U_w from the plotfiles of step restart_tstep, then FillPatch
+            restart_time_ = ReadPlotFile(restart_tstep);
+        } else if (this->flag_analytical_ic || nlev == 1) {
             chk(adg_set_levels(ctx_, nlev, np.data(), boxes.data()), "adg_set_levels");
             for (int l = 0; l < nlev; ++l) chk(adg_project_ic(ctx_, l, ic, prm.data()), "adg_project_ic");
         } else {
@@ -463,7 +473,7 @@ class AmrDG : public Solver<AmrDG<P>> {
             AMR_FillPatch();
             chk(adg_remake(ctx_, nlev, np.data(), boxes.data()), "adg_remake");
         }
-        AMR_average_fine_coarse();
+        if (restart_tstep <= 0) AMR_average_fine_coarse();
         AMR_FillPatch();
         // the limiter (AmrDG::settings) and the regrid (Mesh) settings
         {
@@ -471,7 +481,7 @@ class AmrDG : public Solver<AmrDG<P>> {
             if (tvb.empty()) tvb.assign((size_t)mesh->get_max_level() + 1, 1.0);
             chk(adg_set_limiter(ctx_, limiter_on_ ? 1 : 0, TVB_M_, (int)tvb.size(), tvb.data(),
                                 t_limit_ > 0 ? t_limit_ : 1), "adg_set_limiter");
-            if (limit_ic_) {
+            if (limit_ic_ && restart_tstep <= 0) {
                 chk(adg_limit(ctx_, nullptr), "adg_limit");
                 AMR_average_fine_coarse();
                 AMR_FillPatch();
@@ -498,17 +508,29 @@ class AmrDG : public Solver<AmrDG<P>> {
     // regrid points are replayed from the step graph.
     template <typename EquationType, typename BC>
     void evolve(const std::shared_ptr<ModelEquation<EquationType>>& model_pde, const std::shared_ptr<BC>& bdcond) {
-        this->n = 0;
-        this->t = 0.0;
         n_regrids_ = 0;
-        double t_last_regrid = 0.0;
+        double t_last_regrid = 0.0, t_last_plt = 0.0;
         bool first_step = true;
-        if (verbose_) L1Norm_DG_AMR(model_pde);
+        if (restart_tstep_ > 0) {
+            this->n = restart_tstep_;
+            this->t = restart_time_;
+            t_last_plt = this->t;
+            t_last_regrid = this->t;
+        } else {
+            this->n = 0;
+            this->t = 0.0;
+            if (dtn_outplt_ > 0 || dt_outplt_ > 0) PlotFile(this->n, this->t);
+            if (verbose_) {
+                L1Norm_DG_AMR(model_pde);
+                L2Norm_DG_AMR(model_pde);
+            }
+        }
         set_Dt(model_pde);
         chk(adg_limiter_stats(ctx_, nullptr, 1), "adg_limiter_stats");
+        chk(adg_set_step_count(ctx_, this->n), "adg_set_step_count");
         const int dtn = mesh_->dtn_regrid;
         const double dtr = mesh_->dt_regrid;
-        while (this->t < this->T && (max_steps_ < 0 || this->n < max_steps_)) {
+        while (this->t < this->T && (max_steps_ < 0 || this->n < max_steps_ + restart_steps_offset())) {
             if (dynamic_) {
                 const bool due = first_step || (dtn > 0 && this->n % dtn == 0) ||
                                  (dtr > 0 && this->t - t_last_regrid >= dtr - 1e-12);
@@ -529,7 +551,11 @@ class AmrDG : public Solver<AmrDG<P>> {
                 if (dynamic_ && dtr > 0) room = std::min(room, 0.5 * (t_last_regrid + dtr - this->t) / this->Dt);
                 batch = (int)std::min<double>(graph_batch_, std::floor(room));
                 if (dynamic_ && dtn > 0) batch = std::min(batch, dtn - this->n % dtn);
-                if (max_steps_ >= 0) batch = std::min(batch, max_steps_ - this->n);
+                // and of the plot points
+                if (dtn_outplt_ > 0) batch = std::min(batch, dtn_outplt_ - this->n % dtn_outplt_);
+                if (dt_outplt_ > 0)
+                    batch = std::min<int>(batch, (int)std::floor(0.5 * (t_last_plt + dt_outplt_ - this->t) / this->Dt));
+                if (max_steps_ >= 0) batch = std::min(batch, max_steps_ - this->n + restart_steps_offset());
             }
             if (batch >= 2) {
                 chk(adg_step_graph(ctx_, this->c_dt, this->T, batch, nullptr), "adg_step_graph");
@@ -543,6 +569,15 @@ class AmrDG : public Solver<AmrDG<P>> {
                 this->n += 1;
             }
             chk(adg_get_time(ctx_, &this->t, &this->Dt), "adg_get_time");
+            // plot points (AmrDG.h:1018-1023)
+            const bool dtn_plt = dtn_outplt_ > 0 && this->n % dtn_outplt_ == 0;
+            const bool dt_plt = dt_outplt_ > 0 && this->t - t_last_plt >= dt_outplt_ - 1e-12;
+            if (dtn_plt) {
+                PlotFile(this->n, this->t);
+            } else if (dt_plt) {
+                PlotFile(this->n, this->t);
+                t_last_plt = this->t;
+            }
         }
         if (verbose_) {
             std::printf("Total number of time steps: %d\n", this->n);
@@ -551,6 +586,132 @@ class AmrDG : public Solver<AmrDG<P>> {
             L1Norm_DG_AMR(model_pde);
             L2Norm_DG_AMR(model_pde);
         }
+    }
+
+    // PlotFile (Core/Solver.h:922-968): one plotfile per conserved variable,
+    // data_dir/tstep_<n>_q_<q>_plt, components <name>_<mode>, the valid cells
+    // of every level.
+    void PlotFile(int tstep, double time) {
+        static const char* names[5] = {"mass_density", "momentum_x", "momentum_y", "energy_density",
+                                       "angular_momentum_z"};   // Compressible_Euler.h:15-24
+        int sz[7];
+        chk(adg_sizes(ctx_, sz), "adg_sizes");
+        const int Ns = sz[0];
+        const int nlev = mesh_->get_finest_lev() + 1;
+        std::vector<std::vector<double>> U((size_t)nlev);
+        for (int l = 0; l < nlev; ++l) U[(size_t)l] = get_U_w(l);
+        const RealBox& rb = mesh_->get_RealBox();
+        for (int q = 0; q < Q; ++q) {
+            plt::Plotfile pf;
+            pf.dim = SPACEDIM;
+            pf.time = time;
+            pf.step = tstep;
+            for (int m = 0; m < Ns; ++m) pf.names.push_back(std::string(names[q]) + "_" + std::to_string(m));
+            for (int d = 0; d < SPACEDIM; ++d) { pf.prob_lo[d] = rb.lo[d]; pf.prob_hi[d] = rb.hi[d]; }
+            for (int l = 0; l < nlev; ++l) {
+                plt::Level L;
+                for (int d = 0; d < SPACEDIM; ++d) {
+                    const int n = mesh_->get_n_cell()[d] << l;
+                    L.domain_hi[d] = n - 1;
+                    L.dx[d] = (rb.hi[d] - rb.lo[d]) / n;
+                }
+                const std::vector<double>& u = U[(size_t)l];
+                const long ns = (long)u.size() / ((long)Q * Ns);
+                long off = 0;
+                for (const Box& b : mesh_->get_BoxArray(l)) {
+                    plt::IBox ib;
+                    long g[3] = {1, 1, 1};
+                    for (int d = 0; d < SPACEDIM; ++d) {
+                        ib.lo[d] = b.lo[d];
+                        ib.hi[d] = b.hi[d];
+                        g[d] = b.hi[d] - b.lo[d] + 3;
+                    }
+                    const long np = plt::box_cells(ib, SPACEDIM);
+                    std::vector<double> v((size_t)Ns * np);
+                    long k = 0;
+                    for (int z = 0; z < (SPACEDIM == 3 ? b.hi[2] - b.lo[2] + 1 : 1); ++z)
+                        for (int y = 0; y <= b.hi[1] - b.lo[1]; ++y)
+                            for (int x = 0; x <= b.hi[0] - b.lo[0]; ++x, ++k) {
+                                const long s = off + (x + 1) + g[0] * ((y + 1) + g[1] * (SPACEDIM == 3 ? z + 1 : 0));
+                                for (int m = 0; m < Ns; ++m) v[(size_t)m * np + k] = u[((size_t)q * Ns + m) * ns + s];
+                            }
+                    off += g[0] * g[1] * g[2];
+                    L.boxes.push_back(ib);
+                    L.data.push_back(std::move(v));
+                }
+                pf.levels.push_back(std::move(L));
+            }
+            plt::write(data_dir_ + "/tstep_" + std::to_string(tstep) + "_q_" + std::to_string(q) + "_plt", pf);
+        }
+    }
+
+    // ReadPlotFile (Core/Solver.h:599-652): the level layout (at most
+    // max_level) from the q = 0 plotfile of step tstep, U_w of every level
+    // from the plotfile of each q; returns the time stored there.
+    double ReadPlotFile(int tstep) {
+        const auto path = [&](int q) {
+            return data_dir_ + "/tstep_" + std::to_string(tstep) + "_q_" + std::to_string(q) + "_plt";
+        };
+        plt::Plotfile meta;
+        try {
+            meta = plt::read(path(0));
+        } catch (const std::exception& e) {
+            throw AdgError(std::string("AmrDG::ReadPlotFile: ") + e.what());
+        }
+        const int nlev = std::min((int)meta.levels.size(), mesh_->get_max_level() + 1);
+        std::vector<std::vector<Box>> lv((size_t)nlev);
+        std::vector<int> np, boxes;
+        for (int l = 0; l < nlev; ++l) {
+            for (const plt::IBox& ib : meta.levels[(size_t)l].boxes) {
+                Box b;
+                for (int d = 0; d < SPACEDIM; ++d) { b.lo[d] = ib.lo[d]; b.hi[d] = ib.hi[d]; }
+                lv[(size_t)l].push_back(b);
+                for (int d = 0; d < SPACEDIM; ++d) boxes.push_back(b.lo[d]);
+                for (int d = 0; d < SPACEDIM; ++d) boxes.push_back(b.hi[d]);
+            }
+            np.push_back((int)lv[(size_t)l].size());
+        }
+        mesh_->set_layout(lv);
+        chk(adg_set_levels(ctx_, nlev, np.data(), boxes.data()), "adg_set_levels");
+        int sz[7];
+        chk(adg_sizes(ctx_, sz), "adg_sizes");
+        const int Ns = sz[0];
+        std::vector<std::vector<double>> U((size_t)nlev);
+        for (int l = 0; l < nlev; ++l) {
+            const int ns = adg_level_nslot(ctx_, l);
+            U[(size_t)l].assign((size_t)Q * Ns * ns, 0.0);
+        }
+        for (int q = 0; q < Q; ++q) {
+            plt::Plotfile pf;
+            try {
+                pf = q == 0 ? meta : plt::read(path(q));
+            } catch (const std::exception& e) {
+                throw AdgError(std::string("AmrDG::ReadPlotFile: ") + e.what());
+            }
+            for (int l = 0; l < nlev; ++l) {
+                std::vector<double>& u = U[(size_t)l];
+                const long ns = (long)u.size() / ((long)Q * Ns);
+                long off = 0;
+                const plt::Level& L = pf.levels[(size_t)l];
+                for (size_t bi = 0; bi < L.boxes.size(); ++bi) {
+                    const plt::IBox& b = L.boxes[bi];
+                    long g[3] = {1, 1, 1};
+                    for (int d = 0; d < SPACEDIM; ++d) g[d] = b.hi[d] - b.lo[d] + 3;
+                    const long npt = plt::box_cells(b, SPACEDIM);
+                    const std::vector<double>& v = L.data[bi];
+                    long k = 0;
+                    for (int z = 0; z < (SPACEDIM == 3 ? b.hi[2] - b.lo[2] + 1 : 1); ++z)
+                        for (int y = 0; y <= b.hi[1] - b.lo[1]; ++y)
+                            for (int x = 0; x <= b.hi[0] - b.lo[0]; ++x, ++k) {
+                                const long s = off + (x + 1) + g[0] * ((y + 1) + g[1] * (SPACEDIM == 3 ? z + 1 : 0));
+                                for (int m = 0; m < Ns; ++m) u[((size_t)q * Ns + m) * ns + s] = v[(size_t)m * npt + k];
+                            }
+                    off += g[0] * g[1] * g[2];
+                }
+            }
+        }
+        for (int l = 0; l < nlev; ++l) chk(adg_upload_Uw(ctx_, l, U[(size_t)l].data()), "adg_upload_Uw");
+        return meta.time;
     }
 
     // Cells limited since the start of evolve (n_limited_total, AmrDG.h:864).
@@ -641,6 +802,8 @@ class AmrDG : public Solver<AmrDG<P>> {
     adg_ctx* get_ctx() const { return ctx_; }
 
   private:
+    // max_steps counts the steps of this run, also after a restart
+    int restart_steps_offset() const { return restart_tstep_ > 0 ? restart_tstep_ : 0; }
     // The Mesh's boxes after a regrid, from the library's layout.
     void sync_mesh_layout() {
         int nl = 0, np[16] = {0};
@@ -693,6 +856,9 @@ class AmrDG : public Solver<AmrDG<P>> {
     std::vector<double> AMR_TVB_C_;
     int t_limit_ = -1;
     int n_regrids_ = 0;
+    int dtn_outplt_ = -1, restart_tstep_ = 0;
+    double dt_outplt_ = -1.0, restart_time_ = 0.0;
+    std::string data_dir_;
 };
 
 // ---------------------------------------------------------------------------
@@ -702,11 +868,24 @@ class Simulation {
   public:
     Simulation() = default;
 
+    // Simulation::run (Core/Simulation.h:89-118): the output directory is
+    // "../Results/Simulation Data/<case>[/<sim_run>]" unless setDataDir gave
+    // one; a fresh run with output removes the plotfiles of earlier runs there
+    // (tstep_*_plt directories; the reference clears the whole directory).
     void run() {
         if (!model || !solver || !mesh) throw AdgError("Simulation::run: model, numerical and geometry settings first");
         if (bdcond) bdcond->check(mesh->get_is_periodic());
+        std::string dir = data_dir_;
+        if (dir.empty()) {
+            dir = "../Results/Simulation Data/" + model->model_case;
+            if (!sim_run.empty()) dir += "/" + sim_run;
+        }
+        if (dtn_outplt > 0 || dt_outplt > 0 || restart_tstep > 0) {
+            plt::mkdir_p(dir);
+            if (restart_tstep == 0) remove_plotfiles(dir);
+        }
         solver->init(std::static_pointer_cast<ModelEquation<EquationType>>(model), mesh, dtn_outplt, dt_outplt,
-                     sim_run, restart_tstep);
+                     dir, restart_tstep);
         solver->evolve(std::static_pointer_cast<ModelEquation<EquationType>>(model), bdcond);
     }
 
@@ -745,6 +924,8 @@ class Simulation {
         if (solver) solver->setValidationMode(use_analytical_ic);
     }
     int getQ() { return model ? model->Q_model : 0; }
+    // Output / restart directory in place of the reference's fixed path.
+    void setDataDir(const std::string& dir) { data_dir_ = dir; }
 
     std::shared_ptr<EquationType> model;
     std::shared_ptr<NumericalMethodType> solver;
@@ -752,11 +933,37 @@ class Simulation {
     std::shared_ptr<BoundaryCondition<EquationType, NumericalMethodType>> bdcond;
 
   private:
+    static void remove_plotfiles(const std::string& dir) {
+        DIR* d = ::opendir(dir.c_str());
+        if (!d) return;
+        std::vector<std::string> doomed;
+        while (dirent* e = ::readdir(d)) {
+            const std::string n = e->d_name;
+            if (n.rfind("tstep_", 0) == 0 && n.size() > 4 && n.compare(n.size() - 4, 4, "_plt") == 0)
+                doomed.push_back(dir + "/" + n);
+        }
+        ::closedir(d);
+        for (const std::string& p : doomed) remove_tree(p);
+    }
+    static void remove_tree(const std::string& p) {
+        if (DIR* d = ::opendir(p.c_str())) {
+            while (dirent* e = ::readdir(d)) {
+                const std::string n = e->d_name;
+                if (n != "." && n != "..") remove_tree(p + "/" + n);
+            }
+            ::closedir(d);
+            ::rmdir(p.c_str());
+        } else {
+            std::remove(p.c_str());
+        }
+    }
+
     int dtn_outplt = -1;
     double dt_outplt = -1;
     std::string sim_run;
     int restart_tstep = 0;
     bool validation_ = true;
+    std::string data_dir_;
 };
 
 }  // namespace adg

@@ -7,7 +7,8 @@
 //                    [--weights FILE --tol TOL --max_iters K]
 //                    [--regrid] [--dtn_regrid N] [--dt_regrid T] [--amr_c C]
 //                    [--limiter TVB --tvb_m M --t_limit N] [--limit_ic]
-//                    [--projection_ic]
+//                    [--projection_ic] [--plot_every N] [--plot_dt T]
+//                    [--data_dir DIR] [--restart N]
 //                    [--graph_steps K] [--max_steps N] [--dump FILE] [--quiet]
 // Without --regrid a fine level is the static centre half of the domain; with
 // --regrid the levels start full-domain and follow the tags at the case's
@@ -36,6 +37,9 @@ struct Options {
     int dtn_regrid = -2, t_limit = -2;
     double dt_regrid = -2.0, amr_c = -1.0, tvb_m = -1.0;
     std::string limiter = "-";
+    int plot_every = -1, restart = 0;
+    double plot_dt = -1.0;
+    std::string data_dir;
 };
 
 void usage() {
@@ -46,6 +50,7 @@ void usage() {
         "                [--precision fp64|fp32] [--weights FILE --tol TOL --max_iters K]\n"
         "                [--regrid] [--dtn_regrid N] [--dt_regrid T] [--amr_c C]\n"
         "                [--limiter TVB --tvb_m M --t_limit N] [--limit_ic] [--projection_ic]\n"
+        "                [--plot_every N] [--plot_dt T] [--data_dir DIR] [--restart N]\n"
         "                [--graph_steps K] [--max_steps N] [--dump FILE] [--quiet]\n"
         "space dimension: %d\n",
         adg::SPACEDIM);
@@ -82,6 +87,10 @@ bool parse(int argc, char** argv, Options* o) {
         else if (a == "--t_limit") o->t_limit = std::atoi(next());
         else if (a == "--limit_ic") o->limit_ic = true;
         else if (a == "--projection_ic") o->projection_ic = true;
+        else if (a == "--plot_every") o->plot_every = std::atoi(next());
+        else if (a == "--plot_dt") o->plot_dt = std::atof(next());
+        else if (a == "--data_dir") o->data_dir = next();
+        else if (a == "--restart") o->restart = std::atoi(next());
         else throw adg::AdgError("unknown option " + a);
     }
     return true;
@@ -167,7 +176,8 @@ int run(const Options& opt) {
     sim.setModelSettings(simulation_case);
     sim.setNumericalSettings(T, c_dt, limiter_type, TVB_M, amr_tvb_c, t_limit);
     sim.setValidationMode(!opt.projection_ic);
-    sim.setIO(-1, -1.0);
+    sim.setIO(opt.plot_every, opt.plot_dt, "", opt.restart);
+    if (!opt.data_dir.empty()) sim.setDataDir(opt.data_dir);
     sim.setBoundaryConditions(bc_type, bc_type);
     sim.setGeometrySettings(domain, max_level, n_cell, 0, amr_ratio, is_periodic, amr_c,
                             opt.regrid ? dtn_regrid : -1, opt.regrid ? dt_regrid : -1.0, 1);

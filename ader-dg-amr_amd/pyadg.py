@@ -32,6 +32,7 @@ EXPORTS = [
     "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
     "adg_conserved_totals_noalloc", "adg_set_limiter", "adg_limit", "adg_limiter_stats",
     "adg_set_amr", "adg_set_tagging", "adg_tag_cells", "adg_regrid", "adg_remake", "adg_get_levels",
+    "adg_set_step_count",
 ]
 
 
@@ -75,6 +76,7 @@ def lib():
     L.adg_set_limiter.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.c_int]
     L.adg_limit.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
     L.adg_limiter_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
+    L.adg_set_step_count.argtypes = [C.c_void_p, C.c_int]
     L.adg_set_amr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.adg_set_tagging.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _dp]
     L.adg_tag_cells.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_ubyte)]

@@ -220,6 +220,10 @@ int adg_get_time(adg_ctx* ctx, double* t_out, double* dt_out);
  * step count since this call (AmrDG.h:990-997).  Drops a captured step
  * graph. */
 int adg_set_limiter(adg_ctx* ctx, int type, double M, int ntvb, const double* tvb_c, int t_limit);
+/* Sets the device step counter n of the limiter's n % t_limit (the step
+ * index of AmrDG::evolve, AmrDG.h:866-875; a restart continues from
+ * restart_tstep).  Every step advanced by the device adds one. */
+int adg_set_step_count(adg_ctx* ctx, int n);
 /* Replaces AmrDG::Limiter_w (AmrDG.h:1657-1770) with Limiter_linear_tvb
  * (AmrDG.h:1772-1855): the characteristic TVB limiting of the linear modes
  * of every valid cell of the finest level, now, with the settings of
