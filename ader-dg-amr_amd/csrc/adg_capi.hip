@@ -134,6 +134,7 @@ struct adg_ctx {
     bool keep_hw = false;
     int iters_override = 0;    // > 0: Picard iterations of the next plain predictor launches (collection)
     bool independent = false;  // adg_set_levels builds a batch of independent periodic patches
+    bool conservative_reflux = false;   // reflux every cell the registers reach (not the reference's rule)
 
     // learned warm start
     struct Mlp {
@@ -438,7 +439,8 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if (nlev < 1 || nlev > adg::kMaxLev || !npatch || !boxes) return fail(c, ADG_E_ARG, "bad level arguments");
     free_levels(c);
     try {
-        c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes, c->independent ? 1 : 0);
+        c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes, c->independent ? 1 : 0,
+                                    c->conservative_reflux ? 1 : 0);
     } catch (const std::exception& e) {
         return fail(c, ADG_E_ARG, e.what());
     }
@@ -651,6 +653,21 @@ int adg_set_independent_patches(adg_ctx* c, int on) {
     if (!c) return ADG_E_ARG;
     c->independent = on != 0;
     return ADG_OK;
+}
+
+int adg_set_conservative_reflux(adg_ctx* c, int on) {
+    if (!c) return ADG_E_ARG;
+    if ((on != 0) == c->conservative_reflux) return ADG_OK;
+    c->conservative_reflux = on != 0;
+    if (!c->levels_set) return ADG_OK;
+    // rebuild the register lists on the current layout (the data carry over)
+    std::vector<int> np((size_t)c->H.nlev), bx;
+    for (int l = 0; l < c->H.nlev; ++l) np[(size_t)l] = c->H.lev[l].p_end - c->H.lev[l].p_begin;
+    for (const adg::PatchLayout& P : c->H.patch) {
+        for (int d = 0; d < c->D; ++d) bx.push_back(P.lo[d]);
+        for (int d = 0; d < c->D; ++d) bx.push_back(P.hi[d]);
+    }
+    return adg_remake(c, c->H.nlev, np.data(), bx.data());
 }
 
 int adg_keep_Hw(adg_ctx* c, int on) {

@@ -50,7 +50,7 @@ void for_box(int D, const int* lo, const int* hi, F&& f) {
 }  // namespace
 
 Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
-                          const int* per, int nlev, const int* npatch, const int* boxes, int independent) {
+                          const int* per, int nlev, const int* npatch, const int* boxes, int independent, int conservative_reflux) {
     if (D < 2 || D > 3 || nlev < 1 || nlev > 8) throw std::runtime_error("bad dim or level count");
     if (independent && nlev != 1) throw std::runtime_error("independent patches form a single level");
     Hierarchy h;
@@ -393,7 +393,8 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         }
         // corrected coarse cells: reference L2ProjInterp::reflux applies the
         // correction only where a face of the cell is flagged in its own
-        // patch's (ownership-filtered) cf_face_b_coarse.  Cells covered by
+        // patch's (ownership-filtered) cf_face_b_coarse (unless
+        // conservative_reflux: every cell the registers reach).  Cells covered by
         // the fine level are left out: the average down that follows the
         // reflux (AmrDG.h:1000) overwrites them, so their correction has no
         // effect, and leaving them out lets the reflux run beside it.
@@ -414,7 +415,7 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     if (ith != land[d].end())
                         for (auto& e : ith->second) acc[e.first] -= e.second;
                 }
-                if (!at_if) return;
+                if (!at_if && !conservative_reflux) return;
                 int start = (int)h.re_mom.size();
                 for (auto& e : acc)
                     if (e.second != 0.0) { h.re_mom.push_back(e.first); h.re_coef.push_back(e.second); }

@@ -86,7 +86,13 @@ struct Hierarchy {
 // independent != 0: one level whose patches are independent periodic
 // domains (a batch of separate problems): ghost cells wrap inside their own
 // patch.
+// conservative_reflux != 0: every uncovered coarse cell the registers touch
+// is corrected; 0 (the reference): only cells with a face flagged in their
+// own patch's ownership-filtered cf_face_b_coarse (AmrDG_Interpolater.tpp:
+// 39-69), which drops the correction of a cell whose coarse-fine face is the
+// low face of its patch -- the reference loses mass at such faces.
 Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
-                          const int* per, int nlev, const int* npatch, const int* boxes, int independent = 0);
+                          const int* per, int nlev, const int* npatch, const int* boxes, int independent = 0,
+                          int conservative_reflux = 0);
 
 }  // namespace adg

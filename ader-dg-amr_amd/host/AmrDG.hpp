@@ -25,6 +25,9 @@
 //  - setLimitInitialState(true) applies the limiter once to the projected
 //    initial state (a discontinuous state projected at p >= 2 has negative
 //    nodal density and pressure, and the first step diverges without it);
+//  - setConservativeReflux(true) corrects every coarse cell the flux
+//    registers reach (the reference skips cells whose coarse-fine face is
+//    the low face of their box, and loses mass there);
 //  - plotfiles go to Simulation::setDataDir when given, else to the
 //    reference's "../Results/Simulation Data/<case>" directory;
 //  - the space dimension is the compile-time ADG_SPACEDIM (default 2), as
@@ -400,6 +403,10 @@ class AmrDG : public Solver<AmrDG<P>> {
     // Apply the limiter once to the projected initial state (not done by
     // the reference; needed for discontinuous initial states at p >= 2).
     void setLimitInitialState(bool on) { limit_ic_ = on; }
+    // Reflux every uncovered coarse cell the flux registers reach (not the
+    // reference's rule, which skips a cell whose coarse-fine face is the low
+    // face of its box and so loses mass there; AmrDG_Interpolater.tpp:39-69).
+    void setConservativeReflux(bool on) { conservative_reflux_ = on; }
 
     // Device forms: ADG_FP64 (plain) or ADG_FP32_MFMA (precision form).
     void setPrecision(int precision) { precision_ = precision; }
@@ -440,6 +447,7 @@ class AmrDG : public Solver<AmrDG<P>> {
             hi[d] = rb.hi[d];
         }
         chk(adg_set_geometry(ctx_, dom, lo, hi, per), "adg_set_geometry");
+        chk(adg_set_conservative_reflux(ctx_, conservative_reflux_ ? 1 : 0), "adg_set_conservative_reflux");
         const int nlev = mesh->get_finest_lev() + 1;
         if (nlev != mesh->get_max_level() + 1)
             throw AdgError("AmrDG::init: give the fine boxes of every level to Mesh::set_levels");
@@ -851,7 +859,7 @@ class AmrDG : public Solver<AmrDG<P>> {
     int autotune_reps_ = 0;
     int max_steps_ = -1;
     bool verbose_ = true;
-    bool limiter_on_ = false, limit_ic_ = false, dynamic_ = false;
+    bool limiter_on_ = false, limit_ic_ = false, dynamic_ = false, conservative_reflux_ = false;
     double TVB_M_ = 0.0;
     std::vector<double> AMR_TVB_C_;
     int t_limit_ = -1;

@@ -32,7 +32,7 @@ EXPORTS = [
     "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
     "adg_conserved_totals_noalloc", "adg_set_limiter", "adg_limit", "adg_limiter_stats",
     "adg_set_amr", "adg_set_tagging", "adg_tag_cells", "adg_regrid", "adg_remake", "adg_get_levels",
-    "adg_set_step_count",
+    "adg_set_step_count", "adg_set_conservative_reflux",
 ]
 
 
@@ -77,6 +77,7 @@ def lib():
     L.adg_limit.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
     L.adg_limiter_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
     L.adg_set_step_count.argtypes = [C.c_void_p, C.c_int]
+    L.adg_set_conservative_reflux.argtypes = [C.c_void_p, C.c_int]
     L.adg_set_amr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.adg_set_tagging.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _dp]
     L.adg_tag_cells.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_ubyte)]
@@ -366,6 +367,9 @@ class Context:
 
     def nvalid(self, lev):
         return sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(self.dim)])) for lo, hi in self.get_levels()[lev])
+
+    def set_conservative_reflux(self, on=True):
+        self._chk(self.L.adg_set_conservative_reflux(self.h, 1 if on else 0), "adg_set_conservative_reflux")
 
     def limit(self):
         """Limiter_w on the finest level now; returns the cells limited."""

@@ -105,6 +105,16 @@ int adg_set_independent_patches(adg_ctx* ctx, int on);
  * ghosts (AmrDG_Interpolater.tpp:251-281,344-364). */
 int adg_fill_ghosts(adg_ctx* ctx);
 
+/* Reflux rule.  0 (default, the reference's): L2ProjInterp::reflux
+ * (AmrDG_Interpolater.tpp:39-69) corrects a coarse cell only where one of
+ * its faces is flagged in its own box's ownership-filtered cf_face_b_coarse
+ * (AmrDG_Level.tpp:330-343), so a cell whose coarse-fine face is the low
+ * face of its box is not corrected and the composite mass changes there.
+ * 1: every uncovered coarse cell the flux registers reach is corrected
+ * (conservative).  No counterpart in the reference (it has only rule 0).
+ * With levels set, the register lists are rebuilt on the same layout. */
+int adg_set_conservative_reflux(adg_ctx* ctx, int on);
+
 /* Replaces AmrDG::AMR_average_fine_coarse (AmrDG_Level.tpp:227-238) ->
  * L2ProjInterp::average_down / amr_gather (AmrDG_Interpolater.tpp:283-398). */
 int adg_average_down(adg_ctx* ctx);

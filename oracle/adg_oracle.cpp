@@ -536,6 +536,7 @@ struct orc_hier {
     int D, nlev;
     double plo[3], phi[3];
     int per[3];
+    int conservative_reflux = 0;   // 0: the reference's reflux rule
     std::vector<OLevel> L;
 
     long cell_lin(int l, const int* iv) const {
@@ -954,6 +955,7 @@ orc_hier* orc_hier_create(int D, int nlev, const int* npatch, const int* boxes, 
     return h;
 }
 void orc_hier_destroy(orc_hier* h) { delete h; }
+void orc_hier_set_conservative_reflux(orc_hier* h, int on) { h->conservative_reflux = on != 0; }
 int orc_hier_nslot(const orc_hier* h, int l) { return (int)h->L[l].nslot; }
 int orc_hier_nfslot(const orc_hier* h, int l, int d) { return (int)h->L[l].nfslot[d]; }
 void orc_hier_valid_mask(const orc_hier* h, int l, int* out) {
@@ -1343,7 +1345,7 @@ void flux_correction(const orc_tables* t, const orc_hier* h, double** Uw,
                                 corr[i] += -1.0 * (corrF[d][(size_t)i * cl.nfslot[d] + fhi] -
                                                    corrF[d][(size_t)i * cl.nfslot[d] + flo]) / 1.0;
                         }
-                        if (!at_if) continue;
+                        if (!at_if && !h->conservative_reflux) continue;
                         long s = h->slot_of(lc, cb, iv);
                         for (int q = 0; q < Q; ++q) {
                             double mv[64];

@@ -54,6 +54,11 @@ orc_hier*   orc_hier_create(int D, int nlev, const int* npatch, const int* boxes
                             const int* dom_n, const double* prob_lo,
                             const double* prob_hi, const int* periodic);
 void        orc_hier_destroy(orc_hier*);
+/* 0 (default): the reference's reflux rule -- a coarse cell is corrected
+ * only where one of its faces is flagged in its own box's ownership-filtered
+ * cf_face_b_coarse (AmrDG_Interpolater.tpp:39-69).  1: every coarse cell the
+ * registers reach is corrected (no counterpart in the reference). */
+void        orc_hier_set_conservative_reflux(orc_hier*, int on);
 int         orc_hier_nslot(const orc_hier*, int lev);
 int         orc_hier_nfslot(const orc_hier*, int lev, int d);
 /* 1 if slot is a valid cell, 0 if ghost; out has nslot ints */

@@ -15,12 +15,14 @@ dptr = C.POINTER(C.c_double)
 iptr = C.POINTER(C.c_int)
 
 
-def lib():
+def lib(path=None):
+    """The oracle library: oracle/liboracle.so, or the build at `path` (the
+    first call decides, e.g. a -march=native build for the CPU baseline)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = path or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+            raise RuntimeError(f"{path} is not built (run make -C oracle)")
         L = C.CDLL(path)
         L.orc_tables_create.restype = C.c_void_p
         L.orc_tables_create.argtypes = [C.c_int, C.c_int]
@@ -31,6 +33,7 @@ def lib():
         L.orc_hier_create.restype = C.c_void_p
         L.orc_hier_create.argtypes = [C.c_int, C.c_int, iptr, iptr, iptr, dptr, dptr, iptr]
         L.orc_hier_destroy.argtypes = [C.c_void_p]
+        L.orc_hier_set_conservative_reflux.argtypes = [C.c_void_p, C.c_int]
         L.orc_hier_nslot.restype = C.c_int
         L.orc_hier_nslot.argtypes = [C.c_void_p, C.c_int]
         L.orc_hier_nfslot.restype = C.c_int
@@ -127,7 +130,7 @@ class Tables:
 class Hier:
     """Boxes per level (ratio 2, one ghost cell)."""
 
-    def __init__(self, D, levels, dom_n, prob_lo, prob_hi, periodic=None):
+    def __init__(self, D, levels, dom_n, prob_lo, prob_hi, periodic=None, conservative_reflux=False):
         self.D = D
         self.levels = [list(map(lambda b: (tuple(b[0]), tuple(b[1])), lev)) for lev in levels]
         self.nlev = len(levels)
@@ -143,6 +146,8 @@ class Hier:
         per = np.array(periodic if periodic is not None else [1] * D, dtype=np.int32)
         self.h = lib().orc_hier_create(D, self.nlev, _i(npatch), _i(boxes), _i(self.dom_n),
                                        _d(self.prob_lo), _d(self.prob_hi), _i(per))
+        if conservative_reflux:
+            lib().orc_hier_set_conservative_reflux(self.h, 1)
 
     def nslot(self, l):
         return lib().orc_hier_nslot(self.h, l)

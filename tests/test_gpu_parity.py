@@ -7,7 +7,7 @@ import pytest
 
 from bars import mode_err
 
-from layouts import kh_bands, single_level, two_level_centre
+from layouts import kh_bands, single_level, tiles, two_level_centre
 
 pytestmark = pytest.mark.gpu
 
@@ -324,3 +324,39 @@ def test_nodelane_wave_counts_agree_bitwise(adg, precision):
         ctx.close()
     for l in range(3):
         assert np.array_equal(out[0][l], out[1][l]), f"level {l}"
+
+
+@pytest.mark.parametrize("conservative", [False, True])
+def test_reflux_rules_on_a_box_edge(adg, orc, conservative):
+    """Both reflux rules against the oracle on a layout where they differ (a
+    coarse-fine face on the low edge of coarse boxes): the reference's rule
+    drifts, the conservative one keeps the composite totals to rounding.
+    The conservative rule is switched on after the levels are set, so the
+    register lists are rebuilt on the live layout."""
+    print(f"case reflux rule conservative={conservative} vortex P=2 16^2 + two fine boxes", flush=True)
+    levels = [tiles(16, 8, 2), [((8, 8), (15, 15)), ((16, 8), (23, 15))]]
+    T = orc.Tables(2, 2)
+    H = orc.Hier(2, levels, [16, 16], [0, 0], [10, 10], conservative_reflux=conservative)
+    Uw = orc.project_ic(T, H, 0, VORTEX)
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    ctx = adg.Context(2, 2, 1.4, 0)
+    ctx.set_geometry([16, 16], [0, 0], [10, 10])
+    ctx.set_levels(levels)
+    for l in range(2):
+        ctx.upload_Uw(l, Uw[l])
+    if conservative:
+        ctx.set_conservative_reflux(True)
+    m0 = ctx.totals()
+    Dt = orc.set_dt(T, H, Uw, 0.5)
+    ctx.set_dt(0.5, 10.0, 0.0)
+    for _ in range(3):
+        ctx.step(0.5, 10.0)
+    orc.evolve(T, H, Uw, 0.0, Dt, 10.0, 0.5, nsteps_max=3)
+    for l in range(2):
+        v = H.valid_mask(l)
+        assert rel_err(ctx.download_Uw(l)[:, :, v], Uw[l][:, :, v]) < TOL, f"level {l}"
+    m = ctx.totals()
+    drift = max(abs(m[q] - m0[q]) / max(abs(m0[q]), 1.0) for q in range(4))
+    assert (drift < 1e-13) if conservative else (drift > 1e-7)
+    ctx.close()
