@@ -7,6 +7,7 @@
 // whole step can be replayed without host round trips.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -135,6 +136,12 @@ struct adg_ctx {
     int iters_override = 0;    // > 0: Picard iterations of the next plain predictor launches (collection)
     bool independent = false;  // adg_set_levels builds a batch of independent periodic patches
     bool conservative_reflux = false;   // reflux every cell the registers reach (not the reference's rule)
+    bool cfl_uncovered = false;         // set_Dt over uncovered cells only (not the reference's rule)
+    // host seconds spent in adg_regrid / adg_remake, cumulative: tags, grid
+    // generation, hierarchy lists, device set-up (uploads, allocation),
+    // data transfer onto the new layout; and the number of layout changes
+    double rg_sec[5] = {0, 0, 0, 0, 0};
+    long rg_changes = 0;
 
     // learned warm start
     struct Mlp {
@@ -438,12 +445,15 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if (!c->geo_set) return fail(c, ADG_E_STATE, "adg_set_geometry must come first");
     if (nlev < 1 || nlev > adg::kMaxLev || !npatch || !boxes) return fail(c, ADG_E_ARG, "bad level arguments");
     free_levels(c);
+    const auto t_h0 = std::chrono::steady_clock::now();
     try {
         c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes, c->independent ? 1 : 0,
                                     c->conservative_reflux ? 1 : 0);
     } catch (const std::exception& e) {
         return fail(c, ADG_E_ARG, e.what());
     }
+    const auto t_h1 = std::chrono::steady_clock::now();
+    c->rg_sec[2] += std::chrono::duration<double>(t_h1 - t_h0).count();
     adg::Hierarchy& H = c->H;
     const int D = c->D, Q = c->Q;
     const adg::ElementTables& t = c->tab;
@@ -532,23 +542,26 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
     if ((rc = dev_alloc_bytes(c, (void**)&c->warm, (size_t)Q * t.Np_st * NS * sizeof(float))) != ADG_OK) return rc;
     {
+        // patches are laid out in slot order: the patch of a slot is the last
+        // one whose offset does not exceed it
         std::vector<double> xc(H.uncovered.size() * 3, 0.0);
         for (size_t i = 0; i < H.uncovered.size(); ++i) {
             const long s = H.uncovered[i];
-            for (const adg::PatchLayout& pt : H.patch) {
-                const long loc = s - pt.off;
-                if (loc < 0 || loc >= (long)pt.g[0] * pt.g[1] * pt.g[2]) continue;
-                const long r[3] = {loc % pt.g[0], (loc / pt.g[0]) % pt.g[1], loc / ((long)pt.g[0] * pt.g[1])};
-                for (int d = 0; d < D; ++d)
-                    xc[i * 3 + d] = H.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * H.lev[pt.lev].dx[d];
-                break;
-            }
+            auto it = std::upper_bound(H.patch.begin(), H.patch.end(), s,
+                                       [](long v, const adg::PatchLayout& pt) { return v < pt.off; });
+            const adg::PatchLayout& pt = *(it - 1);
+            const long loc = s - pt.off;
+            const long r[3] = {loc % pt.g[0], (loc / pt.g[0]) % pt.g[1], loc / ((long)pt.g[0] * pt.g[1])};
+            for (int d = 0; d < D; ++d)
+                xc[i * 3 + d] = H.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * H.lev[pt.lev].dx[d];
         }
         if ((rc = dev_upload(c, &c->unc_centre, xc)) != ADG_OK) return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->levels_set = true;
-    return prepare_kernels(c);
+    rc = prepare_kernels(c);
+    c->rg_sec[3] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_h1).count();
+    return rc;
 }
 
 // Raises the dynamic-LDS limit of the MFMA predictors once, outside any
@@ -670,6 +683,24 @@ int adg_set_conservative_reflux(adg_ctx* c, int on) {
     return adg_remake(c, c->H.nlev, np.data(), bx.data());
 }
 
+int adg_regrid_stats(adg_ctx* c, double* out, int reset) {
+    if (!c || !out) return ADG_E_ARG;
+    out[0] = (double)c->rg_changes;
+    for (int k = 0; k < 5; ++k) out[1 + k] = c->rg_sec[k];
+    if (reset) {
+        c->rg_changes = 0;
+        for (double& v : c->rg_sec) v = 0.0;
+    }
+    return ADG_OK;
+}
+
+int adg_set_cfl_uncovered(adg_ctx* c, int on) {
+    if (!c) return ADG_E_ARG;
+    if ((on != 0) != c->cfl_uncovered) drop_graph(c);
+    c->cfl_uncovered = on != 0;
+    return ADG_OK;
+}
+
 int adg_keep_Hw(adg_ctx* c, int on) {
     if (!c) return ADG_E_ARG;
     // a captured step graph holds the H_w pointer of its predictor launch
@@ -730,11 +761,18 @@ int adg_average_down(adg_ctx* c) {
     return ADG_OK;
 }
 
+// The cells the CFL reduction visits: every valid cell (the reference's
+// set_Dt, covered coarse cells included) or the uncovered ones.
+static const int* cfl_cells(const adg_ctx* c) { return c->cfl_uncovered ? c->uncovered : c->valid_list; }
+static long cfl_ncells(const adg_ctx* c) {
+    return (long)(c->cfl_uncovered ? c->H.uncovered.size() : c->H.valid_list.size());
+}
+
 static int launch_cfl(adg_ctx* c, double c_dt, double T, int advance) {
-    const long nv = (long)c->H.valid_list.size();
+    const long nv = cfl_ncells(c);
     const double cfl = (1.0 / (2.0 * (double)c->P + 1.0)) * (1.0 / (double)c->D);
     const int k = adg::cfl_per_thread(nv);
-    adg::CflArgs a{c->d_tab, c->to, c->Uw, c->valid_list, c->slot_lev, c->ts, c->geo, c->H.NS, nv, c->gamma, cfl, k};
+    adg::CflArgs a{c->d_tab, c->to, c->Uw, cfl_cells(c), c->slot_lev, c->ts, c->geo, c->H.NS, nv, c->gamma, cfl, k};
 #define L_CFL(PP, DD) hipLaunchKernelGGL((adg::k_cfl<PP, DD>), dim3(nblocks(nv, 256 * k)), dim3(256), 0, c->stream, a)
     ADG_DISPATCH(c, L_CFL);
 #undef L_CFL
@@ -1614,9 +1652,9 @@ static adg::EpiArgs<T> epi_args(adg_ctx* c, double c_dt, double T_end) {
     a.nrc = (long)c->H.rc_slot.size();
     a.g_slot = c->g_slot; a.g_src = c->g_src; a.g_kind = c->g_kind; a.g_child = c->g_child;
     a.nghost = (long)c->H.g_slot.size();
-    a.valid_list = c->valid_list;
+    a.valid_list = cfl_cells(c);
     a.slot_lev = c->slot_lev;
-    a.nvalid = (long)c->H.valid_list.size();
+    a.nvalid = cfl_ncells(c);
     a.cfl_k = adg::cfl_per_thread(a.nvalid);
     a.ncfl_blocks = nblocks(a.nvalid, 256 * a.cfl_k);
     a.cfl = (1.0 / (2.0 * (double)c->P + 1.0)) * (1.0 / (double)c->D);
@@ -2112,6 +2150,7 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
                           c->level_allocs.end());
     int rc = adg_set_levels(c, nlev, npatch, boxes);
     if (rc != ADG_OK) { (void)hipFree(old_uw); return rc; }
+    const auto t_x0 = std::chrono::steady_clock::now();
     const adg::Hierarchy& H = c->H;
     // level 0: the same slots
     HIPCHK(c, hipMemcpy2DAsync(c->Uw, (size_t)H.NS * sizeof(double), old_uw, (size_t)old.NS * sizeof(double),
@@ -2197,7 +2236,9 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     }
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(old_uw);
+    c->rg_sec[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_x0).count();
     if (!err.empty()) return fail(c, ADG_E_ARG, err);
+    ++c->rg_changes;
     return ADG_OK;
 }
 
@@ -2209,6 +2250,7 @@ int adg_regrid(adg_ctx* c, double time, int* changed) {
     const adg::Hierarchy& H = c->H;
     const int D = c->D;
     const int max_crse = std::min(H.nlev - 1, c->amr.max_level - 1);
+    const auto t_t0 = std::chrono::steady_clock::now();
     std::vector<adg::TagList> tl((size_t)std::max(max_crse + 1, 0));
     for (int l = 0; l <= max_crse; ++l) {
         const long n = H.lev[l].valid_end - H.lev[l].valid_begin;
@@ -2219,12 +2261,15 @@ int adg_regrid(adg_ctx* c, double time, int* changed) {
         for (long i = 0; i < n; ++i)
             if (tg[(size_t)i]) tl[(size_t)l].push_back(cells[(size_t)i]);
     }
+    const auto t_t1 = std::chrono::steady_clock::now();
+    c->rg_sec[0] += std::chrono::duration<double>(t_t1 - t_t0).count();
     std::vector<std::vector<adg::IBox>> old = level_boxes(H), ng;
     try {
         ng = adg::make_new_grids(D, c->dom0, c->per, c->amr, old, tl);
     } catch (const std::exception& e) {
         return fail(c, ADG_E_ARG, std::string("regrid: ") + e.what());
     }
+    c->rg_sec[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_t1).count();
     std::vector<int> np(1 + ng.size()), bx;
     np[0] = (int)old[0].size();
     for (const adg::IBox& b : old[0]) {

@@ -2,9 +2,13 @@
 #include "adg_hier.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 namespace adg {
@@ -37,6 +41,38 @@ struct Grid {  // owner map of one level over its domain: cell -> (patch, slot)
 };
 
 int floordiv2(int i) { return (i >= 0) ? i / 2 : -((-i + 1) / 2); }
+
+// f(i) for i in [0, n) on up to 16 host threads, items handed out one at a
+// time; the first exception thrown by any item is rethrown.
+template <class F>
+void parallel_for(int n, F&& f) {
+    int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    nt = std::min(nt, n);
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::exception_ptr err;
+    std::mutex m;
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n) return;
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(m);
+                if (!err) err = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (std::thread& t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
 
 template <class F>
 void for_box(int D, const int* lo, const int* hi, F&& f) {
@@ -115,15 +151,28 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         g.D = D;
         long nd = 1;
         for (int d = 0; d < 3; ++d) { g.dom[d] = d < D ? h.lev[l].dom[d] : 1; nd *= g.dom[d]; }
+        // boxes of a level must not overlap: sweep along x over boxes sorted by lo
+        std::vector<int> ord;
+        for (int p = h.lev[l].p_begin; p < h.lev[l].p_end; ++p) ord.push_back(p);
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return h.patch[a].lo[0] < h.patch[b].lo[0]; });
+        for (size_t i = 0; i < ord.size(); ++i)
+            for (size_t j = i + 1; j < ord.size() && h.patch[ord[j]].lo[0] <= h.patch[ord[i]].hi[0]; ++j) {
+                const PatchLayout &A = h.patch[ord[i]], &B = h.patch[ord[j]];
+                bool meet = true;
+                for (int d = 0; d < D; ++d)
+                    if (A.hi[d] < B.lo[d] || B.hi[d] < A.lo[d]) meet = false;
+                if (meet) throw std::runtime_error("overlapping boxes on a level");
+            }
         g.slot.assign(nd, -1);
         g.pidx.assign(nd, -1);
-        for (int p = h.lev[l].p_begin; p < h.lev[l].p_end; ++p)
+        parallel_for(h.lev[l].p_end - h.lev[l].p_begin, [&](int i) {
+            const int p = h.lev[l].p_begin + i;
             for_box(D, h.patch[p].lo, h.patch[p].hi, [&](const int* iv) {
-                long c = g.lin(iv);
-                if (g.slot[c] >= 0) throw std::runtime_error("overlapping boxes on a level");
+                const long c = g.lin(iv);
                 g.slot[c] = (int)h.slot_of(p, iv);
                 g.pidx[c] = p;
             });
+        });
     }
     auto wrap = [&](int l, int* iv) {
         for (int d = 0; d < D; ++d) {
@@ -161,28 +210,39 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     for (int l = 0; l < nlev; ++l)
         std::fill(h.slot_lev.begin() + h.lev[l].slot_off, h.slot_lev.begin() + h.lev[l].slot_off + h.lev[l].nslot,
                   (uint8_t)l);
+    // valid cells in patch order (each patch's range from the cell counts)
+    const int np_all = (int)h.patch.size();
+    std::vector<long> vbeg((size_t)np_all + 1, 0);
+    for (int p = 0; p < np_all; ++p) vbeg[p + 1] = vbeg[p] + (long)h.patch[p].n[0] * h.patch[p].n[1] * h.patch[p].n[2];
+    h.valid_list.assign((size_t)vbeg[np_all], 0);
     for (int d = 0; d < D; ++d)
-        for (int s = 0; s < 2; ++s) h.cell_face[d][s].clear();
+        for (int s = 0; s < 2; ++s) h.cell_face[d][s].assign((size_t)vbeg[np_all], 0);
     for (int l = 0; l < nlev; ++l) {
-        h.lev[l].valid_begin = (long)h.valid_list.size();
-        for (int p = h.lev[l].p_begin; p < h.lev[l].p_end; ++p)
-            for_box(D, h.patch[p].lo, h.patch[p].hi, [&](const int* iv) {
-                h.valid_list.push_back((int)h.slot_of(p, iv));
-                for (int d = 0; d < D; ++d) {
-                    int hv[3] = {iv[0], iv[1], iv[2]};
-                    hv[d] += 1;
-                    h.cell_face[d][0].push_back((int)h.face_of(p, d, iv));
-                    h.cell_face[d][1].push_back((int)h.face_of(p, d, hv));
-                }
-                if (!covered(l, iv)) h.uncovered.push_back((int)h.slot_of(p, iv));
-            });
-        h.lev[l].valid_end = (long)h.valid_list.size();
+        h.lev[l].valid_begin = vbeg[h.lev[l].p_begin];
+        h.lev[l].valid_end = vbeg[h.lev[l].p_end];
     }
+    std::vector<std::vector<int>> unc_p((size_t)np_all);
+    parallel_for(np_all, [&](int p) {
+        long v = vbeg[p];
+        const int l = h.patch[p].lev;
+        for_box(D, h.patch[p].lo, h.patch[p].hi, [&](const int* iv) {
+            h.valid_list[(size_t)v] = (int)h.slot_of(p, iv);
+            for (int d = 0; d < D; ++d) {
+                int hv[3] = {iv[0], iv[1], iv[2]};
+                hv[d] += 1;
+                h.cell_face[d][0][(size_t)v] = (int)h.face_of(p, d, iv);
+                h.cell_face[d][1][(size_t)v] = (int)h.face_of(p, d, hv);
+            }
+            if (!covered(l, iv)) unc_p[(size_t)p].push_back((int)h.slot_of(p, iv));
+            ++v;
+        });
+    });
+    for (const std::vector<int>& u : unc_p) h.uncovered.insert(h.uncovered.end(), u.begin(), u.end());
     // ---- faces
     for (int d = 0; d < D; ++d) {
         h.face_lr[d].assign(2 * h.NF[d], 0);
         h.face_lev[d].assign(h.NF[d], 0);
-        for (int p = 0; p < (int)h.patch.size(); ++p) {
+        parallel_for(np_all, [&](int p) {
             const PatchLayout& P = h.patch[p];
             int hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
             hi[d] += 1;
@@ -194,10 +254,14 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                 h.face_lr[d][2 * f + 1] = (int)h.slot_of(p, iv);
                 h.face_lev[d][f] = (uint8_t)P.lev;
             });
-        }
+        });
     }
-    // ---- ghost sources (FillPatchSingleLevel / FillPatchTwoLevels semantics)
-    for (int p = 0; p < (int)h.patch.size(); ++p) {
+    // ---- ghost sources (FillPatchSingleLevel / FillPatchTwoLevels semantics),
+    // per patch, then concatenated in patch order
+    struct GhostList { std::vector<int> slot, src; std::vector<uint8_t> kind, child; };
+    std::vector<GhostList> gp((size_t)np_all);
+    parallel_for(np_all, [&](int p) {
+        GhostList& gl = gp[(size_t)p];
         const PatchLayout& P = h.patch[p];
         const int l = P.lev;
         int glo[3] = {P.lo[0] - 1, P.lo[1] - 1, D > 2 ? P.lo[2] - 1 : 0};
@@ -211,8 +275,8 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             if (h.independent) {   // periodic inside the patch's own domain
                 int w[3] = {iv[0], iv[1], D > 2 ? iv[2] : 0};
                 for (int d = 0; d < D; ++d) w[d] = P.lo[d] + (((w[d] - P.lo[d]) % P.n[d]) + P.n[d]) % P.n[d];
-                h.g_slot.push_back((int)s); h.g_src.push_back((int)h.slot_of(p, w));
-                h.g_kind.push_back(0); h.g_child.push_back(0);
+                gl.slot.push_back((int)s); gl.src.push_back((int)h.slot_of(p, w));
+                gl.kind.push_back(0); gl.child.push_back(0);
                 return;
             }
             // outside a non-periodic boundary: transmissive (outflow) ghost,
@@ -227,8 +291,8 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             }
             long src = find(l, mv);
             if (src >= 0) {
-                h.g_slot.push_back((int)s); h.g_src.push_back((int)src);
-                h.g_kind.push_back((uint8_t)(0 | (mask << 2))); h.g_child.push_back(0);
+                gl.slot.push_back((int)s); gl.src.push_back((int)src);
+                gl.kind.push_back((uint8_t)(0 | (mask << 2))); gl.child.push_back(0);
                 return;
             }
             if (l == 0) throw std::runtime_error("level-0 ghost cell without a source");
@@ -239,34 +303,50 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             }
             long cs = find(l - 1, civ);
             if (cs < 0) throw std::runtime_error("coarse parent of a fine ghost cell is missing");
-            h.g_slot.push_back((int)s); h.g_src.push_back((int)cs);
-            h.g_kind.push_back((uint8_t)(1 | (mask << 2))); h.g_child.push_back((uint8_t)child);
+            gl.slot.push_back((int)s); gl.src.push_back((int)cs);
+            gl.kind.push_back((uint8_t)(1 | (mask << 2))); gl.child.push_back((uint8_t)child);
         });
+    });
+    for (const GhostList& gl : gp) {
+        h.g_slot.insert(h.g_slot.end(), gl.slot.begin(), gl.slot.end());
+        h.g_src.insert(h.g_src.end(), gl.src.begin(), gl.src.end());
+        h.g_kind.insert(h.g_kind.end(), gl.kind.begin(), gl.kind.end());
+        h.g_child.insert(h.g_child.end(), gl.child.begin(), gl.child.end());
     }
     // ---- average down groups
     const int nch = 1 << D;
     h.ad_coarse.assign(nlev, {});
     h.ad_fine.assign(nlev, {});
-    for (int l = 1; l < nlev; ++l)
-        for (int p = h.lev[l].p_begin; p < h.lev[l].p_end; ++p) {
+    for (int l = 1; l < nlev; ++l) {
+        // a fine patch's groups: its coarse footprint, in patch order
+        const int pb = h.lev[l].p_begin, npl = h.lev[l].p_end - pb;
+        std::vector<long> gb((size_t)npl + 1, 0);
+        for (int i = 0; i < npl; ++i) gb[i + 1] = gb[i] + (vbeg[pb + i + 1] - vbeg[pb + i]) / nch;
+        h.ad_coarse[l].assign((size_t)gb[npl], 0);
+        h.ad_fine[l].assign((size_t)gb[npl] * nch, 0);
+        parallel_for(npl, [&](int i) {
+            const int p = pb + i;
+            long g = gb[i];
             int clo[3], chi[3];
             for (int d = 0; d < D; ++d) { clo[d] = h.patch[p].lo[d] >> 1; chi[d] = h.patch[p].hi[d] >> 1; }
             for_box(D, clo, chi, [&](const int* iv) {
-                h.ad_coarse[l].push_back((int)find(l - 1, iv));
+                h.ad_coarse[l][(size_t)g] = (int)find(l - 1, iv);
                 for (int c = 0; c < nch; ++c) {
                     int fv[3] = {2 * iv[0], 2 * iv[1], 2 * iv[2]};
                     for (int d = 0; d < D; ++d) fv[d] += (c >> (D - 1 - d)) & 1;
-                    h.ad_fine[l].push_back((int)h.slot_of(p, fv));
+                    h.ad_fine[l][(size_t)g * nch + c] = (int)h.slot_of(p, fv);
                 }
+                ++g;
             });
-        }
+        });
+    }
     // ---- C-F face masks (AmrDG_Level.tpp:240-414)
     for (int d = 0; d < D; ++d) {
         h.bcoarse[d].assign(h.NF[d], 0);
         h.bfine[d].assign(h.NF[d], 0);
         h.cidx[d].assign(h.NF[d], 0);
     }
-    for (int p = 0; p < (int)h.patch.size(); ++p) {
+    parallel_for(np_all, [&](int p) {
         const PatchLayout& P = h.patch[p];
         const int l = P.lev;
         // coverage / validity as seen through one ghost layer of this patch
@@ -312,16 +392,16 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                 }
             });
         }
-    }
+    });
     // ---- flux-register algebra (FluxRegister CrseAdd(-1) / FineAdd(+1) /
     // Reflux(vol=1) as called in AmrDG.h:1138-1153 and AmrDG_Level.tpp:416-446)
-    std::map<std::tuple<int, int, long>, int> mom_id;  // (kind, d, face) -> index
+    std::unordered_map<uint64_t, int> mom_id;  // (kind, d, face) -> index
     auto moment = [&](int kind, int d, long face, int sel, int lev) {
-        auto key = std::make_tuple(kind, d, face);
+        const uint64_t key = ((uint64_t)kind << 62) | ((uint64_t)d << 60) | (uint64_t)face;
         auto it = mom_id.find(key);
         if (it != mom_id.end()) return it->second;
         int id = (int)h.mom_face.size();
-        mom_id[key] = id;
+        mom_id.emplace(key, id);
         h.mom_face.push_back((int)face);
         h.mom_d.push_back((uint8_t)d);
         h.mom_kind.push_back((uint8_t)kind);
@@ -338,23 +418,35 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             if (fv0[d] == 0) imgs[nimg++] = h.lev[lc].dom[d];
             else if (fv0[d] == h.lev[lc].dom[d]) imgs[nimg++] = 0;
         }
+        // a face lies in the face array of the patches owning the cell on
+        // either side of it (patches in increasing order)
         for (int ii = 0; ii < nimg; ++ii) {
-            int fv[3] = {fv0[0], fv0[1], fv0[2]};
+            int fv[3] = {fv0[0], fv0[1], D > 2 ? fv0[2] : 0};
             fv[d] = imgs[ii];
-            for (int cp = h.lev[lc].p_begin; cp < h.lev[lc].p_end; ++cp) {
-                const PatchLayout& C = h.patch[cp];
-                bool in = true;
-                for (int a = 0; a < D; ++a)
-                    if (fv[a] < C.lo[a] || fv[a] > C.hi[a] + (a == d ? 1 : 0)) in = false;
-                if (in) out.push_back({cp, h.face_of(cp, d, fv)});
+            bool indom = true;
+            for (int a = 0; a < D; ++a)
+                if (a != d && (fv[a] < 0 || fv[a] >= h.lev[lc].dom[a])) indom = false;
+            if (!indom) continue;
+            int own[2] = {-1, -1};
+            int lv[3] = {fv[0], fv[1], fv[2]};
+            lv[d] -= 1;
+            if (lv[d] >= 0) own[0] = G[lc].pidx[G[lc].lin(lv)];
+            if (fv[d] < h.lev[lc].dom[d]) own[1] = G[lc].pidx[G[lc].lin(fv)];
+            if (own[0] > own[1]) std::swap(own[0], own[1]);
+            for (int k = 0; k < 2; ++k) {
+                if (own[k] < 0 || (k == 1 && own[1] == own[0])) continue;
+                out.push_back({own[k], h.face_of(own[k], d, fv)});
             }
         }
     };
     for (int lf = 1; lf < nlev; ++lf) {
         const int lc = lf - 1;
-        // per coarse face (direction d): accumulated register terms
-        std::vector<std::unordered_map<long, std::vector<std::pair<int, double>>>> land(D);
+        // register terms landing on coarse faces, per direction: (face,
+        // moment, coefficient), grouped by face below
+        struct Land { long face; int mom; double coef; };
+        std::vector<std::vector<Land>> land(D);
         std::vector<std::pair<int, long>> arrs;
+        std::vector<std::pair<int, double>> reg;
         for (int B = h.lev[lf].p_begin; B < h.lev[lf].p_end; ++B) {
             int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
             for (int d = 0; d < D; ++d) { clo[d] = h.patch[B].lo[d] >> 1; chi[d] = h.patch[B].hi[d] >> 1; }
@@ -363,7 +455,7 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     int tlo[3] = {clo[0], clo[1], clo[2]}, thi[3] = {chi[0], chi[1], chi[2]};
                     tlo[d] = thi[d] = side == 0 ? clo[d] : chi[d] + 1;
                     for_box(D, tlo, thi, [&](const int* fv) {
-                        std::vector<std::pair<int, double>> reg;
+                        reg.clear();
                         coarse_arrays(lc, d, fv, arrs);
                         for (auto& a : arrs) {
                             int b = h.bcoarse[d][a.second];
@@ -384,48 +476,87 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                                 reg.push_back({moment(1, d, f, (b == 1 ? 1 : 0) | (h.cidx[d][f] << 1), lf), 1.0});
                         }
                         if (reg.empty()) return;
-                        for (auto& a : arrs) {
-                            auto& v = land[d][a.second];
-                            v.insert(v.end(), reg.begin(), reg.end());
-                        }
+                        for (auto& a : arrs)
+                            for (auto& r : reg) land[d].push_back({a.second, r.first, r.second});
                     });
                 }
         }
+        for (int d = 0; d < D; ++d)
+            std::stable_sort(land[d].begin(), land[d].end(), [](const Land& a, const Land& b) { return a.face < b.face; });
+        // the entries of face f of direction d: [first, last)
+        auto land_of = [&](int d, long f) {
+            auto lo = std::lower_bound(land[d].begin(), land[d].end(), f,
+                                       [](const Land& e, long v) { return e.face < v; });
+            auto hi = lo;
+            while (hi != land[d].end() && hi->face == f) ++hi;
+            return std::make_pair(lo, hi);
+        };
         // corrected coarse cells: reference L2ProjInterp::reflux applies the
         // correction only where a face of the cell is flagged in its own
         // patch's (ownership-filtered) cf_face_b_coarse (unless
         // conservative_reflux: every cell the registers reach).  Cells covered by
         // the fine level are left out: the average down that follows the
         // reflux (AmrDG.h:1000) overwrites them, so their correction has no
-        // effect, and leaving them out lets the reflux run beside it.
-        for (int cp = h.lev[lc].p_begin; cp < h.lev[lc].p_end; ++cp)
-            for_box(D, h.patch[cp].lo, h.patch[cp].hi, [&](const int* iv) {
+        // effect, and leaving them out lets the reflux run beside it.  Only
+        // the valid cells beside a face with register terms can be corrected;
+        // they are visited in slot order (patch by patch, x fastest).
+        std::vector<int> cand;
+        for (int d = 0; d < D; ++d)
+            for (size_t k = 0; k < land[d].size(); ++k) {
+                if (k > 0 && land[d][k].face == land[d][k - 1].face) continue;
+                for (int side = 0; side < 2; ++side) cand.push_back(h.face_lr[d][2 * land[d][k].face + side]);
+            }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        for (int slot : cand) {
+            int cp = h.lev[lc].p_begin, hi_p = h.lev[lc].p_end - 1;   // patch holding the slot
+            while (cp < hi_p) {
+                const int mid = (cp + hi_p + 1) / 2;
+                if (h.patch[mid].off <= slot) cp = mid; else hi_p = mid - 1;
+            }
+            const PatchLayout& P = h.patch[cp];
+            int ivc[3] = {0, 0, 0};
+            long r = slot - P.off;
+            bool valid = true;
+            for (int d = 0; d < D; ++d) {
+                ivc[d] = P.lo[d] - 1 + (int)(r % P.g[d]);
+                r /= P.g[d];
+                if (ivc[d] < P.lo[d] || ivc[d] > P.hi[d]) valid = false;
+            }
+            if (!valid) continue;
+            [&](const int* iv) {
                 if (covered(lc, iv)) return;
                 bool at_if = false;
-                std::map<int, double> acc;
+                reg.clear();   // (moment, signed coefficient), summed per moment below
                 for (int d = 0; d < D; ++d) {
                     int hv[3] = {iv[0], iv[1], iv[2]};
                     hv[d] += 1;
                     long flo = h.face_of(cp, d, iv), fhi = h.face_of(cp, d, hv);
                     if (h.bcoarse[d][flo] != 0 || h.bcoarse[d][fhi] != 0) at_if = true;
-                    auto itl = land[d].find(flo);
-                    if (itl != land[d].end())
-                        for (auto& e : itl->second) acc[e.first] += e.second;
-                    auto ith = land[d].find(fhi);
-                    if (ith != land[d].end())
-                        for (auto& e : ith->second) acc[e.first] -= e.second;
+                    auto rl = land_of(d, flo);
+                    for (auto e = rl.first; e != rl.second; ++e) reg.push_back({e->mom, e->coef});
+                    auto rh = land_of(d, fhi);
+                    for (auto e = rh.first; e != rh.second; ++e) reg.push_back({e->mom, -e->coef});
                 }
                 if (!at_if && !conservative_reflux) return;
+                std::sort(reg.begin(), reg.end(),
+                          [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
                 int start = (int)h.re_mom.size();
-                for (auto& e : acc)
-                    if (e.second != 0.0) { h.re_mom.push_back(e.first); h.re_coef.push_back(e.second); }
+                for (size_t k = 0; k < reg.size();) {
+                    size_t e = k;
+                    double sum = 0.0;   // sums of +-1: exact in any order
+                    while (e < reg.size() && reg[e].first == reg[k].first) sum += reg[e++].second;
+                    if (sum != 0.0) { h.re_mom.push_back(reg[k].first); h.re_coef.push_back(sum); }
+                    k = e;
+                }
                 int cnt = (int)h.re_mom.size() - start;
                 if (cnt == 0) return;
                 h.rc_slot.push_back((int)h.slot_of(cp, iv));
                 h.rc_lev.push_back((uint8_t)lc);
                 h.rc_start.push_back(start);
                 h.rc_count.push_back(cnt);
-            });
+            }(ivc);
+        }
     }
     return h;
 }

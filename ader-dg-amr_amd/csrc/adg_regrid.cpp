@@ -76,26 +76,42 @@ int find_cut(const std::vector<long>& sig, int lo, int* status) {
     return lo + mid;
 }
 
-// Merge pairs of boxes that share a whole face, until none does.
+// Merge boxes that share a whole face, until none does: per direction d,
+// boxes with equal extents in the other directions are sorted along d and
+// runs of abutting boxes joined; the directions are swept until a sweep
+// joins nothing.
 void simplify(int D, std::vector<IBox>& bl) {
-    bool merged = true;
-    while (merged) {
-        merged = false;
-        for (size_t i = 0; i < bl.size() && !merged; ++i)
-            for (size_t j = i + 1; j < bl.size() && !merged; ++j) {
-                for (int d = 0; d < D && !merged; ++d) {
+    bool any = true;
+    while (any) {
+        any = false;
+        for (int d = 0; d < D; ++d) {
+            auto key_less = [&](const IBox& a, const IBox& b) {
+                for (int e = 0; e < D; ++e) {
+                    if (e == d) continue;
+                    if (a.lo[e] != b.lo[e]) return a.lo[e] < b.lo[e];
+                    if (a.hi[e] != b.hi[e]) return a.hi[e] < b.hi[e];
+                }
+                return a.lo[d] < b.lo[d];
+            };
+            std::sort(bl.begin(), bl.end(), key_less);
+            std::vector<IBox> out;
+            out.reserve(bl.size());
+            for (const IBox& b : bl) {
+                if (!out.empty()) {
+                    IBox& o = out.back();
                     bool same = true;
                     for (int e = 0; e < D; ++e)
-                        if (e != d && (bl[i].lo[e] != bl[j].lo[e] || bl[i].hi[e] != bl[j].hi[e])) same = false;
-                    if (!same) continue;
-                    if (bl[i].hi[d] + 1 == bl[j].lo[d] || bl[j].hi[d] + 1 == bl[i].lo[d]) {
-                        bl[i].lo[d] = std::min(bl[i].lo[d], bl[j].lo[d]);
-                        bl[i].hi[d] = std::max(bl[i].hi[d], bl[j].hi[d]);
-                        bl.erase(bl.begin() + (long)j);
-                        merged = true;
+                        if (e != d && (o.lo[e] != b.lo[e] || o.hi[e] != b.hi[e])) same = false;
+                    if (same && o.hi[d] + 1 == b.lo[d]) {
+                        o.hi[d] = b.hi[d];
+                        any = true;
+                        continue;
                     }
                 }
+                out.push_back(b);
             }
+            bl.swap(out);
+        }
     }
 }
 

@@ -28,6 +28,8 @@
 //  - setConservativeReflux(true) corrects every coarse cell the flux
 //    registers reach (the reference skips cells whose coarse-fine face is
 //    the low face of their box, and loses mass there);
+//  - setCflUncovered(true) takes the time step from the uncovered cells
+//    only (the reference also reads covered coarse cells);
 //  - plotfiles go to Simulation::setDataDir when given, else to the
 //    reference's "../Results/Simulation Data/<case>" directory;
 //  - the space dimension is the compile-time ADG_SPACEDIM (default 2), as
@@ -407,6 +409,10 @@ class AmrDG : public Solver<AmrDG<P>> {
     // reference's rule, which skips a cell whose coarse-fine face is the low
     // face of its box and so loses mass there; AmrDG_Interpolater.tpp:39-69).
     void setConservativeReflux(bool on) { conservative_reflux_ = on; }
+    // Time step from the uncovered cells only (the reference's set_Dt also
+    // reads covered coarse cells, whose projected data can have negative
+    // nodal density across a discontinuity and collapse Dt).
+    void setCflUncovered(bool on) { cfl_uncovered_ = on; }
 
     // Device forms: ADG_FP64 (plain) or ADG_FP32_MFMA (precision form).
     void setPrecision(int precision) { precision_ = precision; }
@@ -448,6 +454,7 @@ class AmrDG : public Solver<AmrDG<P>> {
         }
         chk(adg_set_geometry(ctx_, dom, lo, hi, per), "adg_set_geometry");
         chk(adg_set_conservative_reflux(ctx_, conservative_reflux_ ? 1 : 0), "adg_set_conservative_reflux");
+        chk(adg_set_cfl_uncovered(ctx_, cfl_uncovered_ ? 1 : 0), "adg_set_cfl_uncovered");
         const int nlev = mesh->get_finest_lev() + 1;
         if (nlev != mesh->get_max_level() + 1)
             throw AdgError("AmrDG::init: give the fine boxes of every level to Mesh::set_levels");
@@ -860,6 +867,7 @@ class AmrDG : public Solver<AmrDG<P>> {
     int max_steps_ = -1;
     bool verbose_ = true;
     bool limiter_on_ = false, limit_ic_ = false, dynamic_ = false, conservative_reflux_ = false;
+    bool cfl_uncovered_ = false;
     double TVB_M_ = 0.0;
     std::vector<double> AMR_TVB_C_;
     int t_limit_ = -1;

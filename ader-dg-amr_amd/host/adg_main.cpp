@@ -7,7 +7,7 @@
 //                    [--weights FILE --tol TOL --max_iters K]
 //                    [--regrid] [--dtn_regrid N] [--dt_regrid T] [--amr_c C]
 //                    [--limiter TVB --tvb_m M --t_limit N] [--limit_ic]
-//                    [--projection_ic] [--conservative_reflux]
+//                    [--projection_ic] [--conservative_reflux] [--cfl_uncovered]
 //                    [--plot_every N] [--plot_dt T]
 //                    [--data_dir DIR] [--restart N]
 //                    [--graph_steps K] [--max_steps N] [--dump FILE] [--quiet]
@@ -35,6 +35,7 @@ struct Options {
     std::string precision = "fp64", weights, dump;
     bool quiet = false;
     bool regrid = false, limit_ic = false, projection_ic = false, conservative_reflux = false;
+    bool cfl_uncovered = false;
     int dtn_regrid = -2, t_limit = -2;
     double dt_regrid = -2.0, amr_c = -1.0, tvb_m = -1.0;
     std::string limiter = "-";
@@ -51,7 +52,7 @@ void usage() {
         "                [--precision fp64|fp32] [--weights FILE --tol TOL --max_iters K]\n"
         "                [--regrid] [--dtn_regrid N] [--dt_regrid T] [--amr_c C]\n"
         "                [--limiter TVB --tvb_m M --t_limit N] [--limit_ic] [--projection_ic]\n"
-        "                [--conservative_reflux]\n"
+        "                [--conservative_reflux] [--cfl_uncovered]\n"
         "                [--plot_every N] [--plot_dt T] [--data_dir DIR] [--restart N]\n"
         "                [--graph_steps K] [--max_steps N] [--dump FILE] [--quiet]\n"
         "space dimension: %d\n",
@@ -90,6 +91,7 @@ bool parse(int argc, char** argv, Options* o) {
         else if (a == "--limit_ic") o->limit_ic = true;
         else if (a == "--projection_ic") o->projection_ic = true;
         else if (a == "--conservative_reflux") o->conservative_reflux = true;
+        else if (a == "--cfl_uncovered") o->cfl_uncovered = true;
         else if (a == "--plot_every") o->plot_every = std::atoi(next());
         else if (a == "--plot_dt") o->plot_dt = std::atof(next());
         else if (a == "--data_dir") o->data_dir = next();
@@ -187,6 +189,7 @@ int run(const Options& opt) {
     if (!opt.regrid && max_level == 1) sim.mesh->set_centre_level();
     sim.solver->setLimitInitialState(limit_ic);
     sim.solver->setConservativeReflux(opt.conservative_reflux);
+    sim.solver->setCflUncovered(opt.cfl_uncovered);
 
     sim.solver->setPrecision(opt.precision == "fp32" ? ADG_FP32_MFMA : ADG_FP64);
     if (!opt.weights.empty()) sim.solver->setLearnedWarmStart(opt.weights, opt.tol, opt.max_iters);

@@ -32,7 +32,7 @@ EXPORTS = [
     "adg_autotune", "adg_project_ic_patches", "adg_set_independent_patches",
     "adg_conserved_totals_noalloc", "adg_set_limiter", "adg_limit", "adg_limiter_stats",
     "adg_set_amr", "adg_set_tagging", "adg_tag_cells", "adg_regrid", "adg_remake", "adg_get_levels",
-    "adg_set_step_count", "adg_set_conservative_reflux",
+    "adg_set_step_count", "adg_set_conservative_reflux", "adg_set_cfl_uncovered", "adg_regrid_stats",
 ]
 
 
@@ -78,6 +78,8 @@ def lib():
     L.adg_limiter_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_int]
     L.adg_set_step_count.argtypes = [C.c_void_p, C.c_int]
     L.adg_set_conservative_reflux.argtypes = [C.c_void_p, C.c_int]
+    L.adg_set_cfl_uncovered.argtypes = [C.c_void_p, C.c_int]
+    L.adg_regrid_stats.argtypes = [C.c_void_p, _dp, C.c_int]
     L.adg_set_amr.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
     L.adg_set_tagging.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _dp]
     L.adg_tag_cells.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_ubyte)]
@@ -370,6 +372,16 @@ class Context:
 
     def set_conservative_reflux(self, on=True):
         self._chk(self.L.adg_set_conservative_reflux(self.h, 1 if on else 0), "adg_set_conservative_reflux")
+
+    def set_cfl_uncovered(self, on=True):
+        self._chk(self.L.adg_set_cfl_uncovered(self.h, 1 if on else 0), "adg_set_cfl_uncovered")
+
+    def regrid_stats(self, reset=False):
+        """{"changes", "tags", "grids", "hierarchy", "setup", "transfer"}: layout
+        changes and host seconds per regrid phase since the last reset."""
+        o = np.zeros(6)
+        self._chk(self.L.adg_regrid_stats(self.h, _d(o), 1 if reset else 0), "adg_regrid_stats")
+        return dict(zip(("changes", "tags", "grids", "hierarchy", "setup", "transfer"), o.tolist()))
 
     def limit(self):
         """Limiter_w on the finest level now; returns the cells limited."""

@@ -115,6 +115,26 @@ int adg_fill_ghosts(adg_ctx* ctx);
  * With levels set, the register lists are rebuilt on the same layout. */
 int adg_set_conservative_reflux(adg_ctx* ctx, int on);
 
+/* Cells of the CFL reduction.  0 (default, the reference's set_Dt,
+ * AmrDG.h:1253-1357): every valid cell of every level, covered coarse cells
+ * included.  1: the uncovered cells only.  A covered coarse cell holds the
+ * L2 projection of its children, which across a discontinuity can have
+ * negative density at spatial node 0 where the wave speed is taken, so
+ * rule 0 can collapse Dt while the composite solution is physical (3D
+ * shock tubes on three levels); covered cells are overwritten by the
+ * average down every step and take no part in the composite update (with
+ * the conservative reflux rule).  No counterpart in the reference. */
+int adg_set_cfl_uncovered(adg_ctx* ctx, int on);
+
+/* Host time of the regrid path, cumulative since the last reset: out[0] the
+ * number of layout changes (adg_remake calls that succeeded), out[1..5]
+ * seconds in refinement tags (device tagging and copy back), grid
+ * generation, hierarchy lists, device set-up of the new layout (uploads,
+ * allocations, kernel preparation) and the transfer of U_w onto it.  The
+ * reference times none of this.  reset != 0 clears the counters after
+ * reading them. */
+int adg_regrid_stats(adg_ctx* ctx, double* out, int reset);
+
 /* Replaces AmrDG::AMR_average_fine_coarse (AmrDG_Level.tpp:227-238) ->
  * L2ProjInterp::average_down / amr_gather (AmrDG_Interpolater.tpp:283-398). */
 int adg_average_down(adg_ctx* ctx);

@@ -537,6 +537,7 @@ struct orc_hier {
     double plo[3], phi[3];
     int per[3];
     int conservative_reflux = 0;   // 0: the reference's reflux rule
+    int cfl_uncovered = 0;         // 0: the reference's set_Dt over every valid cell
     std::vector<OLevel> L;
 
     long cell_lin(int l, const int* iv) const {
@@ -956,6 +957,7 @@ orc_hier* orc_hier_create(int D, int nlev, const int* npatch, const int* boxes, 
 }
 void orc_hier_destroy(orc_hier* h) { delete h; }
 void orc_hier_set_conservative_reflux(orc_hier* h, int on) { h->conservative_reflux = on != 0; }
+void orc_hier_set_cfl_uncovered(orc_hier* h, int on) { h->cfl_uncovered = on != 0; }
 int orc_hier_nslot(const orc_hier* h, int l) { return (int)h->L[l].nslot; }
 int orc_hier_nfslot(const orc_hier* h, int l, int d) { return (int)h->L[l].nfslot[d]; }
 void orc_hier_valid_mask(const orc_hier* h, int l, int* out) {
@@ -1524,6 +1526,11 @@ extern "C" double orc_set_dt(const orc_tables* t, const orc_hier* h, double** Uw
 #pragma omp parallel for reduction(min : lvmin) schedule(static)
         for (long s = 0; s < ns; ++s) {
             if (!lv.valid[s]) continue;
+            if (h->cfl_uncovered && l + 1 < h->nlev) {
+                int b, iv[3];
+                h->slot_decode(l, s, &b, iv);
+                if (h->covered_by_finer(l, iv)) continue;
+            }
             double u[5];
             for (int q = 0; q < Q; ++q) {
                 double a = 0.0;

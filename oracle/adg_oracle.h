@@ -59,6 +59,10 @@ void        orc_hier_destroy(orc_hier*);
  * cf_face_b_coarse (AmrDG_Interpolater.tpp:39-69).  1: every coarse cell the
  * registers reach is corrected (no counterpart in the reference). */
 void        orc_hier_set_conservative_reflux(orc_hier*, int on);
+/* 0 (default): set_Dt over every valid cell of every level, covered
+ * coarse cells included (AmrDG.h:1253-1357).  1: over the uncovered cells
+ * only (no counterpart in the reference). */
+void        orc_hier_set_cfl_uncovered(orc_hier*, int on);
 int         orc_hier_nslot(const orc_hier*, int lev);
 int         orc_hier_nfslot(const orc_hier*, int lev, int d);
 /* 1 if slot is a valid cell, 0 if ghost; out has nslot ints */

@@ -34,6 +34,7 @@ def lib(path=None):
         L.orc_hier_create.argtypes = [C.c_int, C.c_int, iptr, iptr, iptr, dptr, dptr, iptr]
         L.orc_hier_destroy.argtypes = [C.c_void_p]
         L.orc_hier_set_conservative_reflux.argtypes = [C.c_void_p, C.c_int]
+        L.orc_hier_set_cfl_uncovered.argtypes = [C.c_void_p, C.c_int]
         L.orc_hier_nslot.restype = C.c_int
         L.orc_hier_nslot.argtypes = [C.c_void_p, C.c_int]
         L.orc_hier_nfslot.restype = C.c_int
@@ -130,7 +131,8 @@ class Tables:
 class Hier:
     """Boxes per level (ratio 2, one ghost cell)."""
 
-    def __init__(self, D, levels, dom_n, prob_lo, prob_hi, periodic=None, conservative_reflux=False):
+    def __init__(self, D, levels, dom_n, prob_lo, prob_hi, periodic=None, conservative_reflux=False,
+                 cfl_uncovered=False):
         self.D = D
         self.levels = [list(map(lambda b: (tuple(b[0]), tuple(b[1])), lev)) for lev in levels]
         self.nlev = len(levels)
@@ -148,6 +150,8 @@ class Hier:
                                        _d(self.prob_lo), _d(self.prob_hi), _i(per))
         if conservative_reflux:
             lib().orc_hier_set_conservative_reflux(self.h, 1)
+        if cfl_uncovered:
+            lib().orc_hier_set_cfl_uncovered(self.h, 1)
 
     def nslot(self, l):
         return lib().orc_hier_nslot(self.h, l)

@@ -360,3 +360,44 @@ def test_reflux_rules_on_a_box_edge(adg, orc, conservative):
     drift = max(abs(m[q] - m0[q]) / max(abs(m0[q]), 1.0) for q in range(4))
     assert (drift < 1e-13) if conservative else (drift > 1e-7)
     ctx.close()
+
+
+@pytest.mark.parametrize("cfl_uncovered", [False, True])
+def test_cfl_rules_with_a_bad_covered_cell(adg, orc, cfl_uncovered):
+    """set_Dt under both rules against the oracle on a state whose one
+    covered coarse cell has negative density at spatial node 0 (as the
+    projection of a discontinuity can leave it): the reference's rule
+    collapses Dt, the uncovered-cell rule ignores the cell; then steps with
+    the step graph, which must pick the rule up."""
+    print(f"case CFL rule cfl_uncovered={cfl_uncovered} vortex P=2 16^2 + centre", flush=True)
+    levels = two_level_centre(16, 16)
+    T = orc.Tables(2, 2)
+    H = orc.Hier(2, levels, [16, 16], [0, 0], [10, 10], cfl_uncovered=cfl_uncovered)
+    Uw = orc.project_ic(T, H, 0, VORTEX)
+    orc.average_down(T, H, Uw)
+    orc.fill_patch(T, H, Uw)
+    dt_clean = orc.set_dt(T, H, Uw, 0.5)
+    s = int(np.flatnonzero(H.valid_mask(0))[0]) + 18 * 9 + 9   # coarse cell (9, 9), covered
+    Uw[0][0, 1, s] = -5.0 * Uw[0][0, 0, s]
+    dt_ref = orc.set_dt(T, H, Uw, 0.5)
+    assert (dt_ref == dt_clean) if cfl_uncovered else (dt_ref < 0.1 * dt_clean)
+    ctx = adg.Context(2, 2, 1.4, 0)
+    ctx.set_geometry([16, 16], [0, 0], [10, 10])
+    ctx.set_levels(levels)
+    for l in range(2):
+        ctx.upload_Uw(l, Uw[l])
+    ctx.set_dt(0.5, 10.0, 0.0)
+    ctx.step_graph(0.5, 10.0, 1)      # a graph captured under the default rule
+    for l in range(2):
+        ctx.upload_Uw(l, Uw[l])
+    ctx.set_cfl_uncovered(cfl_uncovered)
+    dt = ctx.set_dt(0.5, 10.0, 0.0)
+    assert abs(dt - dt_ref) <= 1e-14 * dt_ref
+    ctx.step_graph(0.5, 10.0, 2)
+    _, t, Dt = orc.evolve(T, H, Uw, 0.0, dt_ref, 10.0, 0.5, nsteps_max=2)
+    t_dev, Dt_dev = ctx.get_time()
+    assert abs(t_dev - t) <= 1e-14 * t and abs(Dt_dev - Dt) <= 1e-12 * Dt
+    for l in range(2):
+        v = H.valid_mask(l)
+        assert rel_err(ctx.download_Uw(l)[:, :, v], Uw[l][:, :, v]) < TOL, f"level {l}"
+    ctx.close()

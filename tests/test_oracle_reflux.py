@@ -51,3 +51,36 @@ def test_rules_agree_off_box_edges(orc):
     for l in range(2):
         v = H.valid_mask(l)
         assert np.array_equal(Ua[l][:, :, v], Ub[l][:, :, v]), f"level {l}"
+
+
+def _planted_state(orc, cfl_uncovered):
+    """Vortex on two levels with one covered coarse cell given a negative
+    density at spatial node 0 (mode 1 of density made large), as the L2
+    projection of a discontinuity can leave it."""
+    levels = two_level_centre(16, 16)
+    T = orc.Tables(2, 2)
+    H = orc.Hier(2, levels, [16, 16], [0, 0], [10.0, 10.0], cfl_uncovered=cfl_uncovered)
+    U = orc.project_ic(T, H, 0, VORTEX)
+    orc.average_down(T, H, U)
+    orc.fill_patch(T, H, U)
+    s = int(np.flatnonzero(H.valid_mask(0))[0]) + 18 * 9 + 9   # coarse cell (9, 9): under the fine level
+    assert H.valid_mask(0)[s]
+    return T, H, U, s
+
+
+def test_cfl_over_uncovered_cells(orc):
+    """The reference's set_Dt reads covered coarse cells: a negative node
+    density there collapses Dt.  Over the uncovered cells only, Dt is the
+    one of the state without the planted cell."""
+    T, H, U, s = _planted_state(orc, False)
+    dt_clean = orc.set_dt(T, H, U, 0.5)
+    U[0][0, 1, s] = -5.0 * U[0][0, 0, s]       # node 0 at x1 = +0.77: rho0 - 3.9 rho0 < 0
+    dt_ref = orc.set_dt(T, H, U, 0.5)
+    assert dt_ref < 0.1 * dt_clean
+    T2, H2, U2, s2 = _planted_state(orc, True)
+    assert s2 == s
+    dt_unc_clean = orc.set_dt(T2, H2, U2, 0.5)
+    U2[0][0, 1, s] = -5.0 * U2[0][0, 0, s]
+    assert orc.set_dt(T2, H2, U2, 0.5) == dt_unc_clean
+    # on a healthy state the finest level sets Dt under either rule
+    assert dt_unc_clean == dt_clean
