@@ -639,9 +639,11 @@ int adg_download_Uw(adg_ctx* c, int lev, double* host) {
 int adg_project_ic(adg_ctx* c, int lev, int ic_case, const double* params) {
     if (!c) return ADG_E_ARG;
     if (!c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
-    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 6)
+    if (lev < 0 || lev >= c->H.nlev || !params || ic_case < 0 || ic_case > 7)
         return fail(c, ADG_E_ARG, "bad level, case or params");
-    if ((ic_case <= 1 || ic_case == 4 || ic_case == 6) && c->D != 2)
+    if (ic_case == 7 && !(params[3] >= 1.0))
+        return fail(c, ADG_E_ARG, "case 7 needs a noise grid of at least one cell");
+    if ((ic_case <= 1 || ic_case == 4 || ic_case == 6 || ic_case == 7) && c->D != 2)
         return fail(c, ADG_E_UNSUP, "vortex / Kelvin-Helmholtz states are 2D");
     std::vector<double> host;
     adg::project_ic(c->tab, c->H, lev, ic_case, params, c->gamma, host);

@@ -19,10 +19,15 @@
 //  4  Kelvin-Helmholtz shear layer as in 1 with the seeded multi-mode
 //     perturbation v = sum_{k=1..8} a_k sin(2 pi k x), a_k = amax * r_k,
 //     r_k a seeded uniform number (no counterpart in the reference);
+//  7  Kelvin-Helmholtz as in 1 with the noise constant on the cells of a
+//     fixed N x N grid of the unit square (params: amp, noise_amp, seed, N):
+//     the same initial state at every resolution, so that runs at different
+//     resolutions can be compared (no counterpart in the reference);
 //  6  the reference's kelvin_helmolz_instability state
 //     (Compressible_Euler.h:237-256): v = w0 sin(4 pi x) times Gaussians of
 //     width sigma about y = 0.25 and 0.75 (params: w0, sigma; the reference
 //     uses w0 = 0.2, sigma = 0.05/sqrt 2).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -79,6 +84,17 @@ void pointwise_state(int D, int ic_case, const double* prm, double gamma, const 
         const bool in = std::fabs(x[1] - 0.5) < 0.25;
         const double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
         const double u2 = amp * std::sin(4.0 * M_PI * x[0]) + namp * (2.0 * cell_uniform(seed, lev, iv) - 1.0);
+        u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
+        u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+    } else if (ic_case == 7) {
+        const double amp = prm[0], namp = prm[1];
+        const uint64_t seed = (uint64_t)prm[2];
+        const int N = (int)prm[3];
+        int nv[3] = {0, 0, 0};
+        for (int d = 0; d < 2; ++d) nv[d] = std::min(N - 1, std::max(0, (int)std::floor(x[d] * N)));
+        const bool in = std::fabs(x[1] - 0.5) < 0.25;
+        const double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
+        const double u2 = amp * std::sin(4.0 * M_PI * x[0]) + namp * (2.0 * cell_uniform(seed, 0, nv) - 1.0);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2;
         u[D + 1] = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
     } else if (ic_case == 4) {

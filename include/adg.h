@@ -87,7 +87,10 @@ int adg_download_Uw(adg_ctx* ctx, int lev, double* host);
  *   ic_case 6: the reference's kelvin_helmolz_instability state
  *              (Compressible_Euler.h:237-256), v = w0 sin(4 pi x) times
  *              Gaussians of width sigma about y = 0.25, 0.75, params
- *              {w0, sigma} (reference: 0.2, 0.05/sqrt(2)) */
+ *              {w0, sigma} (reference: 0.2, 0.05/sqrt(2))
+ *   ic_case 7: Kelvin-Helmholtz as 1 with the noise constant on the cells
+ *              of a fixed N x N grid of the unit square, params {amp,
+ *              noise_amp, seed, N}: the same state at every resolution */
 int adg_project_ic(adg_ctx* ctx, int lev, int ic_case, const double* params);
 /* As adg_project_ic with one parameter set of nprm values per patch of the
  * level (params [npatch][nprm]); for independent patches the coordinates are

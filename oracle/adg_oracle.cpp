@@ -1037,6 +1037,19 @@ static void ic_state(int D, int ic_case, const double* prm, double gamma, const 
         double u2 = amp * std::sin(4.0 * M_PI * x[0]) + namp * (2.0 * cell_noise(seed, lev, iv) - 1.0);
         double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
         u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
+    } else if (ic_case == 7) {
+        // KH as case 1, noise constant on the cells of an N x N grid of the
+        // unit square
+        double amp = prm[0], namp = prm[1];
+        uint64_t seed = (uint64_t)prm[2];
+        int N = (int)prm[3];
+        int nv[3] = {0, 0, 0};
+        for (int d = 0; d < 2; ++d) nv[d] = std::min(N - 1, std::max(0, (int)std::floor(x[d] * N)));
+        bool in = std::fabs(x[1] - 0.5) < 0.25;
+        double rho = in ? 2.0 : 1.0, u1 = in ? 0.5 : -0.5, prs = 2.5;
+        double u2 = amp * std::sin(4.0 * M_PI * x[0]) + namp * (2.0 * cell_noise(seed, 0, nv) - 1.0);
+        double E = (prs / (gamma - 1.0)) + 0.5 * rho * (u1 * u1 + u2 * u2);
+        u[0] = rho; u[1] = rho * u1; u[2] = rho * u2; u[D + 1] = E;
     } else if (ic_case == 4) {
         // KH with v = sum_{k=1..8} a_k sin(2 pi k x), a_k = amax * r_k
         uint64_t seed = (uint64_t)prm[0];

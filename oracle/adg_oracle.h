@@ -82,6 +82,9 @@ void        orc_hier_slot_centre(const orc_hier*, int lev, int slot, double* x);
  *         cube, velocity (1, 0.5, 0.25), pressure p0 (params: amp, p0)
  * case 4: Kelvin-Helmholtz with v = sum_{k=1..8} a_k sin(2 pi k x),
  *         a_k = amax * r_k, r_k seeded uniform (params: seed, amax)  [no reference IC]
+ * case 7: Kelvin-Helmholtz as case 1 with the noise constant on the cells of
+ *         an N x N grid of the unit square (params: amp, noise_amp, seed, N)
+ *         [no reference IC]
  * Fills every slot (ghosts too, as AmrDG::set_initial_condition on growntilebox). */
 void        orc_project_ic(const orc_tables*, const orc_hier*, int lev, int ic_case,
                            const double* params, double gamma, double* Uw);

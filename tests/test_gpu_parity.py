@@ -158,6 +158,17 @@ def test_gpu_ic_matches_oracle(adg, orc):
     ctx.project_ic(0, 1, [0.01, 1e-3, 7])
     ref = orc.project_ic(T, H, 1, [0.01, 1e-3, 7])[0]
     assert rel_err(ctx.download_Uw(0), ref) < 1e-13
+    # the fixed-grid noise variant on the unit square, two levels
+    levels = two_level_centre(16, 8)
+    H = orc.Hier(2, levels, [16, 16], [0, 0], [1, 1])
+    ctx = adg.Context(2, 3)
+    ctx.set_geometry([16, 16], [0, 0], [1, 1])
+    ctx.set_levels(levels)
+    prm = [0.01, 1e-3, 7, 8]
+    ref = orc.project_ic(T, H, 7, prm)
+    for l in range(2):
+        ctx.project_ic(l, 7, prm)
+        assert rel_err(ctx.download_Uw(l), ref[l]) < 1e-13, f"level {l}"
 
 
 @pytest.mark.parametrize("kind,waves", [(2, 8), (3, 8), (3, 16)])
