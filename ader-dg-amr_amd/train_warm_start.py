@@ -13,11 +13,21 @@ Evaluation: held-out seeds, distance of the learned start after 1 and 2
 Picard iterations to the converged predictor, next to the plain predictor
 after 1, 2 and 3 iterations.
 
+With --case kh-job (the committed weights/kh_p3_2d): the pairs come from
+whole Kelvin-Helmholtz jobs, t = 0 -> 1, on three levels built and rebuilt
+from density-gradient tags (a regrid every 0.05), the TVB limiter (M = 5000)
+and the composite rules, one collection after every regrid: odd seeds the
+multi-mode state (initial condition 4), even seeds the fixed-grid noise
+state (initial condition 7); optionally (--band-pairs) pairs from early
+states (a collection every 20 steps from t = 0) on the fixed band hierarchy
+of the bench's kernel measurement; held-out seeds are evaluated at t = 0, 0.5 and
+0.9 of their own jobs.
+
 With --case wave3d: the 3D p=2 network, from runs of the smooth density wave
 (initial condition 5) with seeded amplitude and pressure on the periodic
 [0,1]^3 with two levels (32^3, the centre half refined).
 
-    python train_warm_start.py --out weights/kh_p3_2d
+    python train_warm_start.py --case kh-job --out weights/kh_p3_2d
     python train_warm_start.py --case wave3d --out weights/wave_p2_3d
 """
 import argparse
@@ -74,6 +84,57 @@ def kh_context(seed, P=3, n0=64, nlev=3, precision=pyadg.ADG_FP32_MFMA, ic=4):
     return ctx, levels
 
 
+def kh_job_context(seed, ic, precision=pyadg.ADG_FP32_MFMA, n0=64):
+    """Three levels from the initial state's tags (analytic state on each),
+    the TVB limiter and both composite rules, as the bench's job."""
+    ctx = pyadg.Context(2, 3, 1.4, precision)
+    ctx.set_geometry([n0, n0], [0, 0], [1, 1])
+    ctx.set_conservative_reflux(True)
+    ctx.set_cfl_uncovered(True)
+    ctx.set_amr(2, max_grid_size=32)
+    ctx.set_tagging(2, [0.05])
+    kh_job_reset(ctx, seed, ic, n0)
+    return ctx
+
+
+def kh_job_reset(ctx, seed, ic, n0=64):
+    """The job's initial hierarchy and state for another seed, in place (the
+    dataset on the device is kept)."""
+    from layouts import tiles
+    prm = [seed, 0.02] if ic == 4 else [0.01, 1e-3, 7000 + seed, n0]
+    ctx.set_levels([tiles(n0, 32, 2)])
+    ctx.project_ic(0, ic, prm)
+    ctx.fill_ghosts()
+    ctx.set_limiter(True, 5000.0, [1.0] * 3, 1)
+    for _ in range(2):
+        ctx.regrid(0.0)
+        for l in range(len(ctx.get_levels())):
+            ctx.project_ic(l, ic, prm)
+        ctx.average_down()
+        ctx.fill_ghosts()
+    ctx.set_dt(C_DT, 1.0, 0.0)
+
+
+def kh_job_advance(ctx, t_stop, on_regrid=None):
+    """Steps the job to t_stop with a regrid every 0.05 (graph batches that
+    do not cross a regrid); on_regrid(ctx) after every regrid."""
+    t, Dt = ctx.get_time()
+    next_rg = (int(t / 0.05 + 1e-9) + 1) * 0.05
+    while t < t_stop - 1e-12:
+        room = 0.5 * min(t_stop - t, next_rg - t) / Dt
+        n = int(min(64, room))
+        if n >= 2:
+            ctx.step_graph(C_DT, 1.0, n)
+        else:
+            ctx.step(C_DT, 1.0)
+        t, Dt = ctx.get_time()
+        if t >= next_rg - 1e-12 and t < 1.0 - 1e-12:
+            ctx.regrid(t)
+            next_rg = t + 0.05
+            if on_regrid:
+                on_regrid(ctx)
+
+
 def valid_slots(levels, lev):
     """Slot indices of the valid cells of a level (grown boxes, x fastest)."""
     out, off = [], 0
@@ -86,8 +147,13 @@ def valid_slots(levels, lev):
     return np.array(out)
 
 
-def evaluate(par, seed, ic, P=3, case="kh"):
-    if case == "kh":
+def evaluate(par, seed, ic, P=3, case="kh", t_at=0.0):
+    if case == "kh-job":
+        ctx = kh_job_context(seed, ic, precision=pyadg.ADG_FP64)
+        if t_at > 0:
+            kh_job_advance(ctx, t_at)
+        levels = ctx.get_levels()
+    elif case == "kh":
         ctx, levels = kh_context(seed, precision=pyadg.ADG_FP64, ic=ic)
     else:
         ctx, levels = wave3d_context(seed, precision=pyadg.ADG_FP64)
@@ -115,9 +181,11 @@ def evaluate(par, seed, ic, P=3, case="kh"):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", choices=["kh", "wave3d"], default="kh")
+    ap.add_argument("--case", choices=["kh", "kh-job", "wave3d"], default="kh-job")
     ap.add_argument("--out", default=None)
     ap.add_argument("--pairs", type=int, default=2_000_000)
+    ap.add_argument("--band-pairs", type=int, default=0,
+                    help="kh-job: pairs from early states on the band hierarchy, after the job pairs")
     ap.add_argument("--every", type=int, default=20, help="steps between collections")
     ap.add_argument("--per-seed", type=int, default=9, help="collections per seed")
     ap.add_argument("--batch", type=int, default=8192)
@@ -126,19 +194,60 @@ def main():
     ap.add_argument("--heldout", default="4:1001,4:1002,1:20240611",
                     help="comma list of ic:seed evaluation runs (not used for training)")
     args = ap.parse_args()
-    kh = args.case == "kh"
+    kh = args.case in ("kh", "kh-job")
     P = 3 if kh else 2
     if args.out is None:
         args.out = os.path.join(HERE, "weights", "kh_p3_2d" if kh else "wave_p2_3d")
-    if not kh and args.heldout == ap.get_default("heldout"):
+    if args.case == "wave3d" and args.heldout == ap.get_default("heldout"):
         args.heldout = "5:1001,5:1002"
+    if args.case == "kh-job" and args.heldout == ap.get_default("heldout"):
+        args.heldout = "4:1001,7:1002"
 
     t0 = time.time()
-    ctx, levels = kh_context(1) if kh else wave3d_context(1)
-    ctx.mlp_init(2024)
     seed, n = 1, 0
     seeds = []
-    while n < args.pairs:
+    if args.case == "kh-job":
+        ctx = kh_job_context(1, 4)
+        ctx.mlp_init(2024)
+        state = {"n": 0}
+
+        def collect(c):
+            state["n"] = c.mlp_collect(2 * P + 2)
+
+        while state["n"] < args.pairs:
+            if seed > 1:
+                kh_job_reset(ctx, seed, 4 if seed % 2 else 7)
+            collect(ctx)
+            kh_job_advance(ctx, 1.0, on_regrid=lambda c: collect(c) if state["n"] < args.pairs else None)
+            seeds.append(seed)
+            print(f"seed {seed}: {state['n']} pairs ({time.time() - t0:.1f} s)", flush=True)
+            seed += 1
+        # early states on the fixed band hierarchy of the bench's kernel
+        # measurement, collected every --every steps from t = 0
+        levels = kh_bands(64, 3, 32)
+        target = state["n"] + args.band_pairs
+        while state["n"] < target:
+            ic = 4 if seed % 2 else 7
+            prm = [seed, 0.02] if ic == 4 else [0.01, 1e-3, 7000 + seed, 64]
+            ctx.set_levels(levels)
+            for l in range(len(levels)):
+                ctx.project_ic(l, ic, prm)
+            ctx.average_down()
+            ctx.fill_ghosts()
+            ctx.set_dt(C_DT, T_END, 0.0)
+            for _ in range(args.per_seed):
+                collect(ctx)
+                ctx.step_graph(C_DT, T_END, args.every)
+                if state["n"] >= target:
+                    break
+            seeds.append(seed)
+            print(f"seed {seed} (bands): {state['n']} pairs ({time.time() - t0:.1f} s)", flush=True)
+            seed += 1
+        n = state["n"]
+    else:
+        ctx, levels = kh_context(1) if kh else wave3d_context(1)
+        ctx.mlp_init(2024)
+    while args.case != "kh-job" and n < args.pairs:
         if not kh and seed > 1:
             wave3d_project(ctx, levels, seed)
         elif seed > 1:   # same hierarchy, next seed (the dataset stays on the device)
@@ -173,14 +282,21 @@ def main():
     evals = {}
     for hs in args.heldout.split(","):
         ic, sd = hs.split(":")
-        evals[hs] = evaluate(par, int(sd), int(ic), P=P, case=args.case)
-        print(f"held-out seed {hs}: " + ", ".join(f"{k} {v:.2e}" for k, v in evals[hs].items()), flush=True)
+        for t_at in ((0.0, 0.5, 0.9) if args.case == "kh-job" else (0.0,)):
+            key = hs if t_at == 0.0 else f"{hs}@t={t_at}"
+            evals[key] = evaluate(par, int(sd), int(ic), P=P, case=args.case, t_at=t_at)
+            print(f"held-out seed {key}: " + ", ".join(f"{k} {v:.2e}" for k, v in evals[key].items()), flush=True)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     par.astype(np.float32).tofile(args.out + ".f32")
     sz = ctx.mlp_sizes()
     meta = {"widths": sz, "pairs": n, "train_seeds": seeds, "collect_every_steps": args.every,
             "collect_iterations": 2 * P + 2, "batch": args.batch, "loss_history": hist, "heldout_distance": evals,
-            "workload": ("KH, odd seeds IC 4 (multi-mode, amax 0.02), even seeds IC 1 (single mode + noise), "
+            "workload": ("KH whole jobs t=0..1, odd seeds IC 4 (multi-mode, amax 0.02), even seeds IC 7 "
+                         "(single mode + fixed-grid noise), p=3, 3 tagged levels 64^2/128^2/256^2, regrid every "
+                         "0.05, TVB M=5000, collection after every regrid"
+                         + (f"; then {args.band_pairs} pairs from early states on the 3-level band hierarchy"
+                            if args.band_pairs else "") + ", FP32 form") if args.case == "kh-job" else
+                        ("KH, odd seeds IC 4 (multi-mode, amax 0.02), even seeds IC 1 (single mode + noise), "
                          "p=3, 3 levels 64^2/128^2/256^2 bands, FP32 form") if kh else
                         ("3D p=2, periodic [0,1]^3, 32^3 + centre half refined, density wave (IC 5) with "
                          "seeded amplitude and pressure, FP32 form"),
