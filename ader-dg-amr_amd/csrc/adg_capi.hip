@@ -1678,7 +1678,7 @@ static int launch_epilogue(adg_ctx* c, double c_dt, double T_end) {
         a.ad_coarse = c->ad_coarse[L - 1];
         a.ad_fine = c->ad_fine[L - 1];
         a.ngroup = (long)c->H.ad_coarse[L - 1].size();
-        a.nblk_a = nblocks(a.nrc * c->Q * c->tab.Np_s);
+        a.nblk_a = nblocks(adg::reflux_threads(a.nrc, c->Q, c->tab.Np_s, c->D));
         const unsigned nb = a.nblk_a + nblocks(adg::avg_threads(a.ngroup, c->Q, c->tab.Np_s, c->D));
         if (nb > 0) {
 #define L_RA(PP, DD) hipLaunchKernelGGL((adg::k_reflux_avg<T, PP, DD>), dim3(nb), dim3(256), 0, c->stream, a)

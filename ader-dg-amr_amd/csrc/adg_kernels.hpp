@@ -1173,16 +1173,12 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
 // J (+-1)^{k_d} S, fine faces J * P_flux_{high|low}[d][child] Fnum whose mode
 // 0 row equals the quadrature weights, so it is taken from S as well
 // (AmrDG.tpp:604-647).
+// Mode n of a register moment from the face's node fluxes F (already loaded).
 template <typename T, int P, int D>
-__device__ __forceinline__ double register_moment(const double* __restrict__ tab, const TabOff& to,
-                                                  const T* __restrict__ Fnum_d, long nf, int d, int kind, int sel,
-                                                  long f, int q, int n, double jac) {
+__device__ __forceinline__ double register_moment_of(const double* __restrict__ tab, const TabOff& to, const T* F,
+                                                     int d, int kind, int sel, int n, double jac) {
     using Dm = Dims<P, D>;
     constexpr int Ns = Dm::Ns, Mbd = Dm::Mbd, NFC = Dm::NFC;
-    T F[Mbd];
-    const T* __restrict__ pf = Fnum_d + (long)q * Mbd * nf + f;
-#pragma unroll
-    for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb * nf];
     const T* __restrict__ Mbp = moment_rows<T>(tab, to) + (long)d * Ns * Mbd;
     if (kind == 0 || n == 0) {
         const double sv = double(face_moment<T, Mbd>(Mbp + n * Mbd, F));
@@ -1193,6 +1189,18 @@ __device__ __forceinline__ double register_moment(const double* __restrict__ tab
 #pragma unroll
     for (int mb = 0; mb < Mbd; ++mb) s = fma(M[n * Mbd + mb], double(F[mb]), s);
     return jac * s;
+}
+
+template <typename T, int P, int D>
+__device__ __forceinline__ double register_moment(const double* __restrict__ tab, const TabOff& to,
+                                                  const T* __restrict__ Fnum_d, long nf, int d, int kind, int sel,
+                                                  long f, int q, int n, double jac) {
+    constexpr int Mbd = Dims<P, D>::Mbd;
+    T F[Mbd];
+    const T* __restrict__ pf = Fnum_d + (long)q * Mbd * nf + f;
+#pragma unroll
+    for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb * nf];
+    return register_moment_of<T, P, D>(tab, to, F, d, kind, sel, n, jac);
 }
 
 template <typename T>
@@ -1439,28 +1447,49 @@ struct EpiArgs {
     unsigned nblk_a;   // blocks of the first work type
 };
 
+// Threads of the reflux rows: 2D one per (row, q, n), a short latency-bound
+// item; 3D one per (row, q) for every mode, so that the node fluxes of each
+// register entry are gathered once, not once per mode (3D rows are many and
+// their entries' fluxes 27 scattered values each).
+__host__ __device__ __forceinline__ long reflux_threads(long nrc, int Q, int Ns, int D) {
+    return nrc * Q * (D == 3 ? 1 : Ns);
+}
+
+// Every mode sums its entries in row order with the same arithmetic in both
+// mappings.
 template <typename T, int P, int D>
 __device__ __forceinline__ void reflux_item(const EpiArgs<T>& a, long t) {
     using Dm = Dims<P, D>;
-    constexpr int Q = Dm::Q, Ns = Dm::Ns;
-    if (t >= a.nrc * Q * Ns) return;
+    constexpr int Q = Dm::Q, Ns = Dm::Ns, Mbd = Dm::Mbd;
+    constexpr int NPT = D == 3 ? Ns : 1;   // modes per thread
+    if (t >= a.nrc * Q * (Ns / NPT)) return;
     const int qn = (int)(t / a.nrc);
     const long r = t - (long)qn * a.nrc;
-    const int q = qn / Ns, n = qn - q * Ns;
+    const int q = NPT == Ns ? qn : qn / Ns, n0 = NPT == Ns ? 0 : qn - q * Ns;
     const long slot = a.rc_slot[r];
     const int lev = a.rc_lev[r];
     const double inv_jac = (double)(1 << D) / a.geo.vol[lev];
     const double Dt = a.ts->Dt;
-    double corr = 0.0;
+    double corr[NPT];
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) corr[j] = 0.0;
     const int st = a.rc_start[r], cnt = a.rc_count[r];
     for (int e = 0; e < cnt; ++e) {
         const long j = a.re_mom[st + e];
-        const int d = a.mom_d[j], ml = a.mom_lev[j];
+        const int d = a.mom_d[j], ml = a.mom_lev[j], kind = a.mom_kind[j], sel = a.mom_sel[j];
         const double jac = (Dt / 2.0) * (a.geo.area[ml][d] / (double)(1 << (D - 1)));
-        corr += a.re_coef[st + e] *
-                register_moment<T, P, D>(a.tab, a.to, a.Fnum[d], a.NF[d], d, a.mom_kind[j], a.mom_sel[j], a.mom_face[j], q, n, jac);
+        const double coef = a.re_coef[st + e];
+        T F[Mbd];
+        const T* __restrict__ pf = a.Fnum[d] + (long)q * Mbd * a.NF[d] + a.mom_face[j];
+#pragma unroll
+        for (int mb = 0; mb < Mbd; ++mb) F[mb] = pf[mb * a.NF[d]];
+#pragma unroll
+        for (int k = 0; k < NPT; ++k)
+            corr[k] += coef * register_moment_of<T, P, D>(a.tab, a.to, F, d, kind, sel, n0 + k, jac);
     }
-    a.Uw[((long)q * Ns + n) * a.NS + slot] += inv_jac * (a.tab[a.to.Minv + n] * corr);
+#pragma unroll
+    for (int k = 0; k < NPT; ++k)
+        a.Uw[((long)q * Ns + n0 + k) * a.NS + slot] += inv_jac * (a.tab[a.to.Minv + n0 + k] * corr[k]);
 }
 
 // Threads of the average down: groups padded to whole wavefronts, so that the

@@ -287,6 +287,43 @@ def test_fused_step_equals_unfused(adg, nlev, periodic, precision):
             assert np.array_equal(fused.download_Uw(l), split.download_Uw(l)), f"step {step} level {l}"
 
 
+@pytest.mark.parametrize("periodic", [[1, 1, 1], [0, 0, 0]])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_step_equals_unfused_3d(adg, periodic, precision):
+    """The same in 3D (p=2, three levels, the shock tube limited once):
+    the fused epilogue's reflux rows (one thread per row and variable for
+    every mode) equal the separate flux-register moments and reflux
+    bitwise."""
+    print(f"case fused vs unfused step 3D P=2 8^3 3 levels periodic={periodic} precision={precision}", flush=True)
+    levels = [tiles(8, 8, 3), tiles(16, 8, 3, [4, 4, 4], [11, 11, 11]),
+              tiles(32, 8, 3, [10, 10, 12], [21, 21, 19])]
+    ctxs = []
+    for _ in range(2):
+        ctx = adg.Context(3, 2, 1.4, precision)
+        ctx.set_geometry([8, 8, 8], [0, 0, 0], [1, 1, 1], periodic=periodic)
+        ctx.set_conservative_reflux(True)
+        ctx.set_levels(levels)
+        for l in range(3):
+            ctx.project_ic(l, 3, [0.3, 1.0, 1.0, 0.125, 0.1])
+        ctx.set_limiter(True, 0.0, [1.0] * 3, 1)
+        ctx.limit()
+        ctx.average_down()
+        ctx.fill_ghosts()
+        ctx.set_limiter(False, 0.0, [1.0] * 3, 1)
+        ctx.set_dt(0.5, 1.0, 0.0)
+        ctxs.append(ctx)
+    fused, split = ctxs
+    for step in range(3):
+        fused.step(0.5, 1.0)
+        split.ader()
+        split.average_down()
+        split.fill_ghosts()
+        split.advance(0.5, 1.0)
+        assert fused.get_time() == split.get_time(), f"step {step}"
+        for l in range(3):
+            assert np.array_equal(fused.download_Uw(l), split.download_Uw(l)), f"step {step} level {l}"
+
+
 def test_kh_outflow_y_three_levels(adg, orc):
     """Periodic in x, transmissive (mirror-image) ghosts at y = 0 and 1."""
     print("case KH P=3 16^2 3 levels, outflow in y", flush=True)
