@@ -106,6 +106,10 @@ struct adg_ctx {
     int* valid_list = nullptr;
     int* cell_face[3][2] = {};
     int* face_lr[3] = {};
+    int* face_tr[3] = {};     // face_lr with copy ghosts replaced by their sources (face flux)
+    int* pred_slots = nullptr;
+    int* cg_slot = nullptr;   // copy ghosts and their sources
+    int* cg_src = nullptr;
     int *g_slot = nullptr, *g_src = nullptr;
     uint8_t *g_kind = nullptr, *g_child = nullptr, *slot_lev = nullptr;
     int* ad_coarse[adg::kMaxLev] = {};
@@ -482,6 +486,7 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
         UP(c->cell_face[d][0], H.cell_face[d][0]);
         UP(c->cell_face[d][1], H.cell_face[d][1]);
         UP(c->face_lr[d], H.face_lr[d]);
+        UP(c->face_tr[d], H.face_tr[d]);
     }
     UP(c->g_slot, H.g_slot);
     UP(c->g_src, H.g_src);
@@ -503,6 +508,9 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     UP(c->re_mom, H.re_mom);
     UP(c->re_coef, H.re_coef);
     UP(c->uncovered, H.uncovered);
+    UP(c->pred_slots, H.pred_slots);
+    UP(c->cg_slot, H.copy_ghost);
+    UP(c->cg_src, H.copy_src);
     {
         // the limiter's cells (valid cells of the finest level) and their
         // neighbours per direction and side, a same-level ghost replaced by
@@ -611,6 +619,11 @@ int adg_level_nslot(adg_ctx* c, int lev) {
 int adg_level_nface(adg_ctx* c, int lev, int d) {
     if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev || d < 0 || d >= c->D) return ADG_E_ARG;
     return (int)c->H.lev[lev].nface[d];
+}
+
+int adg_level_npred(adg_ctx* c, int lev) {
+    if (!c || !c->levels_set || lev < 0 || lev >= c->H.nlev) return ADG_E_ARG;
+    return (int)(c->H.pred_off[lev + 1] - c->H.pred_off[lev]);
 }
 
 static int copy_level(adg_ctx* c, int lev, double* dev, int ncomp, double* host, bool up) {
@@ -823,7 +836,7 @@ int adg_set_dt_value(adg_ctx* c, double Dt) {
 
 extern "C++" {
 template <typename T, int P, int D>
-static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool learned) {
+static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool learned, const int* slots) {
     const int v = variant == PV_NL ? 1 : 0;
     adg::PredMArgs<T> a{};
     a.frags = (const T*)c->d_frag[v];
@@ -844,6 +857,7 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool le
     a.tol = c->learned_tol;
     a.learned_iters = c->learned_iters;
     a.stats = c->lstats;
+    a.slots = slots;
     const int wpb = variant == PV_NL ? c->nl_wpb : 8;
     const size_t lds = mfma_lds_bytes(c, variant, wpb);
     const void* kern = variant == PV_NL ? (wpb == 16 ? (const void*)adg::k_predictor_nl<T, P, D, 16>
@@ -907,7 +921,7 @@ static int mlp_warm_const_pad(const adg_ctx* c) {
 }
 
 template <int P, int D>
-static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
+static int launch_mlp_warm(adg_ctx* c, long begin, long n, const int* slots) {
     constexpr int Q = adg::Dims<P, D>::Q, Ns = adg::Dims<P, D>::Ns, Nst = adg::Dims<P, D>::Nst;
     constexpr int NINP = 16 * ((Q * Ns + 1 + D + 15) / 16), NOUTP = 16 * ((Q * Nst + 15) / 16);
     auto& m = c->mlp;
@@ -930,6 +944,7 @@ static int launch_mlp_warm(adg_ctx* c, long begin, long n) {
     a.nelem = n;
     a.gamma = c->gamma;
     a.warm = c->warm;
+    a.slots = slots;
     const size_t lds = (size_t)m.nfrag * 64 * sizeof(uint2) + adg::warm_const_bytes<NINP, NOUTP>();
     const int nt = c->warm_tiles;
     const void* kern = warm_kernel<P, D, NINP, NOUTP>(nt);
@@ -1014,7 +1029,7 @@ int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
         double wbest = 1e300;
         for (int nt = 0; nt <= 2 && rc == ADG_OK; ++nt) {
             c->warm_tiles = nt;
-#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, 0, c->H.NS)
+#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, 0, (long)c->H.pred_slots.size(), c->pred_slots)
             ADG_DISPATCH(c, L_MW);
             HIPCHK(c, hipEventRecord(e0, c->stream));
             for (int r = 0; r < reps && rc == ADG_OK; ++r) ADG_DISPATCH(c, L_MW);
@@ -1037,13 +1052,16 @@ int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
     return ADG_OK;
 }
 
+static int launch_predictor(adg_ctx* c, long begin, long n, int variant, bool learned);
+
 int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     if (mode != 0 && mode != 1) return fail(c, ADG_E_ARG, "mode must be 0 (plain) or 1 (learned warm start)");
     if (lev < -1 || lev >= c->H.nlev) return fail(c, ADG_E_ARG, "bad level");
     if (n_fallback) *n_fallback = 0;
-    long begin = 0, n = c->H.NS;
-    if (lev >= 0) { begin = c->H.lev[lev].slot_off; n = c->H.lev[lev].nslot; }
+    // elements: the predictor's slot list (every slot but the copy ghosts)
+    long begin = 0, n = (long)c->H.pred_slots.size();
+    if (lev >= 0) { begin = c->H.pred_off[lev]; n = c->H.pred_off[lev + 1] - begin; }
     if (n == 0) return ADG_OK;
     const int variant = pred_variant(c);
     const bool learned = mode == 1;
@@ -1053,32 +1071,45 @@ int adg_predictor(adg_ctx* c, int lev, int mode, int* n_fallback) {
         if (!c->mlp.init) return fail(c, ADG_E_STATE, "no network: adg_mlp_init / adg_mlp_set_params first");
         if ((rc = mlp_pack(c)) != ADG_OK) return rc;
         if (n_fallback) HIPCHK(c, hipMemsetAsync(c->lstats, 0, 2 * sizeof(unsigned long long), c->stream));
-#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, begin, n)
+#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, begin, n, c->pred_slots)
         ADG_DISPATCH(c, L_MW);
 #undef L_MW
         if (rc != ADG_OK) return rc;
     }
+    if ((rc = launch_predictor(c, begin, n, variant, learned)) != ADG_OK) return rc;
+    if (c->keep_hw && c->Hw_dbg && !c->H.copy_ghost.empty()) {
+        // the kept H_w of copy ghosts from their sources, for inspection
+        const long ng = (long)c->H.copy_ghost.size();
+        hipLaunchKernelGGL(adg::k_copy_slots, dim3(nblocks(ng * c->Q * c->tab.Np_st)), dim3(256), 0, c->stream,
+                           c->Hw_dbg, c->Q * c->tab.Np_st, c->H.NS, c->cg_slot, c->cg_src, ng);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (learned && n_fallback && variant != PV_VALU) {
+        unsigned long long st[2];
+        HIPCHK(c, hipMemcpyAsync(st, c->lstats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *n_fallback = (int)st[1];
+    }
+    return ADG_OK;
+}
+
+static int launch_predictor(adg_ctx* c, long begin, long n, int variant, bool learned) {
+    int rc = ADG_OK;
     if (variant != PV_VALU) {
         if (c->precision == ADG_FP64) {
-#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n, variant, learned)
+#define L_PM(PP, DD) rc = launch_pred_mfma<double, PP, DD>(c, begin, n, variant, learned, c->pred_slots)
             ADG_DISPATCH(c, L_PM);
 #undef L_PM
         } else {
-#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n, variant, learned)
+#define L_PM(PP, DD) rc = launch_pred_mfma<float, PP, DD>(c, begin, n, variant, learned, c->pred_slots)
             ADG_DISPATCH(c, L_PM);
 #undef L_PM
-        }
-        if (rc == ADG_OK && learned && n_fallback) {
-            unsigned long long st[2];
-            HIPCHK(c, hipMemcpyAsync(st, c->lstats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            *n_fallback = (int)st[1];
         }
         return rc;
     }
     adg::PredArgs a{c->d_tab, c->to, c->Uw, (double*)c->Vol, (double*)c->Tr, c->keep_hw ? c->Hw_dbg : nullptr,
                     c->slot_lev, c->ts, c->geo, c->H.NS, begin, n, c->gamma,
-                    c->iters_override > 0 ? c->iters_override : c->P};
+                    c->iters_override > 0 ? c->iters_override : c->P, c->pred_slots};
 #define L_PR(PP, DD)                                                                              \
     {                                                                                             \
         constexpr int epw = adg::Dims<PP, DD>::EPW;                                               \
@@ -1479,7 +1510,7 @@ int adg_mlp_warm_start(adg_ctx* c, int lev, double* host) {
     int rc = mlp_pack(c);
     if (rc != ADG_OK) return rc;
     const long begin = c->H.lev[lev].slot_off, n = c->H.lev[lev].nslot;
-#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, begin, n)
+#define L_MW(PP, DD) rc = launch_mlp_warm<PP, DD>(c, begin, n, nullptr)
     ADG_DISPATCH(c, L_MW);
 #undef L_MW
     if (rc != ADG_OK) return rc;
@@ -1525,11 +1556,11 @@ extern "C++" {
 template <typename T>
 static int launch_faceflux(adg_ctx* c) {
     adg::FaceArgs<T> a{};
-    a.Tr = (const T*)c->Tr;
+    a.Tr = (const T*)c->Tr;   // the faces' trace slots: copy ghosts replaced by their sources
     long tot = 0;
     for (int d = 0; d < c->D; ++d) {
         a.Fnum[d] = (T*)c->Fnum[d];
-        a.face_lr[d] = c->face_lr[d];
+        a.face_lr[d] = c->face_tr[d];
         a.NF[d] = c->H.NF[d];
         tot += c->H.NF[d];
     }

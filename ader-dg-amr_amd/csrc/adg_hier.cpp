@@ -558,6 +558,28 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             }(ivc);
         }
     }
+    // ---- predictor slots and trace sources (copy ghosts skipped)
+    {
+        std::vector<int> src_of((size_t)h.NS, -1);
+        for (size_t g = 0; g < h.g_slot.size(); ++g)
+            if ((h.g_kind[g] & 3) == 0 && (h.g_kind[g] >> 2) == 0) {
+                src_of[(size_t)h.g_slot[g]] = h.g_src[g];
+                h.copy_ghost.push_back(h.g_slot[g]);
+                h.copy_src.push_back(h.g_src[g]);
+            }
+        h.pred_off.assign((size_t)nlev + 1, 0);
+        for (int l = 0; l < nlev; ++l) {
+            h.pred_off[(size_t)l] = (long)h.pred_slots.size();
+            for (long s = h.lev[l].slot_off; s < h.lev[l].slot_off + h.lev[l].nslot; ++s)
+                if (src_of[(size_t)s] < 0) h.pred_slots.push_back((int)s);
+        }
+        h.pred_off[(size_t)nlev] = (long)h.pred_slots.size();
+        for (int d = 0; d < D; ++d) {
+            h.face_tr[d] = h.face_lr[d];
+            for (int& s : h.face_tr[d])
+                if (src_of[(size_t)s] >= 0) s = src_of[(size_t)s];
+        }
+    }
     return h;
 }
 

@@ -75,6 +75,17 @@ struct Hierarchy {
     // uncovered valid cells (for conserved totals)
     std::vector<int> uncovered;
 
+    // The predictor's slots: every slot but the copy ghosts (same-level or
+    // periodic copies, not mirrored), in slot order, level l's in
+    // [pred_off[l], pred_off[l+1]).  A copy ghost's predictor result equals
+    // its source cell's bit for bit (the predictor is element-local), so the
+    // face flux reads the source's traces (face_tr: face_lr with copy ghosts
+    // replaced by their sources) and the copy is never predicted.
+    std::vector<int> pred_slots;
+    std::vector<long> pred_off;
+    std::vector<int> face_tr[3];
+    std::vector<int> copy_ghost, copy_src;
+
     // face masks for tests (global face ids per direction)
     std::vector<int8_t> bcoarse[3], bfine[3], cidx[3];
 

@@ -181,11 +181,18 @@ struct PredArgs {
     const TimeState* __restrict__ ts;
     LevelGeo geo;
     long NS;
-    long slot_begin;
+    long slot_begin;   // with slots: the first list index, else the first slot
     long nelem;
     double gamma;
     int iters;   // Picard iterations (P)
+    const int* __restrict__ slots;   // nullable: the slots to predict (a list), else a slot range
 };
+
+// Slot of the e-th element of a predictor launch: from the slot list when
+// there is one, else from the contiguous range.
+__device__ __forceinline__ long pred_slot(const int* __restrict__ slots, long begin, long e) {
+    return slots ? (long)slots[begin + e] : begin + e;
+}
 
 template <int P, int D>
 __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
@@ -199,7 +206,7 @@ __global__ void __launch_bounds__(256) k_predictor(PredArgs a) {
     long e = wave * EPW + el;
     const bool active = lane_ok && e < a.nelem;
     if (e >= a.nelem) e = a.nelem - 1;
-    const long slot = a.slot_begin + e;
+    const long slot = pred_slot(a.slots, a.slot_begin, e);
     const int lane_base = lane_ok ? lane - q : lane;
     const int lev = a.slot_lev[slot];
     const double Dt = a.ts->Dt;
@@ -425,6 +432,7 @@ struct PredMArgs {
                                              // < 0: exactly learned_iters iterations, no check
     int learned_iters;                       // Picard iterations tried from the learned start
     unsigned long long* __restrict__ stats;  // [elements, elements that fell back]
+    const int* __restrict__ slots;           // nullable: the slots to predict, slot_begin its first index
 };
 
 template <typename T, int P, int D>
@@ -455,7 +463,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
     for (long it = 0; deal_next(deal, it, wave, EPM, nwave, &eb); ++it) {
         const long e = eb + e_loc;
         const bool active = col_ok && e < a.nelem;
-        const long slot = a.slot_begin + (e < a.nelem ? e : a.nelem - 1);
+        const long slot = pred_slot(a.slots, a.slot_begin, e < a.nelem ? e : a.nelem - 1);
         const int lev = a.slot_lev[slot];
         const double Dt = a.ts->Dt;
         T cd[D];
@@ -752,8 +760,8 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     long e0;
     for (long it = 0; deal_next(deal, it, wave, 4, nwave, &e0); ++it) {
         const bool active = e0 + ce < a.nelem;
-        const long slot = a.slot_begin + (active ? e0 + ce : a.nelem - 1);
-        const long slot_f = a.slot_begin + (e0 + g < a.nelem ? e0 + g : a.nelem - 1);
+        const long slot = pred_slot(a.slots, a.slot_begin, active ? e0 + ce : a.nelem - 1);
+        const long slot_f = pred_slot(a.slots, a.slot_begin, e0 + g < a.nelem ? e0 + g : a.nelem - 1);
         const int lev = a.slot_lev[slot_f];
         const double Dt = a.ts->Dt;
         T cd[D];
@@ -1166,6 +1174,17 @@ __global__ void __launch_bounds__(256) k_corrector(CorrArgs<T> a) {
     }
 #pragma unroll
     for (int k = 0; k < Ns; ++k) row_at(Uq + k * NS, sb) = r[k] / mk[k];
+}
+
+// H_w of the copy ghosts from their sources (the kept H_w only: the step
+// never reads a copy ghost's predictor result).  One thread per (ghost, row).
+__global__ void __launch_bounds__(256) k_copy_slots(double* __restrict__ buf, int nrow, long NS,
+                                                   const int* __restrict__ dst, const int* __restrict__ src, long n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nrow) return;
+    const int r = (int)(t / n);
+    const long g = t - (long)r * n;
+    buf[(long)r * NS + dst[g]] = buf[(long)r * NS + src[g]];
 }
 
 // ---------------------------------------------------------------------------

@@ -333,6 +333,7 @@ struct WarmArgs {
     long NS, slot_begin, nelem;
     double gamma;
     float* __restrict__ warm;          // [nout][NS]
+    const int* __restrict__ slots;     // nullable: the predictor's slot list, slot_begin its first index
 };
 
 // Warm-start constants, staged after the fragments: mu_x, 1/sd_x [nin_p],
@@ -420,7 +421,7 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
         for (int tt = 0; tt < NT; ++tt) {
             const long e = eb + (long)ELT * tt + ce;
             active[tt] = e < a.nelem && c < ELT;   // columns past ELT repeat, not stored
-            slot[tt] = a.slot_begin + (e < a.nelem ? e : a.nelem - 1);
+            slot[tt] = pred_slot(a.slots, a.slot_begin, e < a.nelem ? e : a.nelem - 1);
             levs[tt] = a.slot_lev[slot[tt]];
             const double* us = a.Uw + slot[tt];
 #pragma unroll

@@ -20,7 +20,7 @@ _LIB = None
 
 # every entry point declared in include/adg.h
 EXPORTS = [
-    "adg_create", "adg_set_geometry", "adg_set_levels", "adg_level_nslot", "adg_level_nface",
+    "adg_create", "adg_set_geometry", "adg_set_levels", "adg_level_nslot", "adg_level_nface", "adg_level_npred",
     "adg_upload_Uw", "adg_download_Uw", "adg_fill_ghosts", "adg_average_down", "adg_set_dt",
     "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
     "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_set_time", "adg_get_time",
@@ -52,6 +52,7 @@ def lib():
     L.adg_set_levels.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
     L.adg_level_nslot.argtypes = [C.c_void_p, C.c_int]
     L.adg_level_nface.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.adg_level_npred.argtypes = [C.c_void_p, C.c_int]
     L.adg_upload_Uw.argtypes = [C.c_void_p, C.c_int, _dp]
     L.adg_download_Uw.argtypes = [C.c_void_p, C.c_int, _dp]
     L.adg_fill_ghosts.argtypes = [C.c_void_p]
@@ -222,6 +223,10 @@ class Context:
 
     def nface(self, lev, d):
         return self.L.adg_level_nface(self.h, lev, d)
+
+    def npred(self, lev):
+        """Slots of level lev the predictor computes (valid cells and non-copy ghosts)."""
+        return self.L.adg_level_npred(self.h, lev)
 
     def upload_Uw(self, lev, U):
         U = np.ascontiguousarray(U, dtype=np.float64)

@@ -63,6 +63,14 @@ int adg_set_levels(adg_ctx* ctx, int nlev, const int* npatch_per_lev, const int*
 int adg_level_nslot(adg_ctx* ctx, int lev);
 /* Number of faces of direction d of a level (valid faces of all patches). */
 int adg_level_nface(adg_ctx* ctx, int lev, int d);
+/* Slots of level lev that the predictor computes: the valid cells and the
+ * ghosts that are not plain copies (coarse-fine projections, outflow
+ * mirrors).  A copy ghost's predictor result equals its source cell's bit
+ * for bit (the predictor is element-local), so the face flux reads the
+ * source's traces and the copy is skipped; the reference predicts every
+ * ghost (AmrDG.tpp:776).  The learned start's element counts
+ * (adg_learned_stats) count these slots. */
+int adg_level_npred(adg_ctx* ctx, int lev);
 
 /* Host <-> device copies of U_w for one level, layout [q][n][slot]. */
 int adg_upload_Uw(adg_ctx* ctx, int lev, const double* host);

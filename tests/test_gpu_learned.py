@@ -260,6 +260,6 @@ def test_nan_start_falls_back(adg):
     learned = g.download_Hw(0)
     g.predictor(-1, 0)
     plain = g.download_Hw(0)
-    print(f"  {nf} of {g.nslot(0)} elements fell back", flush=True)
-    assert nf == g.nslot(0)
+    print(f"  {nf} of {g.npred(0)} elements fell back", flush=True)
+    assert nf == g.npred(0)
     assert np.array_equal(learned, plain)

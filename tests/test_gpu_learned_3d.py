@@ -170,8 +170,8 @@ def test_learned_3d_nan_start_falls_back(adg):
     learned = [g.download_Hw(l) for l in range(len(levels))]
     g.predictor(-1, 0)
     plain = [g.download_Hw(l) for l in range(len(levels))]
-    nslots = sum(g.nslot(l) for l in range(len(levels)))
-    print(f"  {nf} of {nslots} elements fell back", flush=True)
-    assert nf == nslots
+    npred = sum(g.npred(l) for l in range(len(levels)))
+    print(f"  {nf} of {npred} elements fell back", flush=True)
+    assert nf == npred
     for l in range(len(levels)):
         assert np.array_equal(learned[l], plain[l]), f"level {l}"
