@@ -511,31 +511,9 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     UP(c->pred_slots, H.pred_slots);
     UP(c->cg_slot, H.copy_ghost);
     UP(c->cg_src, H.copy_src);
-    {
-        // the limiter's cells (valid cells of the finest level) and their
-        // neighbours per direction and side, a same-level ghost replaced by
-        // its source cell (FillBoundary's copy)
-        const adg::LevelLayout& fl = H.lev[nlev - 1];
-        const long n = fl.valid_end - fl.valid_begin;
-        std::vector<int> resolve((size_t)H.NS, -1);
-        for (size_t g = 0; g < H.g_slot.size(); ++g)
-            if (H.g_kind[g] == 0 && H.slot_lev[H.g_slot[g]] == nlev - 1) resolve[H.g_slot[g]] = H.g_src[g];
-        std::vector<int> cells((size_t)n), nbr((size_t)2 * D * n);
-        for (long i = 0; i < n; ++i) {
-            const long v = fl.valid_begin + i;
-            cells[i] = H.valid_list[v];
-            for (int d = 0; d < D; ++d)
-                for (int side = 0; side < 2; ++side) {
-                    const int f = H.cell_face[d][side][v];
-                    int sl = H.face_lr[d][2 * (size_t)f + (side == 0 ? 0 : 1)];
-                    if (resolve[sl] >= 0) sl = resolve[sl];
-                    nbr[(size_t)(2 * d + side) * n + i] = sl;
-                }
-        }
-        c->lim_n = n;
-        UP(c->lim_cells, cells);
-        UP(c->lim_nbr, nbr);
-    }
+    c->lim_n = (long)H.lim_cells.size();
+    UP(c->lim_cells, H.lim_cells);
+    UP(c->lim_nbr, H.lim_nbr);
 #undef UP
     const size_t NS = (size_t)H.NS;
     if ((rc = dev_alloc(c, &c->Uw, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
@@ -549,22 +527,7 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
     if ((rc = dev_alloc_bytes(c, (void**)&c->warm, (size_t)Q * t.Np_st * NS * sizeof(float))) != ADG_OK) return rc;
-    {
-        // patches are laid out in slot order: the patch of a slot is the last
-        // one whose offset does not exceed it
-        std::vector<double> xc(H.uncovered.size() * 3, 0.0);
-        for (size_t i = 0; i < H.uncovered.size(); ++i) {
-            const long s = H.uncovered[i];
-            auto it = std::upper_bound(H.patch.begin(), H.patch.end(), s,
-                                       [](long v, const adg::PatchLayout& pt) { return v < pt.off; });
-            const adg::PatchLayout& pt = *(it - 1);
-            const long loc = s - pt.off;
-            const long r[3] = {loc % pt.g[0], (loc / pt.g[0]) % pt.g[1], loc / ((long)pt.g[0] * pt.g[1])};
-            for (int d = 0; d < D; ++d)
-                xc[i * 3 + d] = H.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * H.lev[pt.lev].dx[d];
-        }
-        if ((rc = dev_upload(c, &c->unc_centre, xc)) != ADG_OK) return rc;
-    }
+    if ((rc = dev_upload(c, &c->unc_centre, H.unc_centre)) != ADG_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->levels_set = true;
     rc = prepare_kernels(c);
@@ -1963,9 +1926,14 @@ struct OwnerMap {
             n *= dom[d];
         }
         own.assign((size_t)n, -1);
-        const std::vector<std::array<int, 3>> cells = level_valid_cells(H, lev);
-        const long vb = H.lev[lev].valid_begin;
-        for (size_t i = 0; i < cells.size(); ++i) own[(size_t)lin(cells[i].data())] = H.valid_list[(size_t)(vb + (long)i)];
+        const int pb = H.lev[lev].p_begin;
+        adg::parallel_for(H.lev[lev].p_end - pb, [&](int i) {
+            const adg::PatchLayout& P = H.patch[(size_t)(pb + i)];
+            int iv[3] = {0, 0, 0};
+            for (iv[2] = P.lo[2]; iv[2] <= (D > 2 ? P.hi[2] : P.lo[2]); ++iv[2])
+                for (iv[1] = P.lo[1]; iv[1] <= P.hi[1]; ++iv[1])
+                    for (iv[0] = P.lo[0]; iv[0] <= P.hi[0]; ++iv[0]) own[(size_t)lin(iv)] = (int)H.slot_of(pb + i, iv);
+        });
     }
     long lin(const int* iv) const {
         long k = 0;
@@ -2199,16 +2167,21 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
         const bool had = l < old.nlev;
         if (had) old_own.build(old, l);
         crse_own.build(H, l - 1);
-        std::vector<int> es, esrc;
-        std::vector<uint8_t> ek, ech;
-        for (int p = H.lev[l].p_begin; p < H.lev[l].p_end && err.empty(); ++p) {
-            const adg::PatchLayout& P = H.patch[p];
+        // every slot of the level (ghosts included), in slot order: each
+        // patch fills its own range
+        const long base = H.lev[l].slot_off, nsl = H.lev[l].nslot;
+        std::vector<int> es((size_t)nsl), esrc((size_t)nsl);
+        std::vector<uint8_t> ek((size_t)nsl), ech((size_t)nsl);
+        std::atomic<bool> nested{true};
+        adg::parallel_for(H.lev[l].p_end - H.lev[l].p_begin, [&](int i) {
+            const adg::PatchLayout& P = H.patch[(size_t)(H.lev[l].p_begin + i)];
             int iv[3] = {0, 0, 0};
             long s = P.off;
             const int z0 = D > 2 ? P.lo[2] - 1 : 0, z1 = D > 2 ? P.hi[2] + 1 : 0;
             for (iv[2] = z0; iv[2] <= z1; ++iv[2])
                 for (iv[1] = P.lo[1] - 1; iv[1] <= P.hi[1] + 1; ++iv[1])
                     for (iv[0] = P.lo[0] - 1; iv[0] <= P.hi[0] + 1; ++iv[0], ++s) {
+                        const size_t k = (size_t)(s - base);
                         int mv[3] = {iv[0], iv[1], iv[2]}, mask = 0;
                         for (int d = 0; d < D; ++d) {
                             const int n = H.lev[l].dom[d];
@@ -2217,11 +2190,11 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
                             if (iv[d] >= n) { mv[d] = 2 * n - 1 - iv[d]; mask |= 1 << d; }
                         }
                         const int os = had ? old_own.find(mv) : -1;
-                        es.push_back((int)s);
+                        es[k] = (int)s;
                         if (os >= 0) {
-                            esrc.push_back(os);
-                            ek.push_back((uint8_t)(mask << 2));
-                            ech.push_back(0);
+                            esrc[k] = os;
+                            ek[k] = (uint8_t)(mask << 2);
+                            ech[k] = 0;
                             continue;
                         }
                         int civ[3] = {0, 0, 0}, child = 0;
@@ -2229,13 +2202,14 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
                             civ[d] = floordiv2(mv[d]);
                             child += (mv[d] - 2 * civ[d]) << (D - 1 - d);
                         }
-                        const int cs = crse_own.find(civ);
-                        if (cs < 0) { err = "new level " + std::to_string(l) + " is not nested in level " + std::to_string(l - 1); break; }
-                        esrc.push_back(cs);
-                        ek.push_back((uint8_t)(1 | (mask << 2)));
-                        ech.push_back((uint8_t)child);
+                        int cs = crse_own.find(civ);
+                        if (cs < 0) { nested = false; cs = 0; }   // reported below, nothing is launched
+                        esrc[k] = cs;
+                        ek[k] = (uint8_t)(1 | (mask << 2));
+                        ech[k] = (uint8_t)child;
                     }
-        }
+        });
+        if (!nested) err = "new level " + std::to_string(l) + " is not nested in level " + std::to_string(l - 1);
         if (!err.empty()) break;
         const long ne = (long)es.size();
         hipError_t e = hipMalloc((void**)&d_slot, ne * sizeof(int));

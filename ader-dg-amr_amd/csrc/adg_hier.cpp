@@ -2,13 +2,9 @@
 #include "adg_hier.hpp"
 
 #include <algorithm>
-#include <atomic>
-#include <exception>
 #include <map>
-#include <mutex>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <unordered_map>
 
 namespace adg {
@@ -41,38 +37,6 @@ struct Grid {  // owner map of one level over its domain: cell -> (patch, slot)
 };
 
 int floordiv2(int i) { return (i >= 0) ? i / 2 : -((-i + 1) / 2); }
-
-// f(i) for i in [0, n) on up to 16 host threads, items handed out one at a
-// time; the first exception thrown by any item is rethrown.
-template <class F>
-void parallel_for(int n, F&& f) {
-    int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-    nt = std::min(nt, n);
-    if (nt <= 1) {
-        for (int i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::atomic<int> next{0};
-    std::exception_ptr err;
-    std::mutex m;
-    auto work = [&]() {
-        for (;;) {
-            const int i = next.fetch_add(1);
-            if (i >= n) return;
-            try {
-                f(i);
-            } catch (...) {
-                std::lock_guard<std::mutex> g(m);
-                if (!err) err = std::current_exception();
-            }
-        }
-    };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) th.emplace_back(work);
-    work();
-    for (std::thread& t : th) t.join();
-    if (err) std::rethrow_exception(err);
-}
 
 template <class F>
 void for_box(int D, const int* lo, const int* hi, F&& f) {
@@ -579,6 +543,44 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             for (int& s : h.face_tr[d])
                 if (src_of[(size_t)s] >= 0) s = src_of[(size_t)s];
         }
+        // the limiter's cells and neighbours (FillBoundary's copies resolved)
+        const LevelLayout& fl = h.lev[nlev - 1];
+        const long n = fl.valid_end - fl.valid_begin;
+        h.lim_cells.assign((size_t)n, 0);
+        h.lim_nbr.assign((size_t)2 * D * n, 0);
+        const int nchunk = (int)std::min<long>(256, std::max<long>(1, n / 4096));
+        parallel_for(nchunk, [&](int k) {
+            for (long i = n * k / nchunk; i < n * (k + 1) / nchunk; ++i) {
+                const long v = fl.valid_begin + i;
+                h.lim_cells[(size_t)i] = h.valid_list[(size_t)v];
+                for (int d = 0; d < D; ++d)
+                    for (int side = 0; side < 2; ++side) {
+                        const int f = h.cell_face[d][side][(size_t)v];
+                        int sl = h.face_lr[d][2 * (size_t)f + side];
+                        if (src_of[(size_t)sl] >= 0) sl = src_of[(size_t)sl];
+                        h.lim_nbr[(size_t)(2 * d + side) * n + i] = sl;
+                    }
+            }
+        });
+    }
+    // centres of the uncovered cells; patches are in slot order, so a slot's
+    // patch is the last one whose offset does not exceed it
+    {
+        const long nu = (long)h.uncovered.size();
+        h.unc_centre.assign((size_t)nu * 3, 0.0);
+        const int nchunk = (int)std::min<long>(256, std::max<long>(1, nu / 4096));
+        parallel_for(nchunk, [&](int k) {
+            for (long i = nu * k / nchunk; i < nu * (k + 1) / nchunk; ++i) {
+                const long s = h.uncovered[(size_t)i];
+                auto it = std::upper_bound(h.patch.begin(), h.patch.end(), s,
+                                           [](long v, const PatchLayout& pt) { return v < pt.off; });
+                const PatchLayout& pt = *(it - 1);
+                const long loc = s - pt.off;
+                const long r[3] = {loc % pt.g[0], (loc / pt.g[0]) % pt.g[1], loc / ((long)pt.g[0] * pt.g[1])};
+                for (int d = 0; d < D; ++d)
+                    h.unc_centre[(size_t)i * 3 + d] = h.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * h.lev[pt.lev].dx[d];
+            }
+        });
     }
     return h;
 }

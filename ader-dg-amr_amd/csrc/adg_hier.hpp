@@ -16,10 +16,47 @@
 //     cf_face_b_coarse ownership rule, AmrDG_Level.tpp:240-446).
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace adg {
+
+// f(i) for i in [0, n) on up to 16 host threads, items handed out one at a
+// time; the first exception thrown by any item is rethrown.
+template <class F>
+inline void parallel_for(int n, F&& f) {
+    int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    nt = std::min(nt, n);
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::exception_ptr err;
+    std::mutex m;
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n) return;
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(m);
+                if (!err) err = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (std::thread& t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
 
 struct LevelLayout {
     int dom[3] = {1, 1, 1};
@@ -85,6 +122,13 @@ struct Hierarchy {
     std::vector<long> pred_off;
     std::vector<int> face_tr[3];
     std::vector<int> copy_ghost, copy_src;
+
+    // the limiter's cells (valid cells of the finest level) and their
+    // same-level neighbours per direction and side, a copy ghost replaced by
+    // its source cell ([2d + side][cell]); centres of the uncovered cells
+    // ([cell][3])
+    std::vector<int> lim_cells, lim_nbr;
+    std::vector<double> unc_centre;
 
     // face masks for tests (global face ids per direction)
     std::vector<int8_t> bcoarse[3], bfine[3], cidx[3];
