@@ -359,13 +359,15 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     });
     // ---- flux-register algebra (FluxRegister CrseAdd(-1) / FineAdd(+1) /
     // Reflux(vol=1) as called in AmrDG.h:1138-1153 and AmrDG_Level.tpp:416-446)
-    std::unordered_map<uint64_t, int> mom_id;  // (kind, d, face) -> index
+    // (kind, d, face) -> moment index, a face-indexed table per kind and
+    // direction (-1: none yet)
+    std::vector<int> mom_id[2][3];
     auto moment = [&](int kind, int d, long face, int sel, int lev) {
-        const uint64_t key = ((uint64_t)kind << 62) | ((uint64_t)d << 60) | (uint64_t)face;
-        auto it = mom_id.find(key);
-        if (it != mom_id.end()) return it->second;
+        std::vector<int>& tab = mom_id[kind][d];
+        if (tab.empty()) tab.assign((size_t)h.NF[d], -1);
+        if (tab[(size_t)face] >= 0) return tab[(size_t)face];
         int id = (int)h.mom_face.size();
-        mom_id.emplace(key, id);
+        tab[(size_t)face] = id;
         h.mom_face.push_back((int)face);
         h.mom_d.push_back((uint8_t)d);
         h.mom_kind.push_back((uint8_t)kind);
@@ -419,6 +421,16 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     int tlo[3] = {clo[0], clo[1], clo[2]}, thi[3] = {chi[0], chi[1], chi[2]};
                     tlo[d] = thi[d] = side == 0 ? clo[d] : chi[d] + 1;
                     for_box(D, tlo, thi, [&](const int* fv) {
+                        // the coarse cell across this face, outside patch B:
+                        // covered by the fine level (another fine patch) or
+                        // outside a non-periodic domain, the face has no
+                        // register terms (both sides covered, or no fine face
+                        // flagged)
+                        {
+                            int nb[3] = {fv[0], fv[1], D > 2 ? fv[2] : 0};
+                            if (side == 0) nb[d] -= 1;
+                            if (!wrap(lc, nb) || covered(lc, nb)) return;
+                        }
                         reg.clear();
                         coarse_arrays(lc, d, fv, arrs);
                         for (auto& a : arrs) {
@@ -540,8 +552,13 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         h.pred_off[(size_t)nlev] = (long)h.pred_slots.size();
         for (int d = 0; d < D; ++d) {
             h.face_tr[d] = h.face_lr[d];
-            for (int& s : h.face_tr[d])
-                if (src_of[(size_t)s] >= 0) s = src_of[(size_t)s];
+            std::vector<int>& ft = h.face_tr[d];
+            const long nft = (long)ft.size();
+            const int nch = (int)std::min<long>(256, std::max<long>(1, nft / 65536));
+            parallel_for(nch, [&](int k) {
+                for (long i = nft * k / nch; i < nft * (k + 1) / nch; ++i)
+                    if (src_of[(size_t)ft[(size_t)i]] >= 0) ft[(size_t)i] = src_of[(size_t)ft[(size_t)i]];
+            });
         }
         // the limiter's cells and neighbours (FillBoundary's copies resolved)
         const LevelLayout& fl = h.lev[nlev - 1];

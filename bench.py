@@ -83,8 +83,10 @@ class Job:
     regrid, the end time or a step budget.  dof counts valid elements x Q x
     Np_st for every step taken."""
 
-    def __init__(self, ctx, D, c_dt, T, regrid_dt=None, regrid_steps=None, max_batch=64):
+    def __init__(self, ctx, D, c_dt, T, regrid_dt=None, regrid_steps=None, max_batch=64, name=None):
         self.ctx, self.D, self.c_dt, self.T = ctx, D, c_dt, T
+        self.name = name
+        self.last_log = time.perf_counter()
         self.regrid_dt, self.regrid_steps, self.max_batch = regrid_dt, regrid_steps, max_batch
         self.t, self.Dt = ctx.get_time()
         self.steps = 0
@@ -136,6 +138,9 @@ class Job:
             self.since_rg += batch
             self.dof += batch * self.nvalid * self.per_elem
             self.t, self.Dt = self.ctx.get_time()
+            if self.name and time.perf_counter() - self.last_log > 20:
+                log(f"{self.name}: t={self.t:.4f} steps={self.steps} elements={self.nvalid} Dt={self.Dt:.2e}")
+                self.last_log = time.perf_counter()
         return taken
 
 
@@ -255,6 +260,7 @@ def main():
     ap.add_argument("--shock-steps", type=int, default=128,
                     help="3D shock tube: steps timed after 16 warmup steps (0 with --shock-T: the whole job)")
     ap.add_argument("--shock-T", type=float, default=0.25, help="3D shock tube: end time of a whole-job run")
+    ap.add_argument("--shock-forms", default="fp64,fp32,learned", help="3D shock tube: the forms to run")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the predictor measurement on the band hierarchy (the profiled command)")
     ap.add_argument("--shock-only", action="store_true", help="only the 3D shock-tube measurement")
@@ -465,7 +471,7 @@ def main():
         w3 = np.fromfile(os.path.join(PKG, "weights", "wave_p2_3d.f32"), dtype=np.float32)
         out = {"workload": "3D spherical shock tube p=2, outflow, 3 tagged levels 64^3/128^3/256^3, "
                            "regrid every 16 steps, TVB M=0"}
-        for form in ("fp64", "fp32", "learned"):
+        for form in args.shock_forms.split(","):
             ctx = build_tagged(pyadg, 3, 2, prec_of[form], 64, 3, SHOCK, [0, 0, 0], 32, 0.2, 0.0, True)
             if form == "learned":
                 ctx.mlp_set_params(w3)
@@ -473,7 +479,8 @@ def main():
             ctx.set_dt(c_dt, args.shock_T, 0.0)
             ctx.autotune(3)
             ctx.regrid_stats(reset=True)
-            job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64)
+            job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64,
+                      name=f"shock tube {form}")
             m0 = ctx.totals()
             s = ctx.stream()
             job.run(16)
@@ -498,10 +505,13 @@ def main():
             out[form] = r
             log(f"shock tube {form}: {r['ms_per_step']:.1f} ms/step, t={r['t']:.4f}, elements {r['elements_end']}")
             ctx.close()
-        out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
-        out["learned_over_fp64"] = out["learned"]["value"] / out["fp64"]["value"]
-        out["note"] = ("mass drift through the open boundary: the FP32 forms' round-off momentum of the state at "
-                       "rest (about 1e-8 of the sound speed) crosses it; periodic runs conserve to 1e-15")
+        if "fp64" in out:
+            for f in ("fp32", "learned"):
+                if f in out:
+                    out[f"{f}_over_fp64"] = out[f]["value"] / out["fp64"]["value"]
+        out["note"] = ("mass drift includes the flux through the open boundaries: the shock reaches them at "
+                       "t ~ 0.07 and leaves; before that, only the FP32 forms' round-off momentum of the state at "
+                       "rest (about 1e-8 of the sound speed) crosses them; periodic runs conserve to 1e-15")
         return out
 
     if args.shock_only:
