@@ -141,6 +141,11 @@ struct adg_ctx {
     bool independent = false;  // adg_set_levels builds a batch of independent periodic patches
     bool conservative_reflux = false;   // reflux every cell the registers reach (not the reference's rule)
     bool cfl_uncovered = false;         // set_Dt over uncovered cells only (not the reference's rule)
+    // per-step scratch (volume terms, traces, face fluxes, warm-start
+    // increments): kept across layouts and grown when a layout needs more,
+    // so that a regrid allocates nothing of this size
+    struct Scratch { void* p = nullptr; size_t cap = 0; };
+    Scratch sc_vol, sc_tr, sc_fnum[3], sc_warm;
     // host seconds spent in adg_regrid / adg_remake, cumulative: tags, grid
     // generation, hierarchy lists, device set-up (uploads, allocation),
     // data transfer onto the new layout; and the number of layout changes
@@ -212,6 +217,36 @@ int dev_alloc_bytes(adg_ctx* c, void** dst, size_t bytes) {
     *dst = p;
     return ADG_OK;
 }
+// Scratch of at least `bytes`, zeroed: the kept buffer when it is large
+// enough, else a new one with a quarter more room for the next layouts.
+int scratch_bytes(adg_ctx* c, adg_ctx::Scratch& sc, void** dst, size_t bytes) {
+    *dst = nullptr;
+    if (bytes == 0) return ADG_OK;
+    if (sc.cap < bytes) {
+        if (sc.p) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            (void)hipFree(sc.p);
+            sc.p = nullptr;
+            sc.cap = 0;
+        }
+        size_t cap = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&sc.p, cap);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            cap = bytes;
+            e = hipMalloc(&sc.p, cap);
+        }
+        if (e != hipSuccess) {
+            sc.p = nullptr;
+            return fail(c, ADG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        sc.cap = cap;
+    }
+    HIPCHK(c, hipMemsetAsync(sc.p, 0, bytes, c->stream));
+    *dst = sc.p;
+    return ADG_OK;
+}
+
 int dev_alloc(adg_ctx* c, double** dst, size_t n) {
     void* p = nullptr;
     int rc = dev_alloc_bytes(c, &p, n * sizeof(double));
@@ -518,15 +553,17 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     const size_t NS = (size_t)H.NS;
     if ((rc = dev_alloc(c, &c->Uw, (size_t)Q * t.Np_s * NS)) != ADG_OK) return rc;
     const size_t ves = (size_t)Q * t.Np_s, tes = (size_t)2 * D * t.qMp_st_bd * Q;   // per slot
-    if ((rc = dev_alloc_bytes(c, &c->Vol, ves * NS * c->esz)) != ADG_OK) return rc;
-    if ((rc = dev_alloc_bytes(c, &c->Tr, tes * NS * c->esz)) != ADG_OK) return rc;
+    if ((rc = scratch_bytes(c, c->sc_vol, &c->Vol, ves * NS * c->esz)) != ADG_OK) return rc;
+    if ((rc = scratch_bytes(c, c->sc_tr, &c->Tr, tes * NS * c->esz)) != ADG_OK) return rc;
     for (int d = 0; d < D; ++d)
-        if ((rc = dev_alloc_bytes(c, &c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d] * c->esz)) != ADG_OK)
+        if ((rc = scratch_bytes(c, c->sc_fnum[d], (void**)&c->Fnum[d], (size_t)Q * t.qMp_st_bd * H.NF[d] * c->esz)) !=
+            ADG_OK)
             return rc;
     if ((rc = dev_alloc(c, &c->Mom, (size_t)Q * t.Np_s * H.mom_face.size())) != ADG_OK) return rc;
     if ((rc = dev_alloc(c, &c->partial, (size_t)Q * nblocks((long)H.uncovered.size()) + 1)) != ADG_OK) return rc;
     if (c->keep_hw && (rc = dev_alloc(c, &c->Hw_dbg, (size_t)Q * t.Np_st * NS)) != ADG_OK) return rc;
-    if ((rc = dev_alloc_bytes(c, (void**)&c->warm, (size_t)Q * t.Np_st * NS * sizeof(float))) != ADG_OK) return rc;
+    if ((rc = scratch_bytes(c, c->sc_warm, (void**)&c->warm, (size_t)Q * t.Np_st * NS * sizeof(float))) != ADG_OK)
+        return rc;
     if ((rc = dev_upload(c, &c->unc_centre, H.unc_centre)) != ADG_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->levels_set = true;
@@ -2458,6 +2495,8 @@ void adg_destroy(adg_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_levels(c);
+    for (adg_ctx::Scratch* sc : {&c->sc_vol, &c->sc_tr, &c->sc_fnum[0], &c->sc_fnum[1], &c->sc_fnum[2], &c->sc_warm})
+        if (sc->p) (void)hipFree(sc->p);
     if (c->d_tab) (void)hipFree(c->d_tab);
     for (int v = 0; v < 2; ++v)
         if (c->d_frag[v]) (void)hipFree(c->d_frag[v]);
