@@ -301,18 +301,24 @@ __global__ void __launch_bounds__(64) k_mlp_collect(CollectArgs a) {
     for (int f = 0; f < Q * Ns; ++f) U[f] = a.Uw[(long)f * a.NS + slot];
     double x[NIN];
     mlp_features<P, D>(U, tau, a.gamma, x);
-    float* xr = a.X + i * a.nin_p;
-    for (int f = 0; f < a.nin_p; ++f) xr[f] = f < NIN ? (float)x[f] : 0.f;
     const double* A0 = a.tab + a.to.A0;
     const double sc = 1.0 / (U[0] * tmax);
-    float* yr = a.Y + i * a.nout_p;
+    double y[Q * Nst];
     for (int q = 0; q < Q; ++q)
         for (int n = 0; n < Nst; ++n) {
             double c0 = 0.0;
             for (int m = 0; m < Ns; ++m) c0 += A0[n * Ns + m] * U[q * Ns + m];
-            yr[q * Nst + n] = (float)((a.Hw[((long)q * Nst + n) * a.NS + slot] - c0) * sc);
+            y[q * Nst + n] = (a.Hw[((long)q * Nst + n) * a.NS + slot] - c0) * sc;
         }
-    for (int o = Q * Nst; o < a.nout_p; ++o) yr[o] = 0.f;
+    // a row with a value that is not a finite float (a predictor that did
+    // not converge on a non-physical state) is stored as zeros
+    bool ok = true;
+    for (int f = 0; f < NIN; ++f) ok = ok && fabs(x[f]) < 1e30;
+    for (int o = 0; o < Q * Nst; ++o) ok = ok && fabs(y[o]) < 1e30;
+    float* xr = a.X + i * a.nin_p;
+    for (int f = 0; f < a.nin_p; ++f) xr[f] = (ok && f < NIN) ? (float)x[f] : 0.f;
+    float* yr = a.Y + i * a.nout_p;
+    for (int o = 0; o < a.nout_p; ++o) yr[o] = (ok && o < Q * Nst) ? (float)y[o] : 0.f;
 }
 
 // ---------------------------------------------------------------------------

@@ -23,6 +23,13 @@ states (a collection every 20 steps from t = 0) on the fixed band hierarchy
 of the bench's kernel measurement; held-out seeds are evaluated at t = 0, 0.5 and
 0.9 of their own jobs.
 
+With --case shock3d (weights/shock_p2_3d): the 3D p=2 network for shock
+tubes, from spherical shock tubes with seeded radius and states on the
+outflow [0,1]^3, three levels 16^3 -> 64^3 (--shock-n0) from density-gradient tags
+(0.2), a regrid every 16 steps, the TVB limiter (M = 0) on the finest level
+and on the initial state, the composite rules, one collection after every
+regrid over the first --shock-steps steps.
+
 With --case wave3d: the 3D p=2 network, from runs of the smooth density wave
 (initial condition 5) with seeded amplitude and pressure on the periodic
 [0,1]^3 with two levels (32^3, the centre half refined).
@@ -82,6 +89,66 @@ def kh_context(seed, P=3, n0=64, nlev=3, precision=pyadg.ADG_FP32_MFMA, ic=4):
     ctx.fill_ghosts()
     ctx.set_dt(C_DT, T_END, 0.0)
     return ctx, levels
+
+
+SHOCK_N0 = [16]   # level-0 cells per direction of the shock3d runs
+
+
+def shock3d_params(seed):
+    """Spherical shock tube (IC 3) with seeded radius and states."""
+    rng = np.random.default_rng(9700 + seed)
+    return [rng.uniform(0.25, 0.4), rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0),
+            rng.uniform(0.08, 0.3), rng.uniform(0.05, 0.3)]
+
+
+def shock3d_reset(ctx, seed, n0=None):
+    """Three tagged levels from the seed's initial state (analytic on each,
+    limited), in place (the dataset on the device is kept)."""
+    from layouts import tiles
+    n0 = n0 or SHOCK_N0[0]
+    prm = shock3d_params(seed)
+    ctx.set_levels([tiles(n0, 32, 3)])
+    ctx.project_ic(0, 3, prm)
+    ctx.fill_ghosts()
+    ctx.set_limiter(True, 0.0, [1.0] * 3, 1)
+    for _ in range(2):
+        ctx.limit()
+        ctx.average_down()
+        ctx.fill_ghosts()
+        ctx.regrid(0.0)
+        for l in range(len(ctx.get_levels())):
+            ctx.project_ic(l, 3, prm)
+        ctx.average_down()
+        ctx.fill_ghosts()
+    ctx.limit()
+    ctx.average_down()
+    ctx.fill_ghosts()
+    ctx.set_dt(C_DT, 1.0, 0.0)
+
+
+def shock3d_context(seed, precision=pyadg.ADG_FP32_MFMA, n0=None):
+    n0 = n0 or SHOCK_N0[0]
+    ctx = pyadg.Context(3, 2, 1.4, precision)
+    ctx.set_geometry([n0] * 3, [0] * 3, [1] * 3, [0, 0, 0])
+    ctx.set_conservative_reflux(True)
+    ctx.set_cfl_uncovered(True)
+    ctx.set_amr(2, max_grid_size=32)
+    ctx.set_tagging(2, [0.2])
+    shock3d_reset(ctx, seed, n0)
+    return ctx
+
+
+def shock3d_advance(ctx, nsteps, on_regrid=None):
+    """nsteps steps with a regrid every 16; on_regrid(ctx) after each."""
+    done = 0
+    while done < nsteps:
+        n = min(16, nsteps - done)
+        ctx.step_graph(C_DT, 1.0, n)
+        done += n
+        t, _ = ctx.get_time()
+        ctx.regrid(t)
+        if on_regrid:
+            on_regrid(ctx)
 
 
 def kh_job_context(seed, ic, precision=pyadg.ADG_FP32_MFMA, n0=64):
@@ -153,6 +220,11 @@ def evaluate(par, seed, ic, P=3, case="kh", t_at=0.0):
         if t_at > 0:
             kh_job_advance(ctx, t_at)
         levels = ctx.get_levels()
+    elif case == "shock3d":
+        ctx = shock3d_context(seed, precision=pyadg.ADG_FP64)
+        if t_at > 0:
+            shock3d_advance(ctx, int(t_at))
+        levels = ctx.get_levels()
     elif case == "kh":
         ctx, levels = kh_context(seed, precision=pyadg.ADG_FP64, ic=ic)
     else:
@@ -181,7 +253,9 @@ def evaluate(par, seed, ic, P=3, case="kh", t_at=0.0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", choices=["kh", "kh-job", "wave3d"], default="kh-job")
+    ap.add_argument("--case", choices=["kh", "kh-job", "wave3d", "shock3d"], default="kh-job")
+    ap.add_argument("--shock-steps", type=int, default=128, help="shock3d: steps per seed")
+    ap.add_argument("--shock-n0", type=int, default=16, help="shock3d: level-0 cells per direction")
     ap.add_argument("--out", default=None)
     ap.add_argument("--pairs", type=int, default=2_000_000)
     ap.add_argument("--band-pairs", type=int, default=0,
@@ -197,7 +271,10 @@ def main():
     kh = args.case in ("kh", "kh-job")
     P = 3 if kh else 2
     if args.out is None:
-        args.out = os.path.join(HERE, "weights", "kh_p3_2d" if kh else "wave_p2_3d")
+        args.out = os.path.join(HERE, "weights", "kh_p3_2d" if kh else
+                                "shock_p2_3d" if args.case == "shock3d" else "wave_p2_3d")
+    if args.case == "shock3d" and args.heldout == ap.get_default("heldout"):
+        args.heldout = "3:1001,3:1002"
     if args.case == "wave3d" and args.heldout == ap.get_default("heldout"):
         args.heldout = "5:1001,5:1002"
     if args.case == "kh-job" and args.heldout == ap.get_default("heldout"):
@@ -206,7 +283,27 @@ def main():
     t0 = time.time()
     seed, n = 1, 0
     seeds = []
-    if args.case == "kh-job":
+    if args.case == "shock3d":
+        SHOCK_N0[0] = args.shock_n0
+        ctx = shock3d_context(1)
+        ctx.mlp_init(2024)
+        state = {"n": 0}
+
+        def collect3(c):
+            if state["n"] < args.pairs:
+                state["n"] = c.mlp_collect(2 * P + 2)
+
+        while state["n"] < args.pairs:
+            if seed > 1:
+                shock3d_reset(ctx, seed)
+            # no collection of the initial jump: its Picard iteration has not
+            # converged after 2P + 2 iterations, so its targets are not H_w*
+            shock3d_advance(ctx, args.shock_steps, on_regrid=collect3)
+            seeds.append(seed)
+            print(f"seed {seed}: {state['n']} pairs ({time.time() - t0:.1f} s)", flush=True)
+            seed += 1
+        n = state["n"]
+    elif args.case == "kh-job":
         ctx = kh_job_context(1, 4)
         ctx.mlp_init(2024)
         state = {"n": 0}
@@ -247,7 +344,7 @@ def main():
     else:
         ctx, levels = kh_context(1) if kh else wave3d_context(1)
         ctx.mlp_init(2024)
-    while args.case != "kh-job" and n < args.pairs:
+    while args.case not in ("kh-job", "shock3d") and n < args.pairs:
         if not kh and seed > 1:
             wave3d_project(ctx, levels, seed)
         elif seed > 1:   # same hierarchy, next seed (the dataset stays on the device)
@@ -282,8 +379,9 @@ def main():
     evals = {}
     for hs in args.heldout.split(","):
         ic, sd = hs.split(":")
-        for t_at in ((0.0, 0.5, 0.9) if args.case == "kh-job" else (0.0,)):
-            key = hs if t_at == 0.0 else f"{hs}@t={t_at}"
+        for t_at in ((0.0, 0.5, 0.9) if args.case == "kh-job" else (0.0, 64, 128) if args.case == "shock3d"
+                     else (0.0,)):
+            key = hs if t_at == 0.0 else (f"{hs}@step={int(t_at)}" if args.case == "shock3d" else f"{hs}@t={t_at}")
             evals[key] = evaluate(par, int(sd), int(ic), P=P, case=args.case, t_at=t_at)
             print(f"held-out seed {key}: " + ", ".join(f"{k} {v:.2e}" for k, v in evals[key].items()), flush=True)
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
@@ -298,6 +396,9 @@ def main():
                             if args.band_pairs else "") + ", FP32 form") if args.case == "kh-job" else
                         ("KH, odd seeds IC 4 (multi-mode, amax 0.02), even seeds IC 1 (single mode + noise), "
                          "p=3, 3 levels 64^2/128^2/256^2 bands, FP32 form") if kh else
+                        ("3D p=2 spherical shock tubes (IC 3) with seeded radius U[0.25,0.4] and states, outflow "
+                         f"[0,1]^3, 3 tagged levels from {args.shock_n0}^3, regrid every 16 steps, TVB M=0, collection "
+                         f"after every regrid over {args.shock_steps} steps, FP32 form") if args.case == "shock3d" else
                         ("3D p=2, periodic [0,1]^3, 32^3 + centre half refined, density wave (IC 5) with "
                          "seeded amplitude and pressure, FP32 form"),
             "seconds": time.time() - t0}

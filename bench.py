@@ -251,6 +251,8 @@ def main():
                     help="learned form: accept the warm start when max|H_k - H_k-1| <= tol * rho0")
     ap.add_argument("--max-iters", type=int, default=2, help="learned form: Picard iterations from the warm start")
     ap.add_argument("--weights", default=os.path.join(PKG, "weights", "kh_p3_2d.f32"))
+    ap.add_argument("--weights3d", default=os.path.join(PKG, "weights", "wave_p2_3d.f32"),
+                    help="3D shock tube: the learned form's network")
     ap.add_argument("--no-ref", action="store_true", help="skip the finer FP64 reference run (no L1 error)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -468,7 +470,7 @@ def main():
         on the initial state, composite rules on: 16 warmup steps then
         --shock-steps timed steps (or, with --shock-steps 0, the whole job to
         --shock-T), each form."""
-        w3 = np.fromfile(os.path.join(PKG, "weights", "wave_p2_3d.f32"), dtype=np.float32)
+        w3 = np.fromfile(args.weights3d, dtype=np.float32)
         out = {"workload": "3D spherical shock tube p=2, outflow, 3 tagged levels 64^3/128^3/256^3, "
                            "regrid every 16 steps, TVB M=0"}
         for form in args.shock_forms.split(","):
