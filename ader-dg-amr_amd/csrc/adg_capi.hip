@@ -265,10 +265,16 @@ size_t mfma_lds_bytes(const adg_ctx* c, int variant, int wpb = 8) {
         const size_t xch = c->esz == sizeof(double) ? 0 : (size_t)8 * c->frags[0].MB2 * 256;
         return ((size_t)c->frags[0].total * 64 + xch) * c->esz;
     }
-    const size_t tst = c->esz == sizeof(double) ? adg::nl_tile_stride<double>() : adg::nl_tile_stride<float>();
-    return ((size_t)c->frags[1].total * 64 + (size_t)wpb * c->D * 16 * tst) * c->esz;
+    const bool f64 = c->esz == sizeof(double);
+    const size_t tst = f64 ? adg::nl_tile_stride<double>() : adg::nl_tile_stride<float>();
+    const size_t tdir = f64 ? adg::nl_tile_dirs<double>(c->D) : adg::nl_tile_dirs<float>(c->D);
+    return ((size_t)c->frags[1].total * 64 + (size_t)wpb * tdir * 16 * tst) * c->esz;
 }
-bool nl_available(const adg_ctx* c, int wpb) { return c->Q == 4 && mfma_lds_bytes(c, PV_NL, wpb) <= kLdsMax; }
+// elements per wavefront of the node-lane predictor: 4 (Q = 4) or 3 (Q = 5)
+inline int nl_elems(const adg_ctx* c) { return c->Q == 4 ? 4 : 3; }
+bool nl_available(const adg_ctx* c, int wpb) {
+    return (c->D == 2 || c->P <= 2) && mfma_lds_bytes(c, PV_NL, wpb) <= kLdsMax;
+}
 bool quad_available(const adg_ctx* c) { return mfma_lds_bytes(c, PV_QUAD) <= kLdsMax; }
 // Predictor kernel of the next launch.
 int pred_variant(const adg_ctx* c) {
@@ -587,7 +593,7 @@ static int prepare_pred(adg_ctx* c) {
         return ADG_OK;
     };
     int rc = setup(adg::predictor_mfma_kernel<T, P, D>(), 512, mfma_lds_bytes(c, PV_QUAD), c->pred_per_cu[0]);
-    if (rc == ADG_OK && c->Q == 4) {
+    if (rc == ADG_OK && (D == 2 || P <= 2)) {
         rc = setup((const void*)adg::k_predictor_nl<T, P, D, 8>, 512, mfma_lds_bytes(c, PV_NL, 8), c->pred_per_cu[1]);
         if (rc == ADG_OK)
             rc = setup((const void*)adg::k_predictor_nl<T, P, D, 16>, 1024, mfma_lds_bytes(c, PV_NL, 16),
@@ -865,7 +871,7 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool le
                                         : adg::predictor_mfma_kernel<T, P, D>();
     const int per_cu = c->pred_per_cu[variant == PV_NL ? (wpb == 16 ? 2 : 1) : 0];
     if (per_cu <= 0) return fail(c, ADG_E_UNSUP, "predictor workgroup does not fit a CU");
-    const long per_batch = (long)wpb * (variant == PV_NL ? 4 : adg::Dims<P, D>::EPM);
+    const long per_batch = (long)wpb * (variant == PV_NL ? nl_elems(c) : adg::Dims<P, D>::EPM);
     const long nbatch = (n + per_batch - 1) / per_batch;
     // persistent grid: exactly the workgroups that are resident at once
     // (registers and LDS both limit it), so no workgroup waits for a CU
@@ -883,7 +889,7 @@ int adg_set_predictor_kernel(adg_ctx* c, int kind) {
     if (kind == 1 && c->precision != ADG_FP64) return fail(c, ADG_E_UNSUP, "the VALU predictor is FP64 only");
     if (kind == 2 && !quad_available(c)) return fail(c, ADG_E_UNSUP, "MFMA fragments exceed LDS");
     if (kind == 3 && !nl_available(c, c->nl_wpb))
-        return fail(c, ADG_E_UNSUP, "the node-lane predictor needs Q = 4 and its fragments in LDS");
+        return fail(c, ADG_E_UNSUP, "the node-lane predictor's fragments and tiles exceed the LDS");
     c->pred_kind = kind;
     drop_graph(c);
     return ADG_OK;

@@ -378,6 +378,13 @@ template <typename T>
 __host__ __device__ constexpr int nl_tile_stride() {
     return sizeof(T) == 8 ? 18 : 20;
 }
+// Directions the tile holds at once: all D, except FP64 in 3D, where one
+// direction at a time keeps the p = 2 fragments and eight tiles within the
+// LDS of a CU.
+template <typename T>
+__host__ __device__ constexpr int nl_tile_dirs(int D) {
+    return (sizeof(T) == 8 && D == 3) ? 1 : D;
+}
 
 // Workgroup copy of nbytes (a multiple of 16) from global memory to LDS with
 // eight 16-byte loads in flight per thread before their stores, so that the
@@ -434,6 +441,18 @@ struct PredMArgs {
     unsigned long long* __restrict__ stats;  // [elements, elements that fell back]
     const int* __restrict__ slots;           // nullable: the slots to predict, slot_begin its first index
 };
+
+// Quad-gather K step r of node block nb contracts nodes 16 nb + acc_row(k, r)
+// (lane groups k); it holds a node when the smallest of them is below Mst.
+// Steps without one multiply zero fragments by zero fluxes; they are skipped
+// where the last block holds at most 4 nodes (3D p = 2: the 81st node is
+// alone in the sixth block), elsewhere the branches cost more registers
+// than the few steps they save.
+template <typename T, int Mst>
+__device__ __forceinline__ bool quad_kstep_live(int nb, int r) {
+    if constexpr (Mst % 16 == 0 || Mst % 16 > 4) return true;
+    else return 16 * nb + acc_row<T>(0, r) < Mst;
+}
 
 template <typename T, int P, int D>
 __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
@@ -524,6 +543,11 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                 T gd[D][4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
+                    if (!quad_kstep_live<T, Mst>(nb, r)) {
+#pragma unroll
+                        for (int d = 0; d < D; ++d) gd[d][r] = T(0);
+                        continue;
+                    }
                     T s[Q];
                     gather_state<Q>(h[r], src_base, s);
                     const bool ok = col_ok && (16 * nb + acc_row<T>(g, r)) < Mst;
@@ -534,13 +558,15 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                     for (int d = 0; d < D; ++d) gd[d][r] = ok ? cd[d] * flux_q<D>(d, q, s, h[r], rinv, prs) : T(0);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < 4; ++r) {
+                    if (!quad_kstep_live<T, Mst>(nb, r)) continue;
 #pragma unroll
                     for (int d = 0; d < D; ++d)
 #pragma unroll
                         for (int mb = 0; mb < MB2; ++mb)
                             acc[mb] = M::mma(fr[(a.fo.B + ((mb * D + d) * NB + nb) * 4 + r) * 64 + lane], gd[d][r],
                                              acc[mb]);
+                }
             }
             relay(acc);
         };
@@ -630,6 +656,11 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
             T gd[D][4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                if (!quad_kstep_live<T, Mst>(nb, r)) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) gd[d][r] = T(0);
+                    continue;
+                }
                 T s[Q];
                 gather_state<Q>(h[r], src_base, s);
                 const bool ok = col_ok && (16 * nb + acc_row<T>(g, r)) < Mst;
@@ -640,13 +671,15 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                 for (int d = 0; d < D; ++d) gd[d][r] = ok ? cd[d] * flux_q<D>(d, q, s, h[r], rinv, prs) : T(0);
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < 4; ++r) {
+                if (!quad_kstep_live<T, Mst>(nb, r)) continue;
 #pragma unroll
                 for (int d = 0; d < D; ++d)
 #pragma unroll
                     for (int mbs = 0; mbs < MBS; ++mbs)
                         vol[mbs] = M::mma(fr[(a.fo.S + ((mbs * D + d) * NB + nb) * 4 + r) * 64 + lane], gd[d][r],
                                           vol[mbs]);
+            }
         }
         if (active) {
 #pragma unroll
@@ -709,10 +742,15 @@ inline const void* predictor_mfma_kernel() {
 }
 
 // ---------------------------------------------------------------------------
-// Node-lane MFMA predictor (Q = 4).  A wavefront owns 4 elements; its 16
-// matrix columns are the (element, variable) pairs, column acc_row<T>(e, q),
+// Node-lane MFMA predictor.  Q = 4 (2D): a wavefront owns 4 elements; its
+// 16 matrix columns are the (element, variable) pairs, column acc_row<T>(e, q),
 // so that the nodal product with swapped operands leaves the 4 variables of
-// one (element, node) in the 4 accumulator registers of one lane:
+// one (element, node) in the 4 accumulator registers of one lane.  Q = 5
+// (3D): a wavefront owns 3 elements; lane group e < 3 holds variables 0-3 of
+// element e (column acc_row<T>(e, q)), lane group 3 holds the energies of
+// the 3 elements (column acc_row<T>(3, e)) and one padding column; a lane
+// of group e fetches its energy from lane group 3 (one shuffle per node and
+// element) and evaluates the flux of (element e, node) once:
 //   C0    = A0 U_w                         rows modes, columns (e, q)
 //   H^T   = H_w^T V^T                      lane (e = g, node = c), register q
 //   g_d   = (Dt/dx_d) F_d(H)               once per (element, node)
@@ -739,7 +777,10 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     // disjoint node sets, added at the end of the iteration
     constexpr bool TAIL = sizeof(T) == 4 && MB2 >= 2 && Nst - 16 * (MB2 - 1) == 4;
     constexpr int MBF = TAIL ? MB2 - 1 : MB2;
-    static_assert(Q == 4, "the node-lane layout keeps the 4 variables of a node in one lane");
+    constexpr int EW = Q == 4 ? 4 : 3;   // elements per wavefront
+    constexpr int TD = nl_tile_dirs<T>(D);
+    static_assert(Q == 4 || Q == 5, "node-lane layout for 2D and 3D Euler");
+    static_assert(D % TD == 0, "tile rounds cover the directions");
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T* fr = reinterpret_cast<T*>(smem_raw);
     const int nfr = a.fo.total * 64;
@@ -750,16 +791,21 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     if (threadIdx.x == 0) { wg_stats[0] = 0ull; wg_stats[1] = 0ull; }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    T* tile = fr + nfr + wave * (D * 16 * TST);
+    T* tile = fr + nfr + wave * (TD * 16 * TST);
     const int g = lane >> 4, c = lane & 15;
-    // column c of the modal data holds variable cq of element ce
-    const int ce = sizeof(T) == 8 ? (c & 3) : (c >> 2);
-    const int cq = sizeof(T) == 8 ? (c >> 2) : (c & 3);
+    // column c of the modal data is row acc_row<T>(cg, cr) of the nodal
+    // accumulator and holds variable cq of element ce (ce = 3: padding)
+    const int cg = sizeof(T) == 8 ? (c & 3) : (c >> 2);
+    const int cr = sizeof(T) == 8 ? (c >> 2) : (c & 3);
+    const int ce = (Q == 4 || cg < 3) ? cg : cr;
+    const int cq = (Q == 4 || cg < 3) ? cr : 4;
+    // column of variable q of element e
+    auto col_of = [](int e, int q) { return (Q == 4 || q < 4) ? acc_row<T>(e, q) : acc_row<T>(3, e); };
     const T gm1 = T(a.gamma - 1.0);
-    const Deal deal = make_deal(a.nelem, 4, nwave);
+    const Deal deal = make_deal(a.nelem, EW, nwave);
     long e0;
-    for (long it = 0; deal_next(deal, it, wave, 4, nwave, &e0); ++it) {
-        const bool active = e0 + ce < a.nelem;
+    for (long it = 0; deal_next(deal, it, wave, EW, nwave, &e0); ++it) {
+        const bool active = ce < EW && e0 + ce < a.nelem;
         const long slot = pred_slot(a.slots, a.slot_begin, active ? e0 + ce : a.nelem - 1);
         const long slot_f = pred_slot(a.slots, a.slot_begin, e0 + g < a.nelem ? e0 + g : a.nelem - 1);
         const int lev = a.slot_lev[slot_f];
@@ -788,33 +834,54 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
         for (int ks = 0; ks < KSH; ++ks) hb[ks] = ks < KSU ? hu[ks] : T(0);
 
         // nodal state of node block nb -> (Dt/dx_d) F_d in the LDS tile
-        // [d][column][node]; returns the B operands of the 4 K steps
+        // [d][column][node] (TD directions per round); returns the B
+        // operands of the 4 K steps
         auto nodal_flux = [&](int nb, T (&gv)[D][4]) {
             v4 h = v4{0, 0, 0, 0};
 #pragma unroll
             for (int ks = 0; ks < KSH; ++ks) h = M::mma(hb[ks], fr[(a.fo.V + nb * KSH + ks) * 64 + lane], h);
-            T s[Q] = {h[0], h[1], h[2], h[3]};
-            const bool ok = FULL || (16 * nb + c < Mst);
+            T s[Q];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[q] = h[q];
+            bool ok = FULL || (16 * nb + c < Mst);
+            if constexpr (Q == 5) {
+                // the energy of element g is register g of lane group 3
+                const T en0 = __shfl(h[0], 48 + c, 64), en1 = __shfl(h[1], 48 + c, 64);
+                const T en2 = __shfl(h[2], 48 + c, 64);
+                s[4] = g == 0 ? en0 : (g == 1 ? en1 : en2);
+                ok = ok && g < 3;
+            }
             if (!ok) s[0] = T(1);
             const T rinv = recip(s[0]);
             const T prs = pressure<D>(s, rinv, gm1);
+            T f[D][Q];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const T ud = s[d + 1] * rinv;
-                T f[Q];
-                f[0] = s[d + 1];
+                f[d][0] = s[d + 1];
 #pragma unroll
-                for (int b = 0; b < D; ++b) f[b + 1] = ud * s[b + 1] + ((b == d) ? prs : T(0));
-                f[D + 1] = (s[D + 1] + prs) * ud;
-#pragma unroll
-                for (int q = 0; q < Q; ++q) tile[(d * 16 + acc_row<T>(g, q)) * TST + c] = ok ? cd[d] * f[q] : T(0);
+                for (int b = 0; b < D; ++b) f[d][b + 1] = ud * s[b + 1] + ((b == d) ? prs : T(0));
+                f[d][D + 1] = (s[D + 1] + prs) * ud;
             }
-            wave_lds_sync();
 #pragma unroll
-            for (int d = 0; d < D; ++d)
+            for (int d0 = 0; d0 < D; d0 += TD) {
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4) gv[d][s4] = tile[(d * 16 + c) * TST + 4 * g + s4];
-            wave_lds_sync();
+                for (int dd = 0; dd < TD; ++dd) {
+                    const int d = d0 + dd;
+                    if (Q == 4 || g < 3) {
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) tile[(dd * 16 + col_of(g, q)) * TST + c] = ok ? cd[d] * f[d][q] : T(0);
+                    } else {
+                        tile[(dd * 16 + acc_row<T>(3, 3)) * TST + c] = T(0);   // padding column
+                    }
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int dd = 0; dd < TD; ++dd)
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) gv[d0 + dd][s4] = tile[(dd * 16 + c) * TST + 4 * g + s4];
+                wave_lds_sync();
+            }
         };
 
         // one Picard iteration H_w <- C0 + sum_d B_d g_d(H_w)
@@ -828,7 +895,8 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
                 T gv[D][4];
                 nodal_flux(nb, gv);
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4)
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    if (!FULL && 16 * nb + s4 >= Mst) continue;   // K step without a node (3D p = 2: 81 = 5 x 16 + 1)
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
 #pragma unroll
@@ -838,6 +906,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
                         if constexpr (TAIL)
                             tacc = M::mma4(fr[(a.fo.BT + (d * NB + nb) * 4 + s4) * 64 + lane], gv[d][s4], tacc);
                     }
+                }
             }
             if constexpr (TAIL) {
                 // rows 16(MB2-1) + i summed over the 4 lane groups; register 0
@@ -888,12 +957,23 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
 #pragma unroll
                     for (int ks = 0; ks < KSH; ++ks) r = nanmax(r, T(fabs(hb[ks] - prev[ks])));
                     r *= inv_sc;
-                    // max over the 16 lanes of the element: its 4 columns x 4 lane groups
-                    constexpr int QX0 = sizeof(T) == 8 ? 4 : 1, QX1 = sizeof(T) == 8 ? 8 : 2;
-                    r = nanmax(r, __shfl_xor(r, QX0, 64));
-                    r = nanmax(r, __shfl_xor(r, QX1, 64));
-                    r = nanmax(r, __shfl_xor(r, 16, 64));
-                    r = nanmax(r, __shfl_xor(r, 32, 64));
+                    if constexpr (Q == 4) {
+                        // max over the 16 lanes of the element: its 4 columns x 4 lane groups
+                        constexpr int QX0 = sizeof(T) == 8 ? 4 : 1, QX1 = sizeof(T) == 8 ? 8 : 2;
+                        r = nanmax(r, __shfl_xor(r, QX0, 64));
+                        r = nanmax(r, __shfl_xor(r, QX1, 64));
+                        r = nanmax(r, __shfl_xor(r, 16, 64));
+                        r = nanmax(r, __shfl_xor(r, 32, 64));
+                    } else {
+                        // max over the 4 lane groups, then over the element's 5 columns
+                        r = nanmax(r, __shfl_xor(r, 16, 64));
+                        r = nanmax(r, __shfl_xor(r, 32, 64));
+                        const int eq = ce < EW ? ce : 0;
+                        T re = r;
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) re = nanmax(re, __shfl(r, (lane & 48) | col_of(eq, q), 64));
+                        r = ce < EW ? re : T(0);   // the padding column always passes
+                    }
                     if (pass_e) {
 #pragma unroll
                         for (int ks = 0; ks < KSH; ++ks) hb[ks] = prev[ks];
@@ -937,13 +1017,15 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
             T gv[D][4];
             nodal_flux(nb, gv);
 #pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
+            for (int s4 = 0; s4 < 4; ++s4) {
+                if (!FULL && 16 * nb + s4 >= Mst) continue;
 #pragma unroll
                 for (int d = 0; d < D; ++d)
 #pragma unroll
                     for (int mbs = 0; mbs < MBS; ++mbs)
                         vol[mbs] = M::mma(fr[(a.fo.S + ((mbs * D + d) * NB + nb) * 4 + s4) * 64 + lane], gv[d][s4],
                                           vol[mbs]);
+            }
         }
         if (active) {
 #pragma unroll
@@ -990,7 +1072,8 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
 
 template <typename T, int P, int D, int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_predictor_nl(PredMArgs<T> a) {
-    if constexpr (Dims<P, D>::Q == 4) predictor_nl_body<T, P, D>(a);
+    // 3D p = 3 fragments exceed the LDS: no body
+    if constexpr (D == 2 || P <= 2) predictor_nl_body<T, P, D>(a);
 }
 
 // ---------------------------------------------------------------------------

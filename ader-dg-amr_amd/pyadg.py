@@ -239,7 +239,7 @@ class Context:
         return out
 
     def set_predictor_kernel(self, kind):
-        """0 automatic, 1 VALU (FP64), 2 MFMA quad-gather, 3 MFMA node-lane (2D)."""
+        """0 automatic, 1 VALU (FP64), 2 MFMA quad-gather, 3 MFMA node-lane (2D; 3D p <= 2)."""
         self._chk(self.L.adg_set_predictor_kernel(self.h, kind), "adg_set_predictor_kernel")
 
     def set_predictor_waves(self, waves):

@@ -46,7 +46,7 @@ KH_SEED = 20240611
 # hierarchy (the kernel measurement below), from rocprofv3 --pmc FETCH_SIZE
 # and --pmc WRITE_SIZE scaled by the counters' reading of a plain 256 MiB
 # device copy (x2.0 reads, x1.0 writes)
-PRED_TRAFFIC = {"fp64": 147.2e6, "fp32": 53.1e6, "learned": 53.1e6}
+PRED_TRAFFIC = {"fp64": 147.0e6, "fp32": 49.6e6, "learned": 49.6e6}
 SHOCK = [0.4, 1.0, 1.0, 0.125, 0.1]
 
 
@@ -479,7 +479,7 @@ def main():
                 ctx.mlp_set_params(w3)
                 ctx.set_learned(True, tol=args.tol, max_iters=args.max_iters)
             ctx.set_dt(c_dt, args.shock_T, 0.0)
-            ctx.autotune(3)
+            shapes, choice = ctx.autotune(3)
             ctx.regrid_stats(reset=True)
             job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64,
                       name=f"shock tube {form}")
@@ -500,7 +500,7 @@ def main():
                  "value": (job.dof - d0) / (ms * 1e-3), "elements_end": job.nvalid, "elements_max": job.max_nvalid,
                  "regrids": job.regrids, "regrid_host_s": ctx.regrid_stats(),
                  "mass_drift": abs(m1[0] - m0[0]) / m0[0], "energy_drift": abs(m1[-1] - m0[-1]) / m0[-1],
-                 "Dt": job.Dt}
+                 "Dt": job.Dt, "predictor_shapes_ms": shapes, "predictor_choice": choice}
             if form == "learned":
                 ne, nf = ctx.learned_stats()
                 r["fallback_fraction"] = nf / max(ne, 1)

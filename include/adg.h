@@ -168,15 +168,16 @@ int adg_set_dt_value(adg_ctx* ctx, double Dt);
  *   until max|H_k - H_{k-1}| <= tol rho0 (adg_set_learned, rho0 the element's
  *   mean density, so tol is relative to the state); an element that does
  *   not pass is recomputed by mode 0 (the fallback), one that passes keeps
- *   the iterate it passed with.  MFMA predictors (node-lane in 2D,
- *   quad-gather in 2D and 3D).
+ *   the iterate it passed with.  MFMA predictors (node-lane and
+ *   quad-gather, 2D and 3D).
  * n_fallback (nullable) receives the number of elements that fell back
  * (0 in mode 0). */
 int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
 /* Predictor kernel variant: 0 = automatic, 1 = VALU (FP64 only), 2 = MFMA
- * quad-gather (every Q), 3 = MFMA node-lane (Q = 4, i.e. 2D).  Automatic
- * picks node-lane where it applies, else quad-gather, else VALU; FP64 at
- * p = 1 uses VALU.  No counterpart in the reference: a tuning knob of this
+ * quad-gather (every Q), 3 = MFMA node-lane (2D, and 3D at p <= 2: four
+ * elements per wavefront for Q = 4, three for Q = 5).  Automatic picks
+ * node-lane where its fragments and tiles fit the LDS, else quad-gather,
+ * else VALU; FP64 at p = 1 uses VALU.  No counterpart in the reference: a tuning knob of this
  * implementation. */
 int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
 /* Autotuner: times the predictor launch shapes that fit this context and

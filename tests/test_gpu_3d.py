@@ -62,7 +62,7 @@ def test_3d_ic_matches_oracle(adg, orc):
 
 
 @pytest.mark.parametrize("ic", ["wave", "shock"])
-@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("P", [1, 2])
 def test_3d_predictor(adg, orc, P, kind, ic):
     print(f"case 3D predictor {ic} P={P} 8^3 kind={kind}", flush=True)
@@ -78,7 +78,7 @@ def test_3d_predictor(adg, orc, P, kind, ic):
     assert e < (TOL if ic == "wave" else TOL_SHOCK)
 
 
-@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("kind", [1, 2, 3])
 def test_3d_evolve_single_level(adg, orc, kind):
     print(f"case 3D evolve 4 steps wave P=2 8^3 (8 patches) kind={kind}", flush=True)
     levels = single_level(8, 4, D=3)
@@ -132,13 +132,15 @@ def test_3d_p3_two_levels(adg, orc):
     assert abs(m1[0] - m0[0]) <= 1e-13 * abs(m0[0])
 
 
-def test_3d_fp32_form(adg, orc):
-    """FP32-MFMA contractions on the 3D two-level hierarchy: per-variable error
-    against the FP64 oracle after 3 steps below 1e-6 of the state scale,
-    mass and energy conserved to 1e-13."""
-    print("case 3D FP32 form two levels wave P=2", flush=True)
+@pytest.mark.parametrize("kind,waves", [(2, 8), (3, 8), (3, 16)])
+def test_3d_fp32_form(adg, orc, kind, waves):
+    """FP32-MFMA contractions on the 3D two-level hierarchy, quad-gather and
+    node-lane predictors: per-variable error against the FP64 oracle after 3
+    steps below 1e-6 of the state scale, mass and energy conserved to 1e-13."""
+    print(f"case 3D FP32 form two levels wave P=2 kind={kind} waves={waves}", flush=True)
     levels = two_level_centre(8, 4, D=3)
-    T, H, Uw, ctx = setup(adg, orc, 2, levels, 8, precision=1)
+    T, H, Uw, ctx = setup(adg, orc, 2, levels, 8, kind=kind, precision=1)
+    ctx.set_predictor_waves(waves)
     Dt = orc.set_dt(T, H, Uw, 0.5)
     ctx.set_dt(0.5, 1.0, 0.0)
     m0 = ctx.totals()
@@ -198,3 +200,30 @@ def test_3d_outflow_shock_tube(adg, orc):
         e = rel_err(ctx.download_Uw(l), Uw[l])
         print(f"  level {l} relative error {e:.2e}", flush=True)
         assert e < TOL_SHOCK, f"level {l}"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_3d_nodelane_wave_counts_agree_bitwise(adg, precision):
+    """The 3D node-lane predictor (3 elements per wavefront) with 8 and 16
+    wavefronts per workgroup gives the same bits; 2 steps of the two-level
+    shock tube with outflow boundaries."""
+    print(f"case 3D node-lane 8 vs 16 waves bitwise, precision {precision}", flush=True)
+    levels = two_level_centre(8, 4, D=3)
+    out = []
+    for waves in (8, 16):
+        ctx = adg.Context(3, 2, 1.4, precision)
+        ctx.set_predictor_kernel(3)
+        ctx.set_predictor_waves(waves)
+        ctx.set_geometry([8] * 3, [0] * 3, [1] * 3, periodic=[0, 0, 0])
+        ctx.set_levels(levels)
+        for l in range(2):
+            ctx.project_ic(l, 3, SHOCK)
+        ctx.average_down()
+        ctx.fill_ghosts()
+        ctx.set_dt(0.5, 1.0, 0.0)
+        for _ in range(2):
+            ctx.step(0.5, 1.0)
+        out.append([ctx.download_Uw(l) for l in range(2)])
+        ctx.close()
+    for l in range(2):
+        assert np.array_equal(out[0][l], out[1][l]), f"level {l}"

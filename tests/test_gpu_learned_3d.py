@@ -1,5 +1,5 @@
-"""Learned warm start in 3D (Q = 5, p = 2), run by the quad-gather MFMA
-predictor, with the network trained on seeded 3D density waves
+"""Learned warm start in 3D (Q = 5, p = 2), run by the quad-gather and the
+node-lane MFMA predictors, with the network trained on seeded 3D density waves
 (weights/wave_p2_3d): the inference kernel against the oracle's forward
 pass, the gain of the learned start at equal Picard iterations on a held-out
 wave, the early exit with its per-element fallback (a tolerance nothing
@@ -74,14 +74,16 @@ def test_inference_3d_matches_oracle(adg, orc):
     assert e < 1e-2
 
 
+@pytest.mark.parametrize("kind", [2, 3])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_learned_3d_gain_and_fallback(adg, precision):
+def test_learned_3d_gain_and_fallback(adg, precision, kind):
     """Held-out density wave (amplitude and pressure of no training run).
     At equal iterations the learned start is closer to the converged
     predictor; with a tolerance no element meets, every element falls back
     and the result equals the plain predictor bitwise."""
-    print(f"case 3D learned predictor density wave two levels, precision {precision}", flush=True)
+    print(f"case 3D learned predictor density wave two levels, precision {precision} kind {kind}", flush=True)
     g, _ = _ctx3(adg, precision=precision, ic=(5, [0.27, 0.8]), periodic=True)
+    g.set_predictor_kernel(kind)
     g.mlp_init(1)
     g.mlp_set_params(_weights())
     g.keep_Hw(True)
@@ -153,11 +155,13 @@ def test_learned_3d_conservation_and_accuracy(adg):
     assert all(np.isfinite(c.download_Uw(l)).all() for l in range(len(levels)))
 
 
-def test_learned_3d_nan_start_falls_back(adg):
+@pytest.mark.parametrize("kind", [2, 3])
+def test_learned_3d_nan_start_falls_back(adg, kind):
     """A network whose output is NaN (NaN output biases): every element
     fails the acceptance test and the result is the plain predictor's."""
-    print("case 3D learned start with NaN network output", flush=True)
+    print(f"case 3D learned start with NaN network output, kind {kind}", flush=True)
     g, levels = _ctx3(adg, precision=1, ic=(5, WAVE), periodic=True)
+    g.set_predictor_kernel(kind)
     g.mlp_init(1)
     par = _weights().copy()
     sz = g.mlp_sizes()
