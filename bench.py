@@ -471,6 +471,7 @@ def main():
         --shock-steps timed steps (or, with --shock-steps 0, the whole job to
         --shock-T), each form."""
         w3 = np.fromfile(args.weights3d, dtype=np.float32)
+        f3_pred = algorithmic_flops(2, 3, 5, 10, 15, 81, 27)[1]
         out = {"workload": "3D spherical shock tube p=2, outflow, 3 tagged levels 64^3/128^3/256^3, "
                            "regrid every 16 steps, TVB M=0"}
         for form in args.shock_forms.split(","):
@@ -480,6 +481,7 @@ def main():
                 ctx.set_learned(True, tol=args.tol, max_iters=args.max_iters)
             ctx.set_dt(c_dt, args.shock_T, 0.0)
             shapes, choice = ctx.autotune(3)
+            n_tune = nvalid_of(ctx.get_levels(), 3)
             ctx.regrid_stats(reset=True)
             job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64,
                       name=f"shock tube {form}")
@@ -500,7 +502,15 @@ def main():
                  "value": (job.dof - d0) / (ms * 1e-3), "elements_end": job.nvalid, "elements_max": job.max_nvalid,
                  "regrids": job.regrids, "regrid_host_s": ctx.regrid_stats(),
                  "mass_drift": abs(m1[0] - m0[0]) / m0[0], "energy_drift": abs(m1[-1] - m0[-1]) / m0[-1],
-                 "Dt": job.Dt, "predictor_shapes_ms": shapes, "predictor_choice": choice}
+                 "Dt": job.Dt, "predictor_shapes_ms": shapes, "predictor_choice": choice,
+                 "elements_mean": (job.dof - d0) / job.per_elem / max(taken, 1)}
+            # the autotuned predictor launch on the initial hierarchy against the
+            # matrix peak (algorithmic flops of its valid elements)
+            best = min(shapes.values())
+            fl = f3_pred * n_tune / (best * 1e-3)
+            pk = PEAK_FP64 if form == "fp64" else PEAK_FP32
+            r["predictor_roofline"] = {"elements": n_tune, "flops_per_element": f3_pred, "launch_ms": best,
+                                       "achieved": fl / 1e12, "peak": pk / 1e12, "unit": "TFLOP/s", "frac": fl / pk}
             if form == "learned":
                 ne, nf = ctx.learned_stats()
                 r["fallback_fraction"] = nf / max(ne, 1)
