@@ -407,13 +407,20 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     };
     for (int lf = 1; lf < nlev; ++lf) {
         const int lc = lf - 1;
-        // register terms landing on coarse faces, per direction: (face,
-        // moment, coefficient), grouped by face below
-        struct Land { long face; int mom; double coef; };
-        std::vector<std::vector<Land>> land(D);
-        std::vector<std::pair<int, long>> arrs;
-        std::vector<std::pair<int, double>> reg;
-        for (int B = h.lev[lf].p_begin; B < h.lev[lf].p_end; ++B) {
+        // Register terms of the coarse faces under each fine patch, found per
+        // patch in parallel: for each such face the coarse face arrays it
+        // lies in and its terms (coarse side -1, fine faces +1).  Moment ids
+        // are then assigned patch by patch in the order the terms were found,
+        // so the lists do not depend on the thread count.
+        struct Term { long face; int sel; int kind; double coef; };
+        struct Rec { int d, a0, a1, t0, t1; };
+        struct PatchRegs { std::vector<Rec> rec; std::vector<long> arr; std::vector<Term> term; };
+        const int nfp = h.lev[lf].p_end - h.lev[lf].p_begin;
+        std::vector<PatchRegs> pr((size_t)nfp);
+        parallel_for(nfp, [&](int i) {
+            const int B = h.lev[lf].p_begin + i;
+            PatchRegs& R = pr[(size_t)i];
+            std::vector<std::pair<int, long>> arrs;
             int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
             for (int d = 0; d < D; ++d) { clo[d] = h.patch[B].lo[d] >> 1; chi[d] = h.patch[B].hi[d] >> 1; }
             for (int d = 0; d < D; ++d)
@@ -431,11 +438,11 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                             if (side == 0) nb[d] -= 1;
                             if (!wrap(lc, nb) || covered(lc, nb)) return;
                         }
-                        reg.clear();
+                        const int t0 = (int)R.term.size();
                         coarse_arrays(lc, d, fv, arrs);
                         for (auto& a : arrs) {
                             int b = h.bcoarse[d][a.second];
-                            if (b != 0) reg.push_back({moment(0, d, a.second, b == 1 ? 1 : 0, lc), -1.0});
+                            if (b != 0) R.term.push_back({a.second, b == 1 ? 1 : 0, 0, -1.0});
                         }
                         const int nfc = 1 << (D - 1);
                         for (int c = 0; c < nfc; ++c) {
@@ -448,15 +455,32 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                             }
                             long f = h.face_of(B, d, ff);
                             int b = h.bfine[d][f];
-                            if (b != 0)
-                                reg.push_back({moment(1, d, f, (b == 1 ? 1 : 0) | (h.cidx[d][f] << 1), lf), 1.0});
+                            if (b != 0) R.term.push_back({f, (b == 1 ? 1 : 0) | (h.cidx[d][f] << 1), 1, 1.0});
                         }
-                        if (reg.empty()) return;
-                        for (auto& a : arrs)
-                            for (auto& r : reg) land[d].push_back({a.second, r.first, r.second});
+                        const int t1 = (int)R.term.size();
+                        if (t1 == t0) return;
+                        const int a0 = (int)R.arr.size();
+                        for (auto& a : arrs) R.arr.push_back(a.second);
+                        R.rec.push_back({d, a0, (int)R.arr.size(), t0, t1});
                     });
                 }
-        }
+        });
+        // register terms landing on coarse faces, per direction: (face,
+        // moment, coefficient), grouped by face below
+        struct Land { long face; int mom; double coef; };
+        std::vector<std::vector<Land>> land(D);
+        for (const PatchRegs& R : pr)
+            for (const Rec& r : R.rec) {
+                int ids[8];   // at most 4 coarse face arrays and 4 fine faces
+                if (r.t1 - r.t0 > 8) throw std::runtime_error("register face with more than 8 terms");
+                for (int t = r.t0; t < r.t1; ++t) {
+                    const Term& e = R.term[(size_t)t];
+                    ids[t - r.t0] = moment(e.kind, r.d, e.face, e.sel, e.kind ? lf : lc);
+                }
+                for (int a = r.a0; a < r.a1; ++a)
+                    for (int t = r.t0; t < r.t1; ++t)
+                        land[r.d].push_back({R.arr[(size_t)a], ids[t - r.t0], R.term[(size_t)t].coef});
+            }
         for (int d = 0; d < D; ++d)
             std::stable_sort(land[d].begin(), land[d].end(), [](const Land& a, const Land& b) { return a.face < b.face; });
         // the entries of face f of direction d: [first, last)
@@ -475,7 +499,8 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         // reflux (AmrDG.h:1000) overwrites them, so their correction has no
         // effect, and leaving them out lets the reflux run beside it.  Only
         // the valid cells beside a face with register terms can be corrected;
-        // they are visited in slot order (patch by patch, x fastest).
+        // they are visited in slot order (patch by patch, x fastest), in
+        // chunks on the host threads, concatenated in chunk order.
         std::vector<int> cand;
         for (int d = 0; d < D; ++d)
             for (size_t k = 0; k < land[d].size(); ++k) {
@@ -484,26 +509,32 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             }
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-        for (int slot : cand) {
-            int cp = h.lev[lc].p_begin, hi_p = h.lev[lc].p_end - 1;   // patch holding the slot
-            while (cp < hi_p) {
-                const int mid = (cp + hi_p + 1) / 2;
-                if (h.patch[mid].off <= slot) cp = mid; else hi_p = mid - 1;
-            }
-            const PatchLayout& P = h.patch[cp];
-            int ivc[3] = {0, 0, 0};
-            long r = slot - P.off;
-            bool valid = true;
-            for (int d = 0; d < D; ++d) {
-                ivc[d] = P.lo[d] - 1 + (int)(r % P.g[d]);
-                r /= P.g[d];
-                if (ivc[d] < P.lo[d] || ivc[d] > P.hi[d]) valid = false;
-            }
-            if (!valid) continue;
-            [&](const int* iv) {
-                if (covered(lc, iv)) return;
+        struct Rows { std::vector<int> mom, slot, start, count; std::vector<double> coef; };
+        const long ncand = (long)cand.size();
+        const int nck = (int)std::min<long>(256, std::max<long>(1, ncand / 2048));
+        std::vector<Rows> rows((size_t)nck);
+        parallel_for(nck, [&](int k) {
+            Rows& O = rows[(size_t)k];
+            std::vector<std::pair<int, double>> reg;   // (moment, signed coefficient), summed per moment below
+            for (long ci = ncand * k / nck; ci < ncand * (k + 1) / nck; ++ci) {
+                const int slot = cand[(size_t)ci];
+                int cp = h.lev[lc].p_begin, hi_p = h.lev[lc].p_end - 1;   // patch holding the slot
+                while (cp < hi_p) {
+                    const int mid = (cp + hi_p + 1) / 2;
+                    if (h.patch[mid].off <= slot) cp = mid; else hi_p = mid - 1;
+                }
+                const PatchLayout& P = h.patch[cp];
+                int iv[3] = {0, 0, 0};
+                long r = slot - P.off;
+                bool valid = true;
+                for (int d = 0; d < D; ++d) {
+                    iv[d] = P.lo[d] - 1 + (int)(r % P.g[d]);
+                    r /= P.g[d];
+                    if (iv[d] < P.lo[d] || iv[d] > P.hi[d]) valid = false;
+                }
+                if (!valid || covered(lc, iv)) continue;
                 bool at_if = false;
-                reg.clear();   // (moment, signed coefficient), summed per moment below
+                reg.clear();
                 for (int d = 0; d < D; ++d) {
                     int hv[3] = {iv[0], iv[1], iv[2]};
                     hv[d] += 1;
@@ -514,24 +545,34 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     auto rh = land_of(d, fhi);
                     for (auto e = rh.first; e != rh.second; ++e) reg.push_back({e->mom, -e->coef});
                 }
-                if (!at_if && !conservative_reflux) return;
+                if (!at_if && !conservative_reflux) continue;
                 std::sort(reg.begin(), reg.end(),
                           [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
-                int start = (int)h.re_mom.size();
-                for (size_t k = 0; k < reg.size();) {
-                    size_t e = k;
+                const int start = (int)O.mom.size();
+                for (size_t k2 = 0; k2 < reg.size();) {
+                    size_t e = k2;
                     double sum = 0.0;   // sums of +-1: exact in any order
-                    while (e < reg.size() && reg[e].first == reg[k].first) sum += reg[e++].second;
-                    if (sum != 0.0) { h.re_mom.push_back(reg[k].first); h.re_coef.push_back(sum); }
-                    k = e;
+                    while (e < reg.size() && reg[e].first == reg[k2].first) sum += reg[e++].second;
+                    if (sum != 0.0) { O.mom.push_back(reg[k2].first); O.coef.push_back(sum); }
+                    k2 = e;
                 }
-                int cnt = (int)h.re_mom.size() - start;
-                if (cnt == 0) return;
-                h.rc_slot.push_back((int)h.slot_of(cp, iv));
+                const int cnt = (int)O.mom.size() - start;
+                if (cnt == 0) continue;
+                O.slot.push_back((int)h.slot_of(cp, iv));
+                O.start.push_back(start);
+                O.count.push_back(cnt);
+            }
+        });
+        for (const Rows& O : rows) {
+            const int base = (int)h.re_mom.size();
+            h.re_mom.insert(h.re_mom.end(), O.mom.begin(), O.mom.end());
+            h.re_coef.insert(h.re_coef.end(), O.coef.begin(), O.coef.end());
+            for (size_t i = 0; i < O.slot.size(); ++i) {
+                h.rc_slot.push_back(O.slot[i]);
                 h.rc_lev.push_back((uint8_t)lc);
-                h.rc_start.push_back(start);
-                h.rc_count.push_back(cnt);
-            }(ivc);
+                h.rc_start.push_back(base + O.start[i]);
+                h.rc_count.push_back(O.count[i]);
+            }
         }
     }
     // ---- predictor slots and trace sources (copy ghosts skipped)
