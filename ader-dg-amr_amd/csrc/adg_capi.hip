@@ -255,6 +255,18 @@ int dev_alloc(adg_ctx* c, double** dst, size_t n) {
 }
 
 enum { PV_VALU = 0, PV_QUAD = 1, PV_NL = 2 };
+
+// Grid of a persistent launch over nunit wavefront units (a unit: the
+// elements one wavefront takes per round): no more workgroups than are
+// resident at once (registers and LDS both limit it), so no workgroup waits
+// for a CU, and at least one per CU while there are units to spread, so
+// that a launch of less than one round reaches every CU (the deal gives
+// each workgroup its share of the partial round) instead of filling a few
+// CUs with whole workgroups.
+inline long persistent_grid(long nunit, int wpb, int per_cu, int ncu) {
+    const long nbatch = (nunit + wpb - 1) / wpb;
+    return std::min<long>((long)per_cu * ncu, std::max<long>(nbatch, std::min<long>(ncu, nunit)));
+}
 constexpr size_t kLdsMax = 160 * 1024;
 
 // Dynamic LDS bytes of an MFMA predictor workgroup: the fragments, plus the
@@ -871,11 +883,8 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool le
                                         : adg::predictor_mfma_kernel<T, P, D>();
     const int per_cu = c->pred_per_cu[variant == PV_NL ? (wpb == 16 ? 2 : 1) : 0];
     if (per_cu <= 0) return fail(c, ADG_E_UNSUP, "predictor workgroup does not fit a CU");
-    const long per_batch = (long)wpb * (variant == PV_NL ? nl_elems(c) : adg::Dims<P, D>::EPM);
-    const long nbatch = (n + per_batch - 1) / per_batch;
-    // persistent grid: exactly the workgroups that are resident at once
-    // (registers and LDS both limit it), so no workgroup waits for a CU
-    const long grid = std::min<long>(nbatch, (long)per_cu * c->ncu);
+    const long unit = variant == PV_NL ? nl_elems(c) : adg::Dims<P, D>::EPM;   // elements per wavefront
+    const long grid = persistent_grid((n + unit - 1) / unit, wpb, per_cu, c->ncu);
     void* kargs[] = {&a};
     HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(64 * wpb), kargs, lds, c->stream));
     HIPCHK(c, hipGetLastError());
@@ -955,9 +964,8 @@ static int launch_mlp_warm(adg_ctx* c, long begin, long n, const int* slots) {
     const int nt = c->warm_tiles;
     const void* kern = warm_kernel<P, D, NINP, NOUTP>(nt);
     if (c->warm_per_cu[nt] <= 0) return fail(c, ADG_E_STATE, "warm-start kernel not configured");
-    const long per_wg = 8L * (nt == 0 ? 8 : 16 * nt);   // 8 wavefronts per workgroup
-    const long nbatch = (n + per_wg - 1) / per_wg;
-    const long grid = std::min<long>(nbatch, (long)c->warm_per_cu[nt] * c->ncu);
+    const long per_wave = nt == 0 ? 8 : 16 * nt;   // 8 wavefronts per workgroup
+    const long grid = persistent_grid((n + per_wave - 1) / per_wave, 8, c->warm_per_cu[nt], c->ncu);
     void* kargs[] = {&a};
     HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(512), kargs, lds, c->stream));
     HIPCHK(c, hipGetLastError());
@@ -1809,6 +1817,20 @@ int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_
         c->graph_kind = kind;
     }
     for (int i = 0; i < nsteps; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if (t_out) {
+        HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ADG_OK;
+}
+
+int adg_steps(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_out) {
+    if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
+    if (nsteps < 0) return fail(c, ADG_E_ARG, "nsteps must be >= 0");
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = step_launches(c, c_dt, T_end);
+        if (rc != ADG_OK) return rc;
+    }
     if (t_out) {
         HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));

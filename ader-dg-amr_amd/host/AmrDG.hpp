@@ -422,8 +422,12 @@ class AmrDG : public Solver<AmrDG<P>> {
         learned_tol_ = tol;
         learned_iters_ = max_iters;
     }
-    // Steps replayed per hipGraph launch batch (0: one adg_step per step).
+    // Steps per batch between regrid and plot points (0: one adg_step per
+    // step); a batch runs as stream launches (adg_steps) or, with
+    // setBatchGraph(true), as replays of the captured step graph
+    // (adg_step_graph).
     void setGraphSteps(int max_batch) { graph_batch_ = max_batch; }
+    void setBatchGraph(bool on) { batch_graph_ = on; }
     // Launch-shape autotuning after the hierarchy is set (0: off).
     void setAutotune(int reps) { autotune_reps_ = reps; }
     // Print the t=0 and t=T norms as the reference does.
@@ -520,7 +524,7 @@ class AmrDG : public Solver<AmrDG<P>> {
     // regrid points (every dtn_regrid steps or dt_regrid of time,
     // AmrDG.h:936-986), ADER, the limiter on the finest level, average down,
     // FillPatch, t += Dt, set_Dt and the clamp to T.  Runs of steps between
-    // regrid points are replayed from the step graph.
+    // regrid points run in batches (adg_steps or adg_step_graph).
     template <typename EquationType, typename BC>
     void evolve(const std::shared_ptr<ModelEquation<EquationType>>& model_pde, const std::shared_ptr<BC>& bdcond) {
         n_regrids_ = 0;
@@ -573,7 +577,8 @@ class AmrDG : public Solver<AmrDG<P>> {
                 if (max_steps_ >= 0) batch = std::min(batch, max_steps_ - this->n + restart_steps_offset());
             }
             if (batch >= 2) {
-                chk(adg_step_graph(ctx_, this->c_dt, this->T, batch, nullptr), "adg_step_graph");
+                if (batch_graph_) chk(adg_step_graph(ctx_, this->c_dt, this->T, batch, nullptr), "adg_step_graph");
+                else chk(adg_steps(ctx_, this->c_dt, this->T, batch, nullptr), "adg_steps");
                 this->n += batch;
             } else {
                 time_integration(model_pde, bdcond, this->t);
@@ -863,6 +868,7 @@ class AmrDG : public Solver<AmrDG<P>> {
     double learned_tol_ = 1e-4;
     int learned_iters_ = 2;
     int graph_batch_ = 64;
+    bool batch_graph_ = false;
     int autotune_reps_ = 0;
     int max_steps_ = -1;
     bool verbose_ = true;

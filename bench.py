@@ -76,15 +76,18 @@ def nvalid_of(levels, D):
 
 
 class Job:
-    """One run of the evolve loop with regrids, driven between graph batches.
+    """One run of the evolve loop with regrids, driven in batches of steps
+    (stream launches with no host synchronisation inside a batch, or graph
+    replays with graph=True).
 
     Regrids are due every regrid_dt time units (the reference's dt_regrid)
     or every regrid_steps steps (dtn_regrid); a batch never crosses a due
     regrid, the end time or a step budget.  dof counts valid elements x Q x
     Np_st for every step taken."""
 
-    def __init__(self, ctx, D, c_dt, T, regrid_dt=None, regrid_steps=None, max_batch=64, name=None):
+    def __init__(self, ctx, D, c_dt, T, regrid_dt=None, regrid_steps=None, max_batch=64, name=None, graph=False):
         self.ctx, self.D, self.c_dt, self.T = ctx, D, c_dt, T
+        self.graph = graph
         self.name = name
         self.last_log = time.perf_counter()
         self.regrid_dt, self.regrid_steps, self.max_batch = regrid_dt, regrid_steps, max_batch
@@ -129,7 +132,10 @@ class Job:
             if nsteps is not None:
                 batch = min(batch, nsteps - taken)
             if batch >= 2:
-                self.ctx.step_graph(self.c_dt, self.T, batch)
+                if self.graph:
+                    self.ctx.step_graph(self.c_dt, self.T, batch)
+                else:
+                    self.ctx.steps(self.c_dt, self.T, batch)
             else:
                 batch = 1
                 self.ctx.step(self.c_dt, self.T)
@@ -267,7 +273,10 @@ def main():
                     help="only the predictor measurement on the band hierarchy (the profiled command)")
     ap.add_argument("--shock-only", action="store_true", help="only the 3D shock-tube measurement")
     ap.add_argument("--no-graph", action="store_true",
-                    help="step the jobs one eager step at a time instead of replaying the step graph")
+                    help="step the jobs one step per call (the profiled commands)")
+    ap.add_argument("--graph", action="store_true",
+                    help="run the jobs' batches of steps as replays of the captured step graph instead of "
+                         "stream launches")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -333,7 +342,8 @@ def main():
         out["pred_ms"] = time_launches(ctx, lambda: ctx.predictor(), reps)
         out["faceflux_ms"] = time_launches(ctx, lambda: ctx.face_flux(), reps)
         out["corrector_ms"] = time_launches(ctx, lambda: ctx.corrector(), reps)
-        out["step_ms"] = time_launches(ctx, lambda: ctx.step_graph(c_dt, 1e9, 1), reps)
+        out["step_ms"] = time_launches(ctx, lambda: ctx.steps(c_dt, 1e9, 1), reps)
+        out["step_graph_ms"] = time_launches(ctx, lambda: ctx.step_graph(c_dt, 1e9, 1), reps)
         nslot = sum(ctx.nslot(l) for l in range(len(levels)))
         nface = sum(ctx.nface(l, d) for l in range(len(levels)) for d in range(D))
         nfine = sum(nvalid_of([levels[l]], D) for l in range(1, len(levels)))
@@ -369,7 +379,7 @@ def main():
         tune = ctx.autotune(5)[1]
         ctx.regrid_stats(reset=True)
         s = ctx.stream()
-        job = Job(ctx, D, c_dt, args.T, regrid_dt=0.05, max_batch=1 if args.no_graph else 64)
+        job = Job(ctx, D, c_dt, args.T, regrid_dt=0.05, max_batch=1 if args.no_graph else 64, graph=args.graph)
         levels0 = job.levels
         m0 = ctx.totals()
         j0, e0, e1, j1 = Event(), Event(), Event(), Event()
@@ -428,8 +438,8 @@ def main():
         ctx.fill_ghosts()
         ctx.set_dt(c_dt, 1e9, 0.0)
         tune, choice = ctx.autotune(5)
-        ctx.step_graph(c_dt, 1e9, 3)
-        ms = time_launches(ctx, lambda: ctx.step_graph(c_dt, 1e9, 1), 20)
+        ctx.steps(c_dt, 1e9, 3)
+        ms = time_launches(ctx, lambda: ctx.steps(c_dt, 1e9, 1), 20)
         nel = int(sum(n * n for n in edges))
         Q, Nst = ctx.Q, ctx.Np_st
         ctx.close()
@@ -483,7 +493,7 @@ def main():
             shapes, choice = ctx.autotune(3)
             n_tune = nvalid_of(ctx.get_levels(), 3)
             ctx.regrid_stats(reset=True)
-            job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64,
+            job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64, graph=args.graph,
                       name=f"shock tube {form}")
             m0 = ctx.totals()
             s = ctx.stream()

@@ -253,6 +253,46 @@ def test_step_graph_replays(adg, orc, precision):
             assert rel_err(ctx.download_Uw(l), Uw[l]) < TOL
 
 
+@pytest.mark.parametrize("learned", [False, True])
+def test_batched_steps_equal_single_steps(adg, learned):
+    """adg_steps (a batch of stream launches), adg_step_graph and single
+    adg_step calls give bitwise the same U_w, t and Dt: 3 batches of 5 steps
+    of the 3-level KH state in the FP32 form, plain and with the committed
+    warm-start network."""
+    print(f"case batched steps vs single steps KH P=3 16^2 3 levels learned={learned}", flush=True)
+    import os
+    levels = kh_bands(16, 3, 16)
+    w = np.fromfile(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                 "ader-dg-amr_amd", "weights", "kh_p3_2d.f32"), dtype=np.float32)
+    ctxs = []
+    for _ in range(3):
+        c = adg.Context(2, 3, 1.4, 1)
+        c.set_geometry([16, 16], [0, 0], [1, 1])
+        c.set_levels(levels)
+        for l in range(3):
+            c.project_ic(l, 1, [0.01, 1e-3, 7])
+        c.average_down()
+        c.fill_ghosts()
+        if learned:
+            c.mlp_init(1)
+            c.mlp_set_params(w)
+            c.set_learned(True, tol=1e-4, max_iters=2)
+        c.set_dt(0.5, 1.0, 0.0)
+        ctxs.append(c)
+    for b in range(3):
+        ctxs[0].steps(0.5, 1.0, 5)
+        ctxs[1].step_graph(0.5, 1.0, 5)
+        for _ in range(5):
+            ctxs[2].step(0.5, 1.0)
+        ref = ctxs[2].get_time()
+        for k in (0, 1):
+            assert ctxs[k].get_time() == ref, f"batch {b} form {k}: (t, Dt)"
+            for l in range(3):
+                assert np.array_equal(ctxs[k].download_Uw(l), ctxs[2].download_Uw(l)), f"batch {b} form {k} level {l}"
+    for c in ctxs:
+        c.close()
+
+
 @pytest.mark.parametrize("nlev,periodic", [(3, [1, 1]), (4, [1, 1]), (3, [1, 0]), (2, [1, 1])])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_step_equals_unfused(adg, nlev, periodic, precision):
