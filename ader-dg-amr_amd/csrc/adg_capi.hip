@@ -84,6 +84,7 @@ struct adg_ctx {
     double plo[3] = {0, 0, 0}, phi[3] = {1, 1, 1};
     int per[3] = {1, 1, 1};
     adg::Hierarchy H;
+    adg::Hierarchy H_spare;   // the lists of an earlier layout, whose host storage the next build takes over
     adg::LevelGeo geo{};
 
     double *Uw = nullptr, *Hw_dbg = nullptr, *Mom = nullptr, *partial = nullptr;
@@ -505,7 +506,7 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     const auto t_h0 = std::chrono::steady_clock::now();
     try {
         c->H = adg::build_hierarchy(c->D, c->dom0, c->plo, c->phi, c->per, nlev, npatch, boxes, c->independent ? 1 : 0,
-                                    c->conservative_reflux ? 1 : 0);
+                                    c->conservative_reflux ? 1 : 0, &c->H_spare);
     } catch (const std::exception& e) {
         return fail(c, ADG_E_ARG, e.what());
     }
@@ -1978,27 +1979,19 @@ std::vector<std::vector<adg::IBox>> level_boxes(const adg::Hierarchy& H) {
 }
 
 // Dense owner map of one level: level-index cell -> slot of its valid cell, or -1.
+// Cell -> valid slot of one level (the hierarchy's owner grid), with the
+// periodic wrap.
 struct OwnerMap {
     int D = 2, dom[3] = {1, 1, 1};
     int per[3] = {1, 1, 1};
-    std::vector<int> own;
+    const int* own = nullptr;
     void build(const adg::Hierarchy& H, int lev) {
         D = H.D;
-        long n = 1;
         for (int d = 0; d < 3; ++d) {
             dom[d] = d < D ? H.lev[lev].dom[d] : 1;
             per[d] = d < D ? H.per[d] : 1;
-            n *= dom[d];
         }
-        own.assign((size_t)n, -1);
-        const int pb = H.lev[lev].p_begin;
-        adg::parallel_for(H.lev[lev].p_end - pb, [&](int i) {
-            const adg::PatchLayout& P = H.patch[(size_t)(pb + i)];
-            int iv[3] = {0, 0, 0};
-            for (iv[2] = P.lo[2]; iv[2] <= (D > 2 ? P.hi[2] : P.lo[2]); ++iv[2])
-                for (iv[1] = P.lo[1]; iv[1] <= P.hi[1]; ++iv[1])
-                    for (iv[0] = P.lo[0]; iv[0] <= P.hi[0]; ++iv[0]) own[(size_t)lin(iv)] = (int)H.slot_of(pb + i, iv);
-        });
+        own = H.grid_slot[(size_t)lev].data();
     }
     long lin(const int* iv) const {
         long k = 0;
@@ -2012,7 +2005,7 @@ struct OwnerMap {
                 if (!per[d]) return -1;
                 iv[d] = ((iv[d] % dom[d]) + dom[d]) % dom[d];
             }
-        return own[(size_t)lin(iv)];
+        return own[lin(iv)];
     }
 };
 
@@ -2210,7 +2203,9 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
                     return fail(c, ADG_E_ARG, "level 0 boxes differ");
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    adg::Hierarchy old = c->H;
+    // the old lists move out (the new build recycles the storage of the
+    // layout before them, and these are kept for the next one)
+    adg::Hierarchy old = std::move(c->H);
     double* old_uw = c->Uw;
     c->level_allocs.erase(std::remove(c->level_allocs.begin(), c->level_allocs.end(), (void*)old_uw),
                           c->level_allocs.end());
@@ -2308,6 +2303,7 @@ int adg_remake(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     }
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(old_uw);
+    c->H_spare = std::move(old);
     c->rg_sec[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_x0).count();
     if (!err.empty()) return fail(c, ADG_E_ARG, err);
     ++c->rg_changes;

@@ -16,6 +16,62 @@ long Hierarchy::slot_of(int p, const int* iv) const {
     return P.off + s;
 }
 
+void Hierarchy::take_storage(Hierarchy& o) {
+    auto take = [](auto& dst, auto& src) {
+        dst.swap(src);
+        dst.clear();
+        src.clear();
+        src.shrink_to_fit();
+    };
+    take(lev, o.lev);
+    take(patch, o.patch);
+    take(slot_lev, o.slot_lev);
+    take(valid_list, o.valid_list);
+    for (int d = 0; d < 3; ++d) {
+        take(cell_face[d][0], o.cell_face[d][0]);
+        take(cell_face[d][1], o.cell_face[d][1]);
+        take(face_lr[d], o.face_lr[d]);
+        take(face_lev[d], o.face_lev[d]);
+        take(face_tr[d], o.face_tr[d]);
+        take(bcoarse[d], o.bcoarse[d]);
+        take(bfine[d], o.bfine[d]);
+        take(cidx[d], o.cidx[d]);
+    }
+    take(g_slot, o.g_slot);
+    take(g_src, o.g_src);
+    take(g_kind, o.g_kind);
+    take(g_child, o.g_child);
+    take(ad_coarse, o.ad_coarse);
+    take(ad_fine, o.ad_fine);
+    take(mom_face, o.mom_face);
+    take(mom_d, o.mom_d);
+    take(mom_kind, o.mom_kind);
+    take(mom_sel, o.mom_sel);
+    take(mom_lev, o.mom_lev);
+    take(rc_slot, o.rc_slot);
+    take(rc_start, o.rc_start);
+    take(rc_count, o.rc_count);
+    take(rc_lev, o.rc_lev);
+    take(re_mom, o.re_mom);
+    take(re_coef, o.re_coef);
+    take(uncovered, o.uncovered);
+    take(pred_slots, o.pred_slots);
+    take(pred_off, o.pred_off);
+    take(copy_ghost, o.copy_ghost);
+    take(copy_src, o.copy_src);
+    take(lim_cells, o.lim_cells);
+    take(lim_nbr, o.lim_nbr);
+    take(unc_centre, o.unc_centre);
+    // the owner grids keep their per-level storage (clear keeps the inner
+    // vectors' capacity only if they stay; they are refilled by assign)
+    grid_slot.swap(o.grid_slot);
+    grid_patch.swap(o.grid_patch);
+    o.grid_slot.clear();
+    o.grid_patch.clear();
+    for (int k = 0; k < 2; ++k)
+        for (int d = 0; d < 3; ++d) mom_tab[k][d].swap(o.mom_tab[k][d]);   // contents kept: all -1
+}
+
 long Hierarchy::face_of(int p, int d, const int* iv) const {
     const PatchLayout& P = patch[p];
     long s = 0;
@@ -50,10 +106,12 @@ void for_box(int D, const int* lo, const int* hi, F&& f) {
 }  // namespace
 
 Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
-                          const int* per, int nlev, const int* npatch, const int* boxes, int independent, int conservative_reflux) {
+                          const int* per, int nlev, const int* npatch, const int* boxes, int independent, int conservative_reflux,
+                          Hierarchy* reuse) {
     if (D < 2 || D > 3 || nlev < 1 || nlev > 8) throw std::runtime_error("bad dim or level count");
     if (independent && nlev != 1) throw std::runtime_error("independent patches form a single level");
     Hierarchy h;
+    if (reuse) h.take_storage(*reuse);
     h.D = D;
     h.nlev = nlev;
     h.independent = independent != 0;
@@ -110,8 +168,12 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
 
     // ---- owner grids (valid cells), overlap check
     std::vector<Grid> G(nlev);
+    h.grid_slot.resize(nlev);
+    h.grid_patch.resize(nlev);
     for (int l = 0; l < nlev; ++l) {
         Grid& g = G[l];
+        g.slot.swap(h.grid_slot[l]);   // recycled storage, refilled below
+        g.pidx.swap(h.grid_patch[l]);
         g.D = D;
         long nd = 1;
         for (int d = 0; d < 3; ++d) { g.dom[d] = d < D ? h.lev[l].dom[d] : 1; nd *= g.dom[d]; }
@@ -360,11 +422,12 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     // ---- flux-register algebra (FluxRegister CrseAdd(-1) / FineAdd(+1) /
     // Reflux(vol=1) as called in AmrDG.h:1138-1153 and AmrDG_Level.tpp:416-446)
     // (kind, d, face) -> moment index, a face-indexed table per kind and
-    // direction (-1: none yet)
-    std::vector<int> mom_id[2][3];
+    // direction (-1: none yet), kept with the hierarchy's storage and reset
+    // entry by entry below, so that no build fills a table of every face
+    std::vector<int> (&mom_id)[2][3] = h.mom_tab;
     auto moment = [&](int kind, int d, long face, int sel, int lev) {
         std::vector<int>& tab = mom_id[kind][d];
-        if (tab.empty()) tab.assign((size_t)h.NF[d], -1);
+        if (tab.size() < (size_t)h.NF[d]) tab.resize((size_t)h.NF[d], -1);
         if (tab[(size_t)face] >= 0) return tab[(size_t)face];
         int id = (int)h.mom_face.size();
         tab[(size_t)face] = id;
@@ -575,6 +638,7 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
             }
         }
     }
+    for (size_t m = 0; m < h.mom_face.size(); ++m) mom_id[h.mom_kind[m]][h.mom_d[m]][(size_t)h.mom_face[m]] = -1;
     // ---- predictor slots and trace sources (copy ghosts skipped)
     {
         std::vector<int> src_of((size_t)h.NS, -1);
@@ -639,6 +703,10 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     h.unc_centre[(size_t)i * 3 + d] = h.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * h.lev[pt.lev].dx[d];
             }
         });
+    }
+    for (int l = 0; l < nlev; ++l) {
+        h.grid_slot[l].swap(G[l].slot);
+        h.grid_patch[l].swap(G[l].pidx);
     }
     return h;
 }

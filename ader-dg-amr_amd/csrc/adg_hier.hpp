@@ -133,8 +133,20 @@ struct Hierarchy {
     // face masks for tests (global face ids per direction)
     std::vector<int8_t> bcoarse[3], bfine[3], cidx[3];
 
+    // owner grids: per level, the slot of the valid cell at each cell of the
+    // level's domain (x fastest), -1 where the level has no cell
+    std::vector<std::vector<int>> grid_slot;
+    std::vector<std::vector<int>> grid_patch;   // the patch of that cell, -1 elsewhere
+    // build scratch: (kind, d) face -> moment id tables, -1 everywhere
+    // between builds (a build resets the entries it set)
+    std::vector<int> mom_tab[2][3];
+
     long slot_of(int p, const int* iv) const;
     long face_of(int p, int d, const int* iv) const;
+    // Moves the host storage of every list of o into this hierarchy, emptied
+    // (capacity kept), so that a rebuild writes into memory that is already
+    // mapped instead of faulting fresh pages in; o is left empty.
+    void take_storage(Hierarchy& o);
 };
 
 // Throws std::runtime_error on malformed input.
@@ -146,8 +158,11 @@ struct Hierarchy {
 // own patch's ownership-filtered cf_face_b_coarse (AmrDG_Interpolater.tpp:
 // 39-69), which drops the correction of a cell whose coarse-fine face is the
 // low face of its patch -- the reference loses mass at such faces.
+// reuse (nullable): a hierarchy no longer needed whose host storage the new
+// one takes over (Hierarchy::take_storage); the lists are the same with or
+// without it.
 Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const double* phi,
                           const int* per, int nlev, const int* npatch, const int* boxes, int independent = 0,
-                          int conservative_reflux = 0);
+                          int conservative_reflux = 0, Hierarchy* reuse = nullptr);
 
 }  // namespace adg
