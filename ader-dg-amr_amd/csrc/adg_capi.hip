@@ -104,6 +104,8 @@ struct adg_ctx {
     hipGraphExec_t graph_exec = nullptr;
     double graph_cdt = -1.0, graph_T = -1.0;
     int graph_kind = -1;
+    int graph_unroll = 1;      // steps captured per graph
+    int graph_captured = 0;    // steps in the captured graph
     int* valid_list = nullptr;
     int* cell_face[3][2] = {};
     int* face_lr[3] = {};
@@ -1797,7 +1799,8 @@ int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_
     if (!c || !c->levels_set) return fail(c, ADG_E_STATE, "levels not set");
     if (nsteps < 0) return fail(c, ADG_E_ARG, "nsteps must be >= 0");
     const int kind = (pred_variant(c) * 32 + c->nl_wpb) * 2 + (c->learned ? 1 : 0);   // baked into the graph
-    if (!c->graph_exec || c->graph_cdt != c_dt || c->graph_T != T_end || c->graph_kind != kind) {
+    if (!c->graph_exec || c->graph_cdt != c_dt || c->graph_T != T_end || c->graph_kind != kind ||
+        c->graph_captured != c->graph_unroll) {
         drop_graph(c);
         if (c->learned) {
             if (!c->mlp.init) return fail(c, ADG_E_STATE, "learned mode without a network");
@@ -1806,7 +1809,8 @@ int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = step_launches(c, c_dt, T_end);
+        int rc = ADG_OK;
+        for (int k = 0; k < c->graph_unroll && rc == ADG_OK; ++k) rc = step_launches(c, c_dt, T_end);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc != ADG_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1816,12 +1820,27 @@ int adg_step_graph(adg_ctx* c, double c_dt, double T_end, int nsteps, double* t_
         c->graph_cdt = c_dt;
         c->graph_T = T_end;
         c->graph_kind = kind;
+        c->graph_captured = c->graph_unroll;
     }
-    for (int i = 0; i < nsteps; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    // whole graphs, then the remaining steps as the same launches on the stream
+    const int K = c->graph_captured;
+    int i = 0;
+    for (; i + K <= nsteps; i += K) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    for (; i < nsteps; ++i) {
+        int rc = step_launches(c, c_dt, T_end);
+        if (rc != ADG_OK) return rc;
+    }
     if (t_out) {
         HIPCHK(c, hipMemcpyAsync(t_out, &c->ts->t, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    return ADG_OK;
+}
+
+int adg_set_graph_unroll(adg_ctx* c, int steps) {
+    if (!c) return ADG_E_ARG;
+    if (steps < 1 || steps > 64) return fail(c, ADG_E_ARG, "steps per graph must be 1..64");
+    c->graph_unroll = steps;
     return ADG_OK;
 }
 

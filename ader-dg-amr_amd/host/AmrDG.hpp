@@ -423,9 +423,9 @@ class AmrDG : public Solver<AmrDG<P>> {
         learned_iters_ = max_iters;
     }
     // Steps per batch between regrid and plot points (0: one adg_step per
-    // step); a batch runs as stream launches (adg_steps) or, with
-    // setBatchGraph(true), as replays of the captured step graph
-    // (adg_step_graph).
+    // step); a batch runs as replays of the captured step graph (16 steps
+    // per graph, adg_step_graph) or, with setBatchGraph(false), as stream
+    // launches (adg_steps).
     void setGraphSteps(int max_batch) { graph_batch_ = max_batch; }
     void setBatchGraph(bool on) { batch_graph_ = on; }
     // Launch-shape autotuning after the hierarchy is set (0: off).
@@ -446,6 +446,7 @@ class AmrDG : public Solver<AmrDG<P>> {
             ctx_ = nullptr;
         }
         chk(adg_create(&ctx_, SPACEDIM, P, model_pde->gamma_adiab, precision_, nullptr), "adg_create");
+        chk(adg_set_graph_unroll(ctx_, 16), "adg_set_graph_unroll");   // 16 steps per captured graph
         Q = SPACEDIM + 2;
         const RealBox& rb = mesh->get_RealBox();
         int dom[3] = {0, 0, 0}, per[3] = {0, 0, 0};
@@ -868,7 +869,7 @@ class AmrDG : public Solver<AmrDG<P>> {
     double learned_tol_ = 1e-4;
     int learned_iters_ = 2;
     int graph_batch_ = 64;
-    bool batch_graph_ = false;
+    bool batch_graph_ = true;
     int autotune_reps_ = 0;
     int max_steps_ = -1;
     bool verbose_ = true;

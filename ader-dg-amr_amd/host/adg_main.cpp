@@ -10,7 +10,7 @@
 //                    [--projection_ic] [--conservative_reflux] [--cfl_uncovered]
 //                    [--plot_every N] [--plot_dt T]
 //                    [--data_dir DIR] [--restart N]
-//                    [--graph_steps K] [--graph] [--max_steps N] [--dump FILE] [--quiet]
+//                    [--graph_steps K] [--stream] [--max_steps N] [--dump FILE] [--quiet]
 // Without --regrid a fine level is the static centre half of the domain; with
 // --regrid the levels start full-domain and follow the tags at the case's
 // regrid points (the reference's settings for the vortex and its KH case).
@@ -34,7 +34,7 @@ struct Options {
     double T = -1.0, c_dt = -1.0, tol = 1e-4;
     std::string precision = "fp64", weights, dump;
     bool quiet = false;
-    bool graph = false;   // batches as step-graph replays
+    bool graph = true;   // batches as step-graph replays (--stream: stream launches)
     bool regrid = false, limit_ic = false, projection_ic = false, conservative_reflux = false;
     bool cfl_uncovered = false;
     int dtn_regrid = -2, t_limit = -2;
@@ -55,7 +55,7 @@ void usage() {
         "                [--limiter TVB --tvb_m M --t_limit N] [--limit_ic] [--projection_ic]\n"
         "                [--conservative_reflux] [--cfl_uncovered]\n"
         "                [--plot_every N] [--plot_dt T] [--data_dir DIR] [--restart N]\n"
-        "                [--graph_steps K] [--graph] [--max_steps N] [--dump FILE] [--quiet]\n"
+        "                [--graph_steps K] [--stream] [--max_steps N] [--dump FILE] [--quiet]\n"
         "space dimension: %d\n",
         adg::SPACEDIM);
 }
@@ -79,7 +79,7 @@ bool parse(int argc, char** argv, Options* o) {
         else if (a == "--tol") o->tol = std::atof(next());
         else if (a == "--max_iters") o->max_iters = std::atoi(next());
         else if (a == "--graph_steps") o->graph_steps = std::atoi(next());
-        else if (a == "--graph") o->graph = true;
+        else if (a == "--stream") o->graph = false;
         else if (a == "--max_steps") o->max_steps = std::atoi(next());
         else if (a == "--dump") o->dump = next();
         else if (a == "--quiet") o->quiet = true;

@@ -23,7 +23,7 @@ EXPORTS = [
     "adg_create", "adg_set_geometry", "adg_set_levels", "adg_level_nslot", "adg_level_nface", "adg_level_npred",
     "adg_upload_Uw", "adg_download_Uw", "adg_fill_ghosts", "adg_average_down", "adg_set_dt",
     "adg_set_dt_value", "adg_predictor", "adg_keep_Hw", "adg_download_Hw", "adg_face_flux",
-    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_steps", "adg_set_time", "adg_get_time",
+    "adg_download_Fnum", "adg_corrector", "adg_reflux", "adg_ader", "adg_advance", "adg_step", "adg_step_graph", "adg_steps", "adg_set_graph_unroll", "adg_set_time", "adg_get_time",
     "adg_conserved_totals", "adg_sync", "adg_stream", "adg_sizes", "adg_get_table",
     "adg_project_ic", "adg_set_predictor_kernel", "adg_set_predictor_waves", "adg_set_warm_tiles", "adg_last_error", "adg_destroy",
     "adg_mlp_sizes", "adg_mlp_init", "adg_mlp_get_params", "adg_mlp_set_params", "adg_mlp_collect",
@@ -71,6 +71,7 @@ def lib():
     L.adg_step.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
     L.adg_step_graph.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp]
     L.adg_steps.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp]
+    L.adg_set_graph_unroll.argtypes = [C.c_void_p, C.c_int]
     L.adg_set_time.argtypes = [C.c_void_p, C.c_double]
     L.adg_get_time.argtypes = [C.c_void_p, _dp, _dp]
     L.adg_conserved_totals.argtypes = [C.c_void_p, _dp]
@@ -317,6 +318,10 @@ class Context:
 
     def step_graph(self, c_dt, T, nsteps):
         self._chk(self.L.adg_step_graph(self.h, c_dt, T, nsteps, None), "adg_step_graph")
+
+    def set_graph_unroll(self, steps):
+        """Steps captured per step graph (1..64)."""
+        self._chk(self.L.adg_set_graph_unroll(self.h, steps), "adg_set_graph_unroll")
 
     def steps(self, c_dt, T, nsteps):
         """nsteps steps as stream launches, no host synchronisation between them."""

@@ -76,18 +76,21 @@ def nvalid_of(levels, D):
 
 
 class Job:
-    """One run of the evolve loop with regrids, driven in batches of steps
-    (stream launches with no host synchronisation inside a batch, or graph
-    replays with graph=True).
+    """One run of the evolve loop with regrids, driven in batches of steps:
+    replays of the captured step graph (16 steps per graph, leftover steps
+    as stream launches) or, with graph=False, stream launches; no host
+    synchronisation inside a batch.
 
     Regrids are due every regrid_dt time units (the reference's dt_regrid)
     or every regrid_steps steps (dtn_regrid); a batch never crosses a due
     regrid, the end time or a step budget.  dof counts valid elements x Q x
     Np_st for every step taken."""
 
-    def __init__(self, ctx, D, c_dt, T, regrid_dt=None, regrid_steps=None, max_batch=64, name=None, graph=False):
+    def __init__(self, ctx, D, c_dt, T, regrid_dt=None, regrid_steps=None, max_batch=64, name=None, graph=True):
         self.ctx, self.D, self.c_dt, self.T = ctx, D, c_dt, T
         self.graph = graph
+        if graph:
+            ctx.set_graph_unroll(16)
         self.name = name
         self.last_log = time.perf_counter()
         self.regrid_dt, self.regrid_steps, self.max_batch = regrid_dt, regrid_steps, max_batch
@@ -274,9 +277,9 @@ def main():
     ap.add_argument("--shock-only", action="store_true", help="only the 3D shock-tube measurement")
     ap.add_argument("--no-graph", action="store_true",
                     help="step the jobs one step per call (the profiled commands)")
-    ap.add_argument("--graph", action="store_true",
-                    help="run the jobs' batches of steps as replays of the captured step graph instead of "
-                         "stream launches")
+    ap.add_argument("--stream", action="store_true",
+                    help="run the jobs' batches of steps as stream launches instead of replays of the captured "
+                         "step graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -343,7 +346,10 @@ def main():
         out["faceflux_ms"] = time_launches(ctx, lambda: ctx.face_flux(), reps)
         out["corrector_ms"] = time_launches(ctx, lambda: ctx.corrector(), reps)
         out["step_ms"] = time_launches(ctx, lambda: ctx.steps(c_dt, 1e9, 1), reps)
-        out["step_graph_ms"] = time_launches(ctx, lambda: ctx.step_graph(c_dt, 1e9, 1), reps)
+        out["step_graph1_ms"] = time_launches(ctx, lambda: ctx.step_graph(c_dt, 1e9, 1), reps)   # 1 step per graph
+        ctx.set_graph_unroll(16)
+        ctx.step_graph(c_dt, 1e9, 16)
+        out["step_graph_ms"] = time_launches(ctx, lambda: ctx.step_graph(c_dt, 1e9, 16), 8) / 16
         nslot = sum(ctx.nslot(l) for l in range(len(levels)))
         nface = sum(ctx.nface(l, d) for l in range(len(levels)) for d in range(D))
         nfine = sum(nvalid_of([levels[l]], D) for l in range(1, len(levels)))
@@ -379,7 +385,7 @@ def main():
         tune = ctx.autotune(5)[1]
         ctx.regrid_stats(reset=True)
         s = ctx.stream()
-        job = Job(ctx, D, c_dt, args.T, regrid_dt=0.05, max_batch=1 if args.no_graph else 64, graph=args.graph)
+        job = Job(ctx, D, c_dt, args.T, regrid_dt=0.05, max_batch=1 if args.no_graph else 64, graph=not args.stream)
         levels0 = job.levels
         m0 = ctx.totals()
         j0, e0, e1, j1 = Event(), Event(), Event(), Event()
@@ -493,7 +499,7 @@ def main():
             shapes, choice = ctx.autotune(3)
             n_tune = nvalid_of(ctx.get_levels(), 3)
             ctx.regrid_stats(reset=True)
-            job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64, graph=args.graph,
+            job = Job(ctx, 3, c_dt, args.shock_T, regrid_steps=16, max_batch=1 if args.no_graph else 64, graph=not args.stream,
                       name=f"shock tube {form}")
             m0 = ctx.totals()
             s = ctx.stream()
@@ -626,10 +632,13 @@ def main():
                    "accuracy": accuracy,
                    "algorithmic_flops_per_element_step": f_step,
                    "kernels_band_hierarchy": {f: {x: kern[f][x] for x in ("pred_ms", "faceflux_ms", "corrector_ms",
-                                                                       "step_ms", "autotune")} for f in kern},
+                                                                       "step_ms", "step_graph_ms", "step_graph1_ms",
+                                                                       "autotune")}
+                                              for f in kern},
                    "batch": batch, "vortex": vortex, "shock3d": shock3d,
-                   "step": "hipGraph replay of predictor, face flux, corrector, reflux+average down, limiter, "
-                           "ghost fill+CFL+advance; regrid between batches when due",
+                   "step": "predictor, face flux, corrector, limiter, reflux+average down, ghost fill+CFL+advance, "
+                           "batches of up to 64 steps as replays of a hipGraph of 16 captured steps (--stream: "
+                           "stream launches); regrid between batches when due",
                    "composite_rules": "conservative reflux and CFL over uncovered cells (not the reference's rules)"},
         "roofline": roofline, "cpu_baseline": cpu,
     }

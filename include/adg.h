@@ -244,9 +244,15 @@ int adg_advance(adg_ctx* ctx, double c_dt, double T, double* t_out);
  * clamp to T.  t stays on the device; t_out (nullable) gets a blocking copy. */
 int adg_step(adg_ctx* ctx, double c_dt, double T, double* t_out);
 /* nsteps iterations of adg_step replayed from one captured hipGraph (the
- * graph is captured on first use and re-captured when c_dt, T or the
- * predictor kernel change).  t_out (nullable) gets a blocking copy of t. */
+ * graph is captured on first use and re-captured when c_dt, T, the
+ * predictor kernel or the steps per graph change).  A graph holds
+ * adg_set_graph_unroll steps (default 1); steps left over after whole
+ * graphs run as the same launches on the stream.  t_out (nullable) gets a
+ * blocking copy of t. */
 int adg_step_graph(adg_ctx* ctx, double c_dt, double T, int nsteps, double* t_out);
+/* Steps captured per step graph, 1..64.  No counterpart in the reference: a
+ * launch-shape knob of this implementation. */
+int adg_set_graph_unroll(adg_ctx* ctx, int steps);
 /* nsteps iterations of adg_step enqueued on the library stream as separate
  * launches, with no host synchronisation between them (the same kernels in
  * the same order as adg_step_graph: bitwise the same results).  On MI355X

@@ -255,10 +255,11 @@ def test_step_graph_replays(adg, orc, precision):
 
 @pytest.mark.parametrize("learned", [False, True])
 def test_batched_steps_equal_single_steps(adg, learned):
-    """adg_steps (a batch of stream launches), adg_step_graph and single
-    adg_step calls give bitwise the same U_w, t and Dt: 3 batches of 5 steps
-    of the 3-level KH state in the FP32 form, plain and with the committed
-    warm-start network."""
+    """adg_steps (a batch of stream launches), adg_step_graph with 4 steps
+    per captured graph (a batch of 5: one graph and one stream step) and
+    single adg_step calls give bitwise the same U_w, t and Dt: 3 batches of
+    5 steps of the 3-level KH state in the FP32 form, plain and with the
+    committed warm-start network."""
     print(f"case batched steps vs single steps KH P=3 16^2 3 levels learned={learned}", flush=True)
     import os
     levels = kh_bands(16, 3, 16)
@@ -279,6 +280,7 @@ def test_batched_steps_equal_single_steps(adg, learned):
             c.set_learned(True, tol=1e-4, max_iters=2)
         c.set_dt(0.5, 1.0, 0.0)
         ctxs.append(c)
+    ctxs[1].set_graph_unroll(4)
     for b in range(3):
         ctxs[0].steps(0.5, 1.0, 5)
         ctxs[1].step_graph(0.5, 1.0, 5)
