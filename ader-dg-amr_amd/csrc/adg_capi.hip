@@ -967,7 +967,7 @@ static int launch_mlp_warm(adg_ctx* c, long begin, long n, const int* slots) {
     const int nt = c->warm_tiles;
     const void* kern = warm_kernel<P, D, NINP, NOUTP>(nt);
     if (c->warm_per_cu[nt] <= 0) return fail(c, ADG_E_STATE, "warm-start kernel not configured");
-    const long per_wave = nt == 0 ? 8 : 16 * nt;   // 8 wavefronts per workgroup
+    const long per_wave = nt == 0 ? 8 : 16 * nt;   // elements per wavefront; 8 wavefronts per workgroup
     const long grid = persistent_grid((n + per_wave - 1) / per_wave, 8, c->warm_per_cu[nt], c->ncu);
     void* kargs[] = {&a};
     HIPCHK(c, hipLaunchKernel(kern, dim3((unsigned)grid), dim3(512), kargs, lds, c->stream));
