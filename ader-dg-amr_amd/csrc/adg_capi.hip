@@ -108,8 +108,7 @@ struct adg_ctx {
     int graph_captured = 0;    // steps in the captured graph
     int* valid_list = nullptr;
     int* cell_face[3][2] = {};
-    int* face_lr[3] = {};
-    int* face_tr[3] = {};     // face_lr with copy ghosts replaced by their sources (face flux)
+    int* face_tr[3] = {};     // face -> (left, right) slot, copy ghosts replaced by their sources
     int* pred_slots = nullptr;
     int* cg_slot = nullptr;   // copy ghosts and their sources
     int* cg_src = nullptr;
@@ -541,7 +540,6 @@ int adg_set_levels(adg_ctx* c, int nlev, const int* npatch, const int* boxes) {
     for (int d = 0; d < D; ++d) {
         UP(c->cell_face[d][0], H.cell_face[d][0]);
         UP(c->cell_face[d][1], H.cell_face[d][1]);
-        UP(c->face_lr[d], H.face_lr[d]);
         UP(c->face_tr[d], H.face_tr[d]);
     }
     UP(c->g_slot, H.g_slot);
@@ -2156,7 +2154,7 @@ int adg_tag_cells(adg_ctx* c, int lev, unsigned char* tags) {
     for (int d = 0; d < c->D; ++d) {
         a.cell_face[d][0] = c->cell_face[d][0];
         a.cell_face[d][1] = c->cell_face[d][1];
-        a.face_lr[d] = c->face_lr[d];
+        a.face_lr[d] = c->face_tr[d];   // a copy ghost holds its source's U_w after the ghost fill
     }
     a.out = d_out;
     a.NS = c->H.NS;

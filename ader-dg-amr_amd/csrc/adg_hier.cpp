@@ -23,19 +23,27 @@ void Hierarchy::take_storage(Hierarchy& o) {
         src.clear();
         src.shrink_to_fit();
     };
+    // lists the build sizes and then writes element by element (or refills
+    // in parallel) keep their contents, so that sizing them initialises
+    // nothing it is about to overwrite
+    auto keep = [](auto& dst, auto& src) {
+        dst.swap(src);
+        src.clear();
+        src.shrink_to_fit();
+    };
     take(lev, o.lev);
     take(patch, o.patch);
-    take(slot_lev, o.slot_lev);
-    take(valid_list, o.valid_list);
+    keep(slot_lev, o.slot_lev);
+    keep(valid_list, o.valid_list);
     for (int d = 0; d < 3; ++d) {
-        take(cell_face[d][0], o.cell_face[d][0]);
-        take(cell_face[d][1], o.cell_face[d][1]);
-        take(face_lr[d], o.face_lr[d]);
-        take(face_lev[d], o.face_lev[d]);
-        take(face_tr[d], o.face_tr[d]);
-        take(bcoarse[d], o.bcoarse[d]);
-        take(bfine[d], o.bfine[d]);
-        take(cidx[d], o.cidx[d]);
+        keep(cell_face[d][0], o.cell_face[d][0]);
+        keep(cell_face[d][1], o.cell_face[d][1]);
+        keep(face_lr[d], o.face_lr[d]);
+        keep(face_lev[d], o.face_lev[d]);
+        keep(face_tr[d], o.face_tr[d]);
+        keep(bcoarse[d], o.bcoarse[d]);
+        keep(bfine[d], o.bfine[d]);
+        keep(cidx[d], o.cidx[d]);
     }
     take(g_slot, o.g_slot);
     take(g_src, o.g_src);
@@ -59,9 +67,9 @@ void Hierarchy::take_storage(Hierarchy& o) {
     take(pred_off, o.pred_off);
     take(copy_ghost, o.copy_ghost);
     take(copy_src, o.copy_src);
-    take(lim_cells, o.lim_cells);
-    take(lim_nbr, o.lim_nbr);
-    take(unc_centre, o.unc_centre);
+    keep(lim_cells, o.lim_cells);
+    keep(lim_nbr, o.lim_nbr);
+    keep(unc_centre, o.unc_centre);
     // the owner grids keep their per-level storage (clear keeps the inner
     // vectors' capacity only if they stay; they are refilled by assign)
     grid_slot.swap(o.grid_slot);
@@ -94,6 +102,15 @@ struct Grid {  // owner map of one level over its domain: cell -> (patch, slot)
 
 int floordiv2(int i) { return (i >= 0) ? i / 2 : -((-i + 1) / 2); }
 
+// v sized n and every element set to val, in chunks on the host threads
+template <class T>
+void fill_par(std::vector<T>& v, size_t n, T val) {
+    v.resize(n);
+    const long nn = (long)n;
+    const int nch = (int)std::min<long>(64, std::max<long>(1, nn >> 20));
+    parallel_for(nch, [&](int k) { std::fill(v.begin() + nn * k / nch, v.begin() + nn * (k + 1) / nch, val); });
+}
+
 template <class F>
 void for_box(int D, const int* lo, const int* hi, F&& f) {
     int iv[3] = {lo[0], lo[1], D > 2 ? lo[2] : 0};
@@ -112,6 +129,16 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     if (independent && nlev != 1) throw std::runtime_error("independent patches form a single level");
     Hierarchy h;
     if (reuse) h.take_storage(*reuse);
+    for (int d = D; d < 3; ++d) {   // directions past D hold nothing
+        h.cell_face[d][0].clear();
+        h.cell_face[d][1].clear();
+        h.face_lr[d].clear();
+        h.face_lev[d].clear();
+        h.face_tr[d].clear();
+        h.bcoarse[d].clear();
+        h.bfine[d].clear();
+        h.cidx[d].clear();
+    }
     h.D = D;
     h.nlev = nlev;
     h.independent = independent != 0;
@@ -189,8 +216,8 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     if (A.hi[d] < B.lo[d] || B.hi[d] < A.lo[d]) meet = false;
                 if (meet) throw std::runtime_error("overlapping boxes on a level");
             }
-        g.slot.assign(nd, -1);
-        g.pidx.assign(nd, -1);
+        fill_par(g.slot, (size_t)nd, -1);
+        fill_par(g.pidx, (size_t)nd, -1);
         parallel_for(h.lev[l].p_end - h.lev[l].p_begin, [&](int i) {
             const int p = h.lev[l].p_begin + i;
             for_box(D, h.patch[p].lo, h.patch[p].hi, [&](const int* iv) {
@@ -232,7 +259,7 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         }
 
     // ---- per-slot level, valid list, cell faces
-    h.slot_lev.assign(h.NS, 0);
+    h.slot_lev.resize(h.NS);   // every slot set below
     for (int l = 0; l < nlev; ++l)
         std::fill(h.slot_lev.begin() + h.lev[l].slot_off, h.slot_lev.begin() + h.lev[l].slot_off + h.lev[l].nslot,
                   (uint8_t)l);
@@ -240,9 +267,10 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     const int np_all = (int)h.patch.size();
     std::vector<long> vbeg((size_t)np_all + 1, 0);
     for (int p = 0; p < np_all; ++p) vbeg[p + 1] = vbeg[p] + (long)h.patch[p].n[0] * h.patch[p].n[1] * h.patch[p].n[2];
-    h.valid_list.assign((size_t)vbeg[np_all], 0);
+    // every valid cell's entries written below, patch by patch
+    h.valid_list.resize((size_t)vbeg[np_all]);
     for (int d = 0; d < D; ++d)
-        for (int s = 0; s < 2; ++s) h.cell_face[d][s].assign((size_t)vbeg[np_all], 0);
+        for (int s = 0; s < 2; ++s) h.cell_face[d][s].resize((size_t)vbeg[np_all]);
     for (int l = 0; l < nlev; ++l) {
         h.lev[l].valid_begin = vbeg[h.lev[l].p_begin];
         h.lev[l].valid_end = vbeg[h.lev[l].p_end];
@@ -266,8 +294,8 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     for (const std::vector<int>& u : unc_p) h.uncovered.insert(h.uncovered.end(), u.begin(), u.end());
     // ---- faces
     for (int d = 0; d < D; ++d) {
-        h.face_lr[d].assign(2 * h.NF[d], 0);
-        h.face_lev[d].assign(h.NF[d], 0);
+        h.face_lr[d].resize(2 * h.NF[d]);   // every face of every patch written below
+        h.face_lev[d].resize(h.NF[d]);
         parallel_for(np_all, [&](int p) {
             const PatchLayout& P = h.patch[p];
             int hi[3] = {P.hi[0], P.hi[1], P.hi[2]};
@@ -368,9 +396,9 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     }
     // ---- C-F face masks (AmrDG_Level.tpp:240-414)
     for (int d = 0; d < D; ++d) {
-        h.bcoarse[d].assign(h.NF[d], 0);
-        h.bfine[d].assign(h.NF[d], 0);
-        h.cidx[d].assign(h.NF[d], 0);
+        fill_par(h.bcoarse[d], (size_t)h.NF[d], (int8_t)0);
+        fill_par(h.bfine[d], (size_t)h.NF[d], (int8_t)0);
+        fill_par(h.cidx[d], (size_t)h.NF[d], (int8_t)0);
     }
     parallel_for(np_all, [&](int p) {
         const PatchLayout& P = h.patch[p];
@@ -656,20 +684,23 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         }
         h.pred_off[(size_t)nlev] = (long)h.pred_slots.size();
         for (int d = 0; d < D; ++d) {
-            h.face_tr[d] = h.face_lr[d];
+            const std::vector<int>& lr = h.face_lr[d];
             std::vector<int>& ft = h.face_tr[d];
-            const long nft = (long)ft.size();
+            const long nft = (long)lr.size();
+            ft.resize((size_t)nft);   // every entry written below
             const int nch = (int)std::min<long>(256, std::max<long>(1, nft / 65536));
             parallel_for(nch, [&](int k) {
-                for (long i = nft * k / nch; i < nft * (k + 1) / nch; ++i)
-                    if (src_of[(size_t)ft[(size_t)i]] >= 0) ft[(size_t)i] = src_of[(size_t)ft[(size_t)i]];
+                for (long i = nft * k / nch; i < nft * (k + 1) / nch; ++i) {
+                    const int s = lr[(size_t)i];
+                    ft[(size_t)i] = src_of[(size_t)s] >= 0 ? src_of[(size_t)s] : s;
+                }
             });
         }
         // the limiter's cells and neighbours (FillBoundary's copies resolved)
         const LevelLayout& fl = h.lev[nlev - 1];
         const long n = fl.valid_end - fl.valid_begin;
-        h.lim_cells.assign((size_t)n, 0);
-        h.lim_nbr.assign((size_t)2 * D * n, 0);
+        h.lim_cells.resize((size_t)n);   // every entry written below
+        h.lim_nbr.resize((size_t)2 * D * n);
         const int nchunk = (int)std::min<long>(256, std::max<long>(1, n / 4096));
         parallel_for(nchunk, [&](int k) {
             for (long i = n * k / nchunk; i < n * (k + 1) / nchunk; ++i) {
@@ -689,7 +720,7 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
     // patch is the last one whose offset does not exceed it
     {
         const long nu = (long)h.uncovered.size();
-        h.unc_centre.assign((size_t)nu * 3, 0.0);
+        h.unc_centre.resize((size_t)nu * 3);   // every entry written below
         const int nchunk = (int)std::min<long>(256, std::max<long>(1, nu / 4096));
         parallel_for(nchunk, [&](int k) {
             for (long i = nu * k / nchunk; i < nu * (k + 1) / nchunk; ++i) {
@@ -699,8 +730,9 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                 const PatchLayout& pt = *(it - 1);
                 const long loc = s - pt.off;
                 const long r[3] = {loc % pt.g[0], (loc / pt.g[0]) % pt.g[1], loc / ((long)pt.g[0] * pt.g[1])};
-                for (int d = 0; d < D; ++d)
-                    h.unc_centre[(size_t)i * 3 + d] = h.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * h.lev[pt.lev].dx[d];
+                for (int d = 0; d < 3; ++d)
+                    h.unc_centre[(size_t)i * 3 + d] =
+                        d < D ? h.plo[d] + ((double)(pt.lo[d] + r[d] - 1) + 0.5) * h.lev[pt.lev].dx[d] : 0.0;
             }
         });
     }

@@ -1973,7 +1973,8 @@ __global__ void __launch_bounds__(256) k_limiter_tvb(LimArgs a) {
 //           mean density differs from a face neighbour's by more than
 //           amr_c[lev] times its own.
 // Neighbours are read at the slots beside the cell (ghost slots hold the
-// last ghost fill, which FillBoundary would reproduce).
+// last ghost fill, which FillBoundary would reproduce; a same-level copy is
+// read at its source cell, which holds the same values).
 struct TagArgs {
     const double* __restrict__ Uw;
     const int* __restrict__ valid_list;
