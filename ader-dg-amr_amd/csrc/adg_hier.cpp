@@ -572,8 +572,9 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
                     for (int t = r.t0; t < r.t1; ++t)
                         land[r.d].push_back({R.arr[(size_t)a], ids[t - r.t0], R.term[(size_t)t].coef});
             }
-        for (int d = 0; d < D; ++d)
+        parallel_for(D, [&](int d) {
             std::stable_sort(land[d].begin(), land[d].end(), [](const Land& a, const Land& b) { return a.face < b.face; });
+        });
         // the entries of face f of direction d: [first, last)
         auto land_of = [&](int d, long f) {
             auto lo = std::lower_bound(land[d].begin(), land[d].end(), f,
@@ -592,14 +593,17 @@ Hierarchy build_hierarchy(int D, const int* dom0, const double* plo, const doubl
         // the valid cells beside a face with register terms can be corrected;
         // they are visited in slot order (patch by patch, x fastest), in
         // chunks on the host threads, concatenated in chunk order.
-        std::vector<int> cand;
+        // (the slots beside the faces lie on level lc: marked, then read in order)
+        const long s0 = h.lev[lc].slot_off, nsl = h.lev[lc].nslot;
+        std::vector<uint8_t> hit((size_t)nsl, 0);
         for (int d = 0; d < D; ++d)
             for (size_t k = 0; k < land[d].size(); ++k) {
                 if (k > 0 && land[d][k].face == land[d][k - 1].face) continue;
-                for (int side = 0; side < 2; ++side) cand.push_back(h.face_lr[d][2 * land[d][k].face + side]);
+                for (int side = 0; side < 2; ++side) hit[(size_t)(h.face_lr[d][2 * land[d][k].face + side] - s0)] = 1;
             }
-        std::sort(cand.begin(), cand.end());
-        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        std::vector<int> cand;
+        for (long i = 0; i < nsl; ++i)
+            if (hit[(size_t)i]) cand.push_back((int)(s0 + i));
         struct Rows { std::vector<int> mom, slot, start, count; std::vector<double> coef; };
         const long ncand = (long)cand.size();
         const int nck = (int)std::min<long>(256, std::max<long>(1, ncand / 2048));
