@@ -225,3 +225,51 @@ def test_shock_tube_3d_regrid_conserves(adg, precision):
     assert abs(m1[0] - m0[0]) / m0[0] < 1e-13
     assert abs(m1[4] - m0[4]) / m0[4] < 1e-13
     ctx.close()
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_remake_recycled_lists_equal_fresh(adg, D):
+    """Layouts reached through a chain of remakes, whose hierarchy lists are
+    built into the storage of a layout two changes back (larger or smaller),
+    step bitwise like the same layouts built fresh from the same U_w."""
+    print(f"case remake chain with recycled lists vs fresh builds D={D}", flush=True)
+    if D == 2:
+        n0, ic, prm = 16, 1, KH
+        a = kh_bands(16, 3, 8)
+        b = [tiles(16, 8), [((8, 8), (15, 15))]]
+        c = kh_bands(16, 2, 8)
+        chain = [b, c, a, b, a]
+    else:
+        n0, ic, prm = 8, 3, SHOCK
+        a = [tiles(8, 4, 3), tiles(16, 4, 3, [4] * 3, [11] * 3)]
+        b = [tiles(8, 4, 3), tiles(16, 8, 3)]
+        c = [tiles(8, 4, 3), tiles(16, 4, 3, [2] * 3, [9] * 3), tiles(32, 4, 3, [6] * 3, [13] * 3)]
+        chain = [b, c, a, c, b]
+
+    def new_ctx(levels):
+        x = adg.Context(D, 2, 1.4, 0)
+        x.set_geometry([n0] * D, [0] * D, [1] * D)
+        x.set_levels(levels)
+        return x
+
+    ctx = new_ctx(a)
+    for l in range(len(a)):
+        ctx.project_ic(l, ic, prm)
+    ctx.average_down()
+    ctx.fill_ghosts()
+    for k, lay in enumerate(chain):
+        ctx.remake(lay)
+        ctx.average_down()
+        ctx.fill_ghosts()
+        fresh = new_ctx(lay)
+        for l in range(len(lay)):
+            fresh.upload_Uw(l, ctx.download_Uw(l))
+        for x in (ctx, fresh):
+            x.set_dt(0.5, 1.0, 0.0)
+            for _ in range(2):
+                x.step(0.5, 1.0)
+        assert ctx.get_time() == fresh.get_time(), f"remake {k}: (t, Dt)"
+        for l in range(len(lay)):
+            assert np.array_equal(ctx.download_Uw(l), fresh.download_Uw(l)), f"remake {k} level {l}"
+        fresh.close()
+    ctx.close()
