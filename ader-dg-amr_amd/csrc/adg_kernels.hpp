@@ -411,6 +411,21 @@ __device__ __forceinline__ T nanmax(T a, T b) {
     return (a > b || a != a) ? a : b;
 }
 
+// v + v(lane ^ 16) and v + v(lane ^ 32) with the row swaps of gfx950
+// (v_permlane16_swap / v_permlane32_swap: VALU, no LDS traffic).  The swap
+// of v with itself gives every lane the value of the even and of the odd
+// row of its pair (the lower and the upper half); their sum is the sum with
+// the partner lane (addition commutes, so the bits are those of
+// v + __shfl_xor(v, 16 or 32)).
+__device__ __forceinline__ float sum_xor16(float v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float sum_xor32(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -912,10 +927,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
                 // rows 16(MB2-1) + i summed over the 4 lane groups; register 0
                 // of lane group g of the last block holds row 16(MB2-1) + g
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    tacc[i] += __shfl_xor(tacc[i], 16, 64);
-                    tacc[i] += __shfl_xor(tacc[i], 32, 64);
-                }
+                for (int i = 0; i < 4; ++i) tacc[i] = sum_xor32(sum_xor16(tacc[i]));
                 const T tv = g == 0 ? tacc[0] : g == 1 ? tacc[1] : g == 2 ? tacc[2] : tacc[3];
                 acc[MB2 - 1][0] = c0[MB2 - 1][0] + tv;
             }
