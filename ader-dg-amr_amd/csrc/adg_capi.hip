@@ -99,7 +99,9 @@ struct adg_ctx {
     int pred_kind = 0;       // 0 auto, 1 VALU, 2 MFMA quad-gather, 3 MFMA node-lane
     int nl_wpb = 8;          // wavefronts per workgroup of the node-lane predictor (8 or 16)
     int ncu = 256;
-    int pred_per_cu[3] = {1, 1, 1};   // resident workgroups per CU: quad, node-lane x8, node-lane x16
+    // resident workgroups per CU of the plain [0] and learned [1] predictor
+    // kernels: quad, node-lane x8, node-lane x16
+    int pred_per_cu[2][3] = {{1, 1, 1}, {1, 1, 1}};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     double graph_cdt = -1.0, graph_T = -1.0;
@@ -605,12 +607,16 @@ static int prepare_pred(adg_ctx* c) {
         per_cu = nb > 0 ? nb : 1;
         return ADG_OK;
     };
-    int rc = setup(adg::predictor_mfma_kernel<T, P, D>(), 512, mfma_lds_bytes(c, PV_QUAD), c->pred_per_cu[0]);
-    if (rc == ADG_OK && (D == 2 || P <= 2)) {
-        rc = setup((const void*)adg::k_predictor_nl<T, P, D, 8>, 512, mfma_lds_bytes(c, PV_NL, 8), c->pred_per_cu[1]);
-        if (rc == ADG_OK)
-            rc = setup((const void*)adg::k_predictor_nl<T, P, D, 16>, 1024, mfma_lds_bytes(c, PV_NL, 16),
-                       c->pred_per_cu[2]);
+    int rc = ADG_OK;
+    for (int l = 0; l < 2 && rc == ADG_OK; ++l) {
+        int* pc = c->pred_per_cu[l];
+        rc = setup(l ? adg::predictor_mfma_kernel<T, P, D, true>() : adg::predictor_mfma_kernel<T, P, D, false>(), 512,
+                   mfma_lds_bytes(c, PV_QUAD), pc[0]);
+        if (rc == ADG_OK && (D == 2 || P <= 2)) {
+            rc = setup(adg::predictor_nl_kernel<T, P, D, 8>(l), 512, mfma_lds_bytes(c, PV_NL, 8), pc[1]);
+            if (rc == ADG_OK)
+                rc = setup(adg::predictor_nl_kernel<T, P, D, 16>(l), 1024, mfma_lds_bytes(c, PV_NL, 16), pc[2]);
+        }
     }
     return rc;
 }
@@ -879,10 +885,11 @@ static int launch_pred_mfma(adg_ctx* c, long begin, long n, int variant, bool le
     a.slots = slots;
     const int wpb = variant == PV_NL ? c->nl_wpb : 8;
     const size_t lds = mfma_lds_bytes(c, variant, wpb);
-    const void* kern = variant == PV_NL ? (wpb == 16 ? (const void*)adg::k_predictor_nl<T, P, D, 16>
-                                                     : (const void*)adg::k_predictor_nl<T, P, D, 8>)
-                                        : adg::predictor_mfma_kernel<T, P, D>();
-    const int per_cu = c->pred_per_cu[variant == PV_NL ? (wpb == 16 ? 2 : 1) : 0];
+    const void* kern = variant == PV_NL ? (wpb == 16 ? adg::predictor_nl_kernel<T, P, D, 16>(learned)
+                                                     : adg::predictor_nl_kernel<T, P, D, 8>(learned))
+                                        : (learned ? adg::predictor_mfma_kernel<T, P, D, true>()
+                                                   : adg::predictor_mfma_kernel<T, P, D, false>());
+    const int per_cu = c->pred_per_cu[learned ? 1 : 0][variant == PV_NL ? (wpb == 16 ? 2 : 1) : 0];
     if (per_cu <= 0) return fail(c, ADG_E_UNSUP, "predictor workgroup does not fit a CU");
     const long unit = variant == PV_NL ? nl_elems(c) : adg::Dims<P, D>::EPM;   // elements per wavefront
     const long grid = persistent_grid((n + unit - 1) / unit, wpb, per_cu, c->ncu);
@@ -1010,7 +1017,8 @@ int adg_autotune(adg_ctx* c, int reps, double* ms_out, int* choice_out) {
         double ms = -1.0;
         const int kind = cand[i][0], wpb = cand[i][1];
         const bool ok = (kind == 1 && c->precision == ADG_FP64) || (kind == 2 && quad_available(c)) ||
-                        (kind == 3 && nl_available(c, wpb) && c->pred_per_cu[wpb == 16 ? 2 : 1] > 0);
+                        (kind == 3 && nl_available(c, wpb) && c->pred_per_cu[0][wpb == 16 ? 2 : 1] > 0 &&
+                         c->pred_per_cu[1][wpb == 16 ? 2 : 1] > 0);
         if (ok) {
             c->pred_kind = kind;
             c->nl_wpb = wpb;

@@ -469,7 +469,10 @@ __device__ __forceinline__ bool quad_kstep_live(int nb, int r) {
     else return 16 * nb + acc_row<T>(0, r) < Mst;
 }
 
-template <typename T, int P, int D>
+// LEARN = false compiles the learned start out (a.warm is ignored): the plain
+// form then holds fewer registers (no learned-start arrays live across the
+// Picard loop).  The plain arithmetic is the same in both instantiations.
+template <typename T, int P, int D, bool LEARN>
 __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
     using Dm = Dims<P, D>;
     using M = MF<T>;
@@ -589,7 +592,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
         int nplain = a.iters;
         bool keep_learned = false;   // this column's element passed the check
         T hl[KSH];
-        if (a.warm) {
+        if (LEARN && a.warm) {
             // learned start H_0 = A0 U_w + Delta and Picard iterations until
             // the update of an element passes the tolerance; the decision is
             // per element (its data only), elements that do not pass are
@@ -730,7 +733,7 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
                 }
             }
     }
-    if (a.warm) {
+    if (LEARN && a.warm) {
         __syncthreads();
         if (threadIdx.x == 0 && wg_stats[0]) {
             atomicAdd(a.stats, wg_stats[0]);
@@ -742,18 +745,18 @@ __device__ __forceinline__ void predictor_mfma_body(const PredMArgs<T>& a) {
 // 2D: capped at 128 VGPRs so that two 512-thread workgroups (4 waves per
 // SIMD) share a CU; the 3D tiles need more registers than that cap allows
 // without spilling and run uncapped.
-template <typename T, int P, int D>
+template <typename T, int P, int D, bool LEARN>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_predictor_mfma(PredMArgs<T> a) {
-    predictor_mfma_body<T, P, D>(a);
+    predictor_mfma_body<T, P, D, LEARN>(a);
 }
-template <typename T, int P, int D>
+template <typename T, int P, int D, bool LEARN>
 __global__ void __launch_bounds__(512) k_predictor_mfma_wide(PredMArgs<T> a) {
-    predictor_mfma_body<T, P, D>(a);
+    predictor_mfma_body<T, P, D, LEARN>(a);
 }
-template <typename T, int P, int D>
+template <typename T, int P, int D, bool LEARN>
 inline const void* predictor_mfma_kernel() {
-    if constexpr (D == 2) return (const void*)k_predictor_mfma<T, P, D>;
-    else return (const void*)k_predictor_mfma_wide<T, P, D>;
+    if constexpr (D == 2) return (const void*)k_predictor_mfma<T, P, D, LEARN>;
+    else return (const void*)k_predictor_mfma_wide<T, P, D, LEARN>;
 }
 
 // ---------------------------------------------------------------------------
@@ -775,7 +778,7 @@ inline const void* predictor_mfma_kernel() {
 // 16*blk + 4r + g (the fragment rows of A0, B, S and T are permuted for that),
 // which is the operand layout of K step 4*blk + r of the next nodal product.
 // Final pass: Vol = sum_d S_d g_d and the face traces Tr = Vbd H_w.
-template <typename T, int P, int D>
+template <typename T, int P, int D, bool LEARN>
 __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     using Dm = Dims<P, D>;
     using M = MF<T>;
@@ -938,7 +941,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
         int nplain = a.iters;
         bool keep_learned = false;   // this column's element passed the check
         T hl[KSH];
-        if (a.warm) {
+        if (LEARN && a.warm) {
             // learned start H_0 = A0 U_w + Delta, then Picard iterations until
             // the update of an element passes the tolerance.  The decision is
             // per element (it depends on the element's data only, so a ghost
@@ -1073,7 +1076,7 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
                 }
             }
     }
-    if (a.warm) {
+    if (LEARN && a.warm) {
         __syncthreads();
         if (threadIdx.x == 0 && wg_stats[0]) {
             atomicAdd(a.stats, wg_stats[0]);
@@ -1082,10 +1085,14 @@ __device__ __forceinline__ void predictor_nl_body(const PredMArgs<T>& a) {
     }
 }
 
-template <typename T, int P, int D, int WPB>
+template <typename T, int P, int D, int WPB, bool LEARN>
 __global__ void __launch_bounds__(64 * WPB) k_predictor_nl(PredMArgs<T> a) {
     // 3D p = 3 fragments exceed the LDS: no body
-    if constexpr (D == 2 || P <= 2) predictor_nl_body<T, P, D>(a);
+    if constexpr (D == 2 || P <= 2) predictor_nl_body<T, P, D, LEARN>(a);
+}
+template <typename T, int P, int D, int WPB>
+inline const void* predictor_nl_kernel(bool learn) {
+    return learn ? (const void*)k_predictor_nl<T, P, D, WPB, true> : (const void*)k_predictor_nl<T, P, D, WPB, false>;
 }
 
 // ---------------------------------------------------------------------------

@@ -593,8 +593,8 @@ def main():
     achieved = f_pred * k["elements"] / (k["pred_ms"] * 1e-3)
     peak = PEAK_FP64 if args.form == "fp64" else PEAK_FP32
     roofline = {"bound": "mfma",
-                "kernel": f"k_predictor_nl<{'double' if args.form == 'fp64' else 'float'},3,2,W> "
-                          f"(P Picard iterations; W={k['autotune'][1]})",
+                "kernel": f"k_predictor_nl<{'double' if args.form == 'fp64' else 'float'},3,2,W,false> "
+                          f"(plain form, P Picard iterations; W={k['autotune'][1]})",
                 "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": PRED_TRAFFIC[args.form], "avg_launch_ms": k["pred_ms"],
                 "flops_per_launch": f_pred * k["elements"],

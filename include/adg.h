@@ -182,8 +182,10 @@ int adg_predictor(adg_ctx* ctx, int lev, int mode, int* n_fallback);
 int adg_set_predictor_kernel(adg_ctx* ctx, int kind);
 /* Autotuner: times the predictor launch shapes that fit this context and
  * hierarchy ({VALU} for FP64, {quad-gather, 8 waves}, {node-lane, 8 waves},
- * {node-lane, 16 waves}) over `reps` launches each on the current state and
- * keeps the fastest; with a network loaded it also times the warm-start
+ * {node-lane, 16 waves}) over `reps` plain launches each (the plain form's
+ * kernels; the learned form runs the same shapes with the learned start
+ * compiled in) on the current state and keeps the fastest; with a network
+ * loaded it also times the warm-start
  * kernel with one and two 16-element tiles and one 8-element tile per
  * wavefront.  ms_out[7] (nullable) gets the average launch time of each
  * candidate (-1 where it does not apply), choice_out[3] (nullable) the
