@@ -46,7 +46,7 @@ KH_SEED = 20240611
 # hierarchy (the kernel measurement below), from rocprofv3 --pmc FETCH_SIZE
 # and --pmc WRITE_SIZE scaled by the counters' reading of a plain 256 MiB
 # device copy (x2.0 reads, x1.0 writes)
-PRED_TRAFFIC = {"fp64": 147.0e6, "fp32": 49.6e6, "learned": 49.6e6}
+PRED_TRAFFIC = {"fp64": 98.1e6, "fp32": 56.8e6, "learned": 56.8e6}
 SHOCK = [0.4, 1.0, 1.0, 0.125, 0.1]
 
 
@@ -275,6 +275,9 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the predictor measurement on the band hierarchy (the profiled command)")
     ap.add_argument("--shock-only", action="store_true", help="only the 3D shock-tube measurement")
+    ap.add_argument("--pred-shape", default=None, metavar="KIND,WAVES",
+                    help="kernel measurement: pin the predictor launch shape (e.g. 3,16) instead of autotuning; "
+                         "counter runs use it, since the tuner's timings change under counter collection")
     ap.add_argument("--no-graph", action="store_true",
                     help="step the jobs one step per call (the profiled commands)")
     ap.add_argument("--stream", action="store_true",
@@ -338,7 +341,12 @@ def main():
         if form == "learned":
             ctx.mlp_set_params(weights)
             ctx.set_learned(True, tol=args.tol, max_iters=args.max_iters)
-        tune = ctx.autotune(5)[1]
+        if args.pred_shape:
+            tune = [int(x) for x in args.pred_shape.split(",")]
+            ctx.set_predictor_kernel(tune[0])
+            ctx.set_predictor_waves(tune[1])
+        else:
+            tune = ctx.autotune(5)[1]
         ctx.step_graph(c_dt, 1e9, 10)
         reps = 50
         out = {"autotune": tune, "elements": nvalid_of(levels, D)}
