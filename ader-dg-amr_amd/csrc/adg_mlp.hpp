@@ -391,7 +391,10 @@ __device__ __forceinline__ void stage_to_lds_dma(unsigned char* dst, const uint4
 // (twice the wavefronts, each with half the activation work: the kernel is
 // latency-bound at the sizes of a step, not MFMA-bound).  Layers 1 and 2 keep
 // twice the GELU (their consumers' weights are packed at half scale), and
-// every accumulator starts at its bias.
+// every accumulator starts at its bias.  The hidden layers have no scheduling
+// barrier between their 16-row blocks, so one block's MFMAs overlap the
+// previous block's GELU (bitwise the same output, 2-6 % shorter launches than
+// with a barrier per block); the output layer keeps one per block.
 template <int P, int D, int NINP, int NOUTP, int NT, int ELT>
 __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
     static_assert(ELT == 16 || ELT == 8, "8 or 16 elements per tile");
@@ -506,7 +509,6 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt)
                 h1[tt][u] = pack_bf16_hw(gelu_w(acc[tt][0]), gelu_w(acc[tt][1]), gelu_w(acc[tt][2]), gelu_w(acc[tt][3]));
-            __builtin_amdgcn_sched_barrier(0);   // keeps fragment reads next to their MFMAs
         }
         bf16x4 h2[NT][KS2];
 #pragma unroll
@@ -524,7 +526,6 @@ __device__ __forceinline__ void mlp_warm_body(const WarmArgs& a) {
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt)
                 h2[tt][u] = pack_bf16_hw(gelu_w(acc[tt][0]), gelu_w(acc[tt][1]), gelu_w(acc[tt][2]), gelu_w(acc[tt][3]));
-            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int u = 0; u < UT3; ++u) {
